@@ -1,0 +1,189 @@
+/*
+ * qumasun_gpu.h — C ABI of the MI355X Kohn-Sham H·psi + block-LOBPCG engine.
+ *
+ * Plain C: opaque handles, raw pointers and sizes, int status codes (QG_OK = 0), and
+ * qg_last_error() for the message of the last failure on the calling thread.  No C++
+ * exceptions cross this boundary and no torch types appear in it.
+ *
+ * Conventions (same as the reference's real-space grid code):
+ *   - grid index i = ix + Nx*(iy + Ny*iz), x fastest (GridDifference8th.h:137-138);
+ *   - a block of nbands wavefunctions is band-major: band b starts at psi + b*ld
+ *     (lobpcg_d_multi_mpi.h:898-913), ld >= Nx*Ny*Nz;
+ *   - grid point positions are i*h with no half-cell offset
+ *     (PseudoPotOperator_mpi_v2.h:144-150);
+ *   - every psi/hpsi/X/P/work pointer is DEVICE memory; host arrays are named *_host.
+ *   - `stream` is a hipStream_t passed as void*; NULL is HIP's null stream.  Calls on one
+ *     handle are not thread-safe (neither is the reference's integrator, which holds
+ *     mutable per-call state).
+ */
+#ifndef QUMASUN_GPU_H
+#define QUMASUN_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QG_OK 0
+#define QG_ERR_ARG 1      /* invalid argument (shape, null pointer, unsupported option) */
+#define QG_ERR_HIP 2      /* a HIP runtime call failed */
+#define QG_ERR_LIB 3      /* rocBLAS / rocSOLVER failure */
+#define QG_ERR_NUMERIC 4  /* numerical breakdown (e.g. Cholesky of a non-SPD Gram) */
+
+/* arithmetic type of a psi block */
+#define QG_F64 0
+#define QG_F32 1
+
+const char* qg_last_error(void);
+int qg_version(void);
+/* Number of visible GPUs (0 on a host without one); never faults. */
+int qg_device_count(void);
+
+/* ------------------------------------------------------------------ grid ------ */
+typedef struct qg_grid qg_grid;
+
+/* Periodic Nx*Ny*Nz grid with spacings dx,dy,dz (the reference's GridRange + m_dx/dy/dz,
+ * GridRange.h:14-46, qumasun_mpi.constructor.h:35). */
+int qg_grid_create(int nx, int ny, int nz, double dx, double dy, double dz, qg_grid** out);
+int qg_grid_destroy(qg_grid* g);
+
+/* out[b] = (accumulate ? out[b] : 0) + coef * Lap8 psi[b] for b < nbands, periodic.
+ * Replaces Laplasian8th<T>(GridRange, out, p, coef, dx, dy, dz) (GridDifference8th.h:177-185),
+ * which always accumulates, and Laplasian8th_ddm (GridDifference8th.h:320-344) with
+ * split_dimension 0.  dtype is QG_F64 or QG_F32 (both psi and out). */
+int qg_laplacian8_apply(const qg_grid* g, int dtype, int nbands,
+                        const void* psi, int64_t ld_psi, void* out, int64_t ld_out,
+                        double coef, int accumulate, void* stream);
+
+/* ------------------------------------------------------- nonlocal projectors ---- */
+/* One Kleinman-Bylander projector channel of a species: beta_lm(r) = p(r) Y_lm(r/|r|),
+ * p(r) = 0 for r^2 > cutoff^2 (PseudoPotOperator_mpi_v2.h:156-166).
+ * radial_kind QG_RADIAL_GAUSSIAN: p(r) = amplitude * exp(-r^2/(2 sigma^2)).
+ * radial_kind QG_RADIAL_TABLE:  p(r) from a log-grid table (xi = log r, values on
+ *   xi_min + k*xi_delta, k < n_table), interpolated as RadialGrid2::GetValueByXi
+ *   (RadialGrid2.h:81-136); r^2 < r_min^2 uses table[0]
+ *   (PseudoPotOperator_mpi_v2.h:154-158).  The table holds the j-averaged projector. */
+#define QG_RADIAL_GAUSSIAN 0
+#define QG_RADIAL_TABLE 1
+typedef struct qg_proj_spec {
+    int l;               /* angular momentum, 0 or 1 */
+    double energy;       /* KB energy E_k (Ha), PseudoPot_MBK::projector_energy_up */
+    double cutoff;       /* PseudoPot_MBK::cutoff_r[l] (bohr) */
+    int radial_kind;
+    double sigma;        /* gaussian width (bohr) */
+    double amplitude;    /* gaussian prefactor */
+    int n_table;         /* table length */
+    double xi_min, xi_delta, r_min;
+    const double* table; /* host pointer, n_table values */
+} qg_proj_spec;
+
+typedef struct qg_nonlocal qg_nonlocal;
+
+/* Build the projector geometry for natoms nuclei at xyz_host[3*a..3*a+2] (bohr), each of
+ * species species_host[a]; species s owns nproj_host[s] consecutive entries of specs
+ * (in species order).  Replaces PseudoPotIntegrator_mpi::UpdatePosition
+ * (PseudoPotOperator_mpi_v2.h:231-248) for one process holding the whole grid:
+ * GridForSphere boxes (GridSubgrid.h:483-492), their periodic overlap blocks
+ * (GridSubgrid.h:381-430) and the projector/Ylm values on them. */
+int qg_nonlocal_create(const qg_grid* g, int natoms, const double* xyz_host,
+                       const int* species_host, int nspecies, const int* nproj_host,
+                       const qg_proj_spec* specs, qg_nonlocal** out);
+int qg_nonlocal_destroy(qg_nonlocal* nl);
+/* Number of (atom, projector, m) coefficients per band (sum of TotalProjectorLM). */
+int qg_nonlocal_num_coef(const qg_nonlocal* nl);
+
+/* coef[b*ncoef + j] = dV * sum_i beta_j(i) psi_b(i)  (FP64 device array, ncoef per band).
+ * Replaces mInnerNonlocalPsi_v2 (PseudoPotOperator_mpi_v2.h:336-463) for a band block. */
+int qg_nonlocal_project(qg_nonlocal* nl, int dtype, int nbands, const void* psi, int64_t ld,
+                        double* coef_dev, void* stream);
+/* hpsi[b] (+)= sum_j E_j coef_bj beta_j — ProjectionPP (PseudoPotOperator_mpi_v2.h:465-570)
+ * and its band-block form ProjectionPP_bundle (:789-902), band-major layout. */
+int qg_nonlocal_apply(qg_nonlocal* nl, int dtype, int nbands, const void* psi, int64_t ld,
+                      void* hpsi, int64_t ld_out, int accumulate, void* stream);
+
+/* --------------------------------------------------------------- Hamiltonian ---- */
+typedef struct qg_ham qg_ham;
+
+/* H = V (local, V_host Nx*Ny*Nz values, copied) + V_NL (nl, may be NULL) - 1/2 Lap8.
+ * The operator of QUMASUN_MPI::mHamiltonianMatrix_ddm (qumasun_mpi.matrix.h:21-43). */
+int qg_hamiltonian_create(const qg_grid* g, const double* V_host, qg_nonlocal* nl,
+                          qg_ham** out);
+int qg_hamiltonian_destroy(qg_ham* h);
+/* Replace the local potential (SCF update of ml_Vtot). */
+int qg_hamiltonian_set_potential(qg_ham* h, const double* V_host);
+/* hpsi[b] = H psi[b] for b < nbands in one fused pass (V*psi + nonlocal - 1/2 Lap8 psi).
+ * Composition order of mHamiltonianMatrix_ddm (zero, V*psi, +ProjectionPP, +Laplacian). */
+int qg_hamiltonian_apply(qg_ham* h, int dtype, int nbands, const void* psi, int64_t ld,
+                         void* hpsi, int64_t ld_out, void* stream);
+/* Sector timers of qg_hamiltonian_apply on this handle (hipEvents recorded on the apply's
+ * stream), the counterpart of the reference's StopWatch sectors 13 (PP nonlocal) and
+ * 11/12 (kinetic, V) (qumasun_mpi.matrix.h:30-41): t_project_ms is the projector-dot
+ * kernel, t_fused_ms the fused stencil + V + projector-apply kernel of the last apply.
+ * last_times waits for the recorded events. */
+int qg_hamiltonian_set_timing(qg_ham* h, int enable);
+int qg_hamiltonian_last_times(qg_ham* h, double* t_project_ms, double* t_fused_ms);
+
+/* ------------------------------------------------------------- block LOBPCG ---- */
+/* precision modes */
+#define QG_PREC_F64 0     /* everything FP64 (the reference's arithmetic) */
+#define QG_PREC_MIXED 1   /* FP32 H·psi, BF16-in/FP32-acc MFMA Gram, FP64 small solve */
+#define QG_PREC_F32 2     /* FP32 H·psi and FP32 Gram, FP64 small solve */
+
+/* preconditioners (the reference uses the raw residual, lobpcg_d_multi_mpi.h:982-990) */
+#define QG_PRECOND_NONE 0
+#define QG_PRECOND_KINETIC 1   /* (T + shift)^-1 applied exactly in Fourier space */
+
+typedef struct qg_lobpcg_opts {
+    int precision;           /* QG_PREC_* */
+    int max_iter;            /* iter_max of Eigen_d_multi_mpi */
+    double tol;              /* stop when max_k ||r_k|| (dVol-normalised) <= tol; 0: run max_iter */
+    int refresh_period;      /* full Gram recompute every n iterations (S_MAT_REST_LIMIT = 10) */
+    double rollback_limit;   /* |1-||x'||^2| limit (LIMIT_D_NORM_X_FOR_RESET_MATRIX = 1e-3) */
+    int precond;             /* QG_PRECOND_* */
+    double precond_shift;    /* shift of the kinetic preconditioner (Ha) */
+    int verbose;
+} qg_lobpcg_opts;
+
+typedef struct qg_lobpcg_stats {
+    int iterations;
+    int rollbacks;
+    int hpsi_calls;          /* band applications of H */
+    double max_residual;     /* final max_k ||r_k|| (dVol-normalised) */
+    double t_hpsi, t_gram, t_update, t_eig, t_other, t_precond;   /* seconds per sector */
+} qg_lobpcg_stats;
+
+void qg_lobpcg_default_opts(qg_lobpcg_opts* o);
+
+/* Block LOBPCG for the nbands lowest eigenpairs of H; the device-resident equivalent of
+ * LOBPCG::Eigen_d_multi_mpi (lobpcg_d_multi_mpi.h:882-1572) on one process.
+ *   X_dev: nbands x Ng FP64 band-major, in: start vectors (normalised with dVol), out:
+ *          Ritz vectors normalised with dVol (the reference's eigen_vectors).
+ *   P_dev: nbands x Ng FP64, the persistent conjugate block (keep_P); zeroed when
+ *          scf_step == 0 (lobpcg_d_multi_mpi.h:992-997).
+ *   eigvals_host: nbands Ritz values, ascending.
+ * Work memory is owned by the handle (the reference's `work`, 5*Ng*N doubles). */
+int qg_lobpcg_d_multi(qg_ham* h, const qg_lobpcg_opts* opts, int nbands,
+                      double* eigvals_host, double* X_dev, double* P_dev, int scf_step,
+                      qg_lobpcg_stats* stats);
+
+/* ------------------------------------------------------- synthetic workloads ---- */
+/* Host-only helpers that build the seeded diamond cells used to exercise this path:
+ * ncell^3 diamond supercell of the 8-atom cubic cell with lattice a (bohr) on an
+ * (n*ncell)^3 grid.  Atom fractional sites follow input/diamond8.txt:28-35. */
+int qg_synth_diamond_atoms(int ncell, double a, double* xyz_host /* 24*ncell^3 */);
+/* V(r) = sum_atoms -zion*erf(|r-R|_min/rc)/|r-R|_min + U(-noise, noise), minimum image,
+ * noise from std::mt19937(seed). */
+int qg_synth_vloc(int nx, int ny, int nz, double lx, double ly, double lz, int natoms,
+                  const double* xyz_host, double zion, double rc, double noise,
+                  uint32_t seed, double* V_host);
+/* Random start block: U[0,1) from std::mt19937(seed), each band normalised with dVol
+ * (qumasun_mpi.initialize.h:199-214). */
+int qg_synth_random_states(int64_t ng, int nbands, double dvol, uint32_t seed,
+                           double* X_host);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,180 @@
+"""ctypes binding of oracle/liboracle.so (and oracle/_ref/libqref.so when it was built)."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF = os.path.join(HERE, "_ref", "libqref.so")
+REF_SRC = "/root/reference/source"
+
+c_int, c_double, c_void_p, c_int64, c_uint32 = (ctypes.c_int, ctypes.c_double, ctypes.c_void_p,
+                                                 ctypes.c_int64, ctypes.c_uint32)
+P = ctypes.POINTER
+DP = P(c_double)
+
+__all__ = ["build", "olib", "ref_lib", "OracleNL", "laplacian8", "hamiltonian_block",
+           "lobpcg_hamiltonian", "lobpcg_dense", "dsygvx", "random_states", "radial_by_xi"]
+
+
+def build(with_ref=True):
+    """make liboracle.so (and _ref/ when the reference sources are present)."""
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+    if with_ref and os.path.isdir(REF_SRC):
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+_o = None
+_r = None
+
+
+def olib():
+    global _o
+    if _o is None:
+        if not os.path.exists(LIB):
+            build(with_ref=False)
+        l = ctypes.CDLL(LIB)
+        l.or_laplacian8.argtypes = [c_int] * 3 + [c_double] * 4 + [DP, DP]
+        l.or_radial_by_xi.argtypes = [c_double, DP, c_int, c_double, c_double]
+        l.or_radial_by_xi.restype = c_double
+        l.or_nl_create.restype = c_void_p
+        l.or_nl_create.argtypes = [c_int] * 3 + [c_double] * 3 + [c_int, DP, P(c_int), c_int, P(c_int), c_void_p]
+        l.or_nl_destroy.argtypes = [c_void_p]
+        l.or_nl_num_coef.argtypes = [c_void_p]
+        l.or_nl_block_points.argtypes = [c_void_p, c_int, c_int]
+        l.or_nl_dense_beta.argtypes = [c_void_p, c_int, DP]
+        l.or_nl_energies.argtypes = [c_void_p, DP]
+        l.or_nl_project.argtypes = [c_void_p, DP, DP]
+        l.or_nl_apply_coef.argtypes = [c_void_p, DP, DP]
+        l.or_nl_apply.argtypes = [c_void_p, DP, DP]
+        l.or_hamiltonian.argtypes = [c_int] * 3 + [c_double] * 3 + [DP, c_void_p, DP, DP]
+        l.or_hamiltonian_block.argtypes = [c_int] * 3 + [c_double] * 3 + [DP, c_void_p, c_int, DP, DP]
+        l.or_random_states.argtypes = [c_int64, c_int, c_double, c_uint32, DP, DP]
+        l.or_lobpcg_hamiltonian.argtypes = ([c_int] * 3 + [c_double] * 3 + [DP, c_void_p, c_int, c_int, c_double,
+                                            DP, DP, DP, c_int, P(c_int), P(c_int), DP])
+        l.or_lobpcg_dense.argtypes = [c_int64, DP, c_int, c_int, c_double, DP, DP, DP, c_int, P(c_int), P(c_int), DP]
+        l.or_dsygvx.argtypes = [c_int, DP, DP, DP, DP]
+        l.or_set_threads.argtypes = [c_int]
+        _o = l
+    return _o
+
+
+def ref_lib():
+    """The reference-header harness, or None when it was not built (no reference sources)."""
+    global _r
+    if _r is None and os.path.exists(REF):
+        l = ctypes.CDLL(REF)
+        l.ref_laplacian8.argtypes = [c_int] * 3 + [c_double] * 4 + [DP, DP]
+        l.ref_laplacian8_f32.argtypes = [c_int] * 3 + [c_double] * 4 + [P(ctypes.c_float), P(ctypes.c_float)]
+        _r = l
+    return _r
+
+
+def _d(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(DP)
+
+
+def laplacian8(n, h, coef, p, out):
+    olib().or_laplacian8(n[0], n[1], n[2], h[0], h[1], h[2], coef, _d(p), _d(out))
+    return out
+
+
+def radial_by_xi(xi, values, xi_min, xi_delta):
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    return olib().or_radial_by_xi(xi, _d(v), len(v), xi_min, xi_delta)
+
+
+class OracleNL:
+    """Oracle nonlocal operator (PseudoPotIntegrator_mpi restated, one process)."""
+
+    def __init__(self, n, h, positions, species, species_projectors):
+        import qumasun_amd._native as N  # ProjSpec layout only (a plain C struct)
+        from qumasun_amd import _specs_array
+        pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1)
+        sp = np.ascontiguousarray(species, dtype=np.int32)
+        nproj = np.array([len(s) for s in species_projectors], dtype=np.int32)
+        self._specs, self._keep = _specs_array(species_projectors)
+        self.n, self.h = n, h
+        self._h = olib().or_nl_create(n[0], n[1], n[2], h[0], h[1], h[2], len(sp), _d(pos),
+                                      sp.ctypes.data_as(P(c_int)), len(nproj),
+                                      nproj.ctypes.data_as(P(c_int)), ctypes.cast(self._specs, c_void_p))
+        self.ncoef = olib().or_nl_num_coef(self._h)
+
+    def project(self, psi):
+        c = np.zeros(self.ncoef)
+        olib().or_nl_project(self._h, _d(np.ascontiguousarray(psi)), _d(c))
+        return c
+
+    def apply(self, psi, hp):
+        olib().or_nl_apply(self._h, _d(np.ascontiguousarray(psi)), _d(hp))
+        return hp
+
+    def dense_beta(self):
+        ng = self.n[0] * self.n[1] * self.n[2]
+        B = np.zeros((self.ncoef, ng))
+        for j in range(self.ncoef):
+            col = np.zeros(ng)
+            olib().or_nl_dense_beta(self._h, j, _d(col))
+            B[j] = col
+        return B
+
+    def energies(self):
+        e = np.zeros(self.ncoef)
+        olib().or_nl_energies(self._h, _d(e))
+        return e
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            olib().or_nl_destroy(self._h)
+            self._h = None
+
+
+def hamiltonian_block(n, h, V, nl, psi):
+    psi = np.ascontiguousarray(psi, dtype=np.float64)
+    nb = psi.size // (n[0] * n[1] * n[2])
+    out = np.zeros_like(psi)
+    olib().or_hamiltonian_block(n[0], n[1], n[2], h[0], h[1], h[2], _d(np.ascontiguousarray(V)),
+                                nl._h if nl else None, nb, _d(psi), _d(out))
+    return out
+
+
+def lobpcg_hamiltonian(n, h, V, nl, X, P_, iter_max, tol=0.0, scf_step=0):
+    nb = X.shape[0]
+    eig = np.zeros(nb)
+    it, rb, mr = c_int(), c_int(), c_double()
+    olib().or_lobpcg_hamiltonian(n[0], n[1], n[2], h[0], h[1], h[2], _d(np.ascontiguousarray(V)),
+                                 nl._h if nl else None, nb, iter_max, tol, _d(eig), _d(X), _d(P_),
+                                 scf_step, ctypes.byref(it), ctypes.byref(rb), ctypes.byref(mr))
+    return eig, dict(iterations=it.value, rollbacks=rb.value, max_residual=mr.value)
+
+
+def lobpcg_dense(A, X, P_, iter_max, tol=0.0, scf_step=0):
+    A = np.asfortranarray(A, dtype=np.float64)
+    n = A.shape[0]
+    nb = X.shape[0]
+    eig = np.zeros(nb)
+    it, rb, mr = c_int(), c_int(), c_double()
+    olib().or_lobpcg_dense(n, A.ctypes.data_as(DP), nb, iter_max, tol, _d(eig), _d(X), _d(P_), scf_step,
+                           ctypes.byref(it), ctypes.byref(rb), ctypes.byref(mr))
+    return eig, dict(iterations=it.value, rollbacks=rb.value, max_residual=mr.value)
+
+
+def dsygvx(A, B):
+    """lapack_DSYGVX(A, B) on upper triangles; returns (w, Z) with Z column-major columns."""
+    n = A.shape[0]
+    a = np.asfortranarray(A, dtype=np.float64).copy(order="F")
+    b = np.asfortranarray(B, dtype=np.float64).copy(order="F")
+    w = np.zeros(n)
+    z = np.zeros((n, n), order="F")
+    olib().or_dsygvx(n, a.ctypes.data_as(DP), b.ctypes.data_as(DP), _d(w), z.ctypes.data_as(DP))
+    return w, z
+
+
+def random_states(ng, nbands, dvol, seed=123456789):
+    X = np.zeros(ng * nbands)
+    g = np.zeros(nbands)
+    olib().or_random_states(ng, nbands, dvol, seed, _d(X), _d(g))
+    return X.reshape(nbands, ng), g

@@ -1,0 +1,115 @@
+"""ctypes binding of libqumasun_gpu.so (the C ABI declared in include/qumasun_gpu.h).
+
+The library is built in-tree by qumasun_amd/build.py.  Loading fails loudly when it is
+missing: there is no CPU fallback for any operation of this package.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libqumasun_gpu.so")
+
+QG_OK = 0
+QG_F64 = 0
+QG_F32 = 1
+QG_RADIAL_GAUSSIAN = 0
+QG_RADIAL_TABLE = 1
+QG_PREC_F64 = 0
+QG_PREC_MIXED = 1
+QG_PREC_F32 = 2
+QG_PRECOND_NONE = 0
+QG_PRECOND_KINETIC = 1
+
+c_int, c_double, c_void_p, c_int64, c_uint32 = (ctypes.c_int, ctypes.c_double, ctypes.c_void_p,
+                                                 ctypes.c_int64, ctypes.c_uint32)
+P = ctypes.POINTER
+
+
+class ProjSpec(ctypes.Structure):
+    """qg_proj_spec"""
+    _fields_ = [("l", c_int), ("energy", c_double), ("cutoff", c_double), ("radial_kind", c_int),
+                ("sigma", c_double), ("amplitude", c_double), ("n_table", c_int),
+                ("xi_min", c_double), ("xi_delta", c_double), ("r_min", c_double),
+                ("table", P(c_double))]
+
+
+class LobpcgOpts(ctypes.Structure):
+    """qg_lobpcg_opts"""
+    _fields_ = [("precision", c_int), ("max_iter", c_int), ("tol", c_double),
+                ("refresh_period", c_int), ("rollback_limit", c_double), ("precond", c_int),
+                ("precond_shift", c_double), ("verbose", c_int)]
+
+
+class LobpcgStats(ctypes.Structure):
+    """qg_lobpcg_stats"""
+    _fields_ = [("iterations", c_int), ("rollbacks", c_int), ("hpsi_calls", c_int),
+                ("max_residual", c_double), ("t_hpsi", c_double), ("t_gram", c_double),
+                ("t_update", c_double), ("t_eig", c_double), ("t_other", c_double),
+                ("t_precond", c_double)]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "qg_last_error": (ctypes.c_char_p, []),
+    "qg_version": (c_int, []),
+    "qg_device_count": (c_int, []),
+    "qg_grid_create": (c_int, [c_int, c_int, c_int, c_double, c_double, c_double, P(c_void_p)]),
+    "qg_grid_destroy": (c_int, [c_void_p]),
+    "qg_laplacian8_apply": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, c_void_p, c_int64,
+                                    c_double, c_int, c_void_p]),
+    "qg_nonlocal_create": (c_int, [c_void_p, c_int, P(c_double), P(c_int), c_int, P(c_int),
+                                   P(ProjSpec), P(c_void_p)]),
+    "qg_nonlocal_destroy": (c_int, [c_void_p]),
+    "qg_nonlocal_num_coef": (c_int, [c_void_p]),
+    "qg_nonlocal_project": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, c_void_p, c_void_p]),
+    "qg_nonlocal_apply": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, c_void_p, c_int64,
+                                  c_int, c_void_p]),
+    "qg_hamiltonian_create": (c_int, [c_void_p, P(c_double), c_void_p, P(c_void_p)]),
+    "qg_hamiltonian_destroy": (c_int, [c_void_p]),
+    "qg_hamiltonian_set_potential": (c_int, [c_void_p, P(c_double)]),
+    "qg_hamiltonian_apply": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, c_void_p, c_int64,
+                                     c_void_p]),
+    "qg_hamiltonian_set_timing": (c_int, [c_void_p, c_int]),
+    "qg_hamiltonian_last_times": (c_int, [c_void_p, P(c_double), P(c_double)]),
+    "qg_lobpcg_default_opts": (None, [P(LobpcgOpts)]),
+    "qg_lobpcg_d_multi": (c_int, [c_void_p, P(LobpcgOpts), c_int, P(c_double), c_void_p, c_void_p,
+                                  c_int, P(LobpcgStats)]),
+    "qg_synth_diamond_atoms": (c_int, [c_int, c_double, P(c_double)]),
+    "qg_synth_vloc": (c_int, [c_int, c_int, c_int, c_double, c_double, c_double, c_int, P(c_double),
+                              c_double, c_double, c_double, c_uint32, P(c_double)]),
+    "qg_synth_random_states": (c_int, [c_int64, c_int, c_double, c_uint32, P(c_double)]),
+}
+
+_lib = None
+
+
+class QGError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libqumasun_gpu.so (once).  Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise QGError(f"{LIB_PATH} is missing: build it with `python -m qumasun_amd.build` "
+                          "(there is no CPU fallback)")
+        l = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(l, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = l
+    return _lib
+
+
+def check(rc):
+    if rc != QG_OK:
+        msg = lib().qg_last_error().decode(errors="replace")
+        raise QGError(f"qumasun_gpu error {rc}: {msg}")
+    return rc
+
+
+def dptr(a):
+    """ctypes double* of a C-contiguous float64 numpy array."""
+    return a.ctypes.data_as(P(c_double))

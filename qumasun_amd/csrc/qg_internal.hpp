@@ -1,0 +1,164 @@
+// qg_internal.hpp — shared host/device declarations of the MI355X engine (not part of the
+// public C ABI in include/qumasun_gpu.h).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+#include "../../include/qumasun_gpu.h"
+
+namespace qg {
+
+// ---------------------------------------------------------------- error plumbing ----
+void set_error(const std::string& msg);
+struct Error {
+    int code;
+    std::string msg;
+};
+#define QG_HIP(call)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            throw ::qg::Error{QG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+#define QG_REQUIRE(cond, msg)                                         \
+    do {                                                              \
+        if (!(cond)) throw ::qg::Error{QG_ERR_ARG, std::string(msg)}; \
+    } while (0)
+
+// Run f, converting exceptions into a status code + qg_last_error().
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return QG_OK;
+    } catch (const Error& e) {
+        set_error(e.msg);
+        return e.code;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return QG_ERR_ARG;
+    }
+}
+
+hipStream_t default_stream();
+// NULL is HIP's null stream (the legacy default stream), as everywhere in HIP.
+inline hipStream_t as_stream(void* s) { return (hipStream_t)s; }
+
+// ------------------------------------------------------------- device buffers ----
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        if (count <= n && p) return;
+        release();
+        if (count == 0) return;
+        QG_HIP(hipMalloc(&p, count * sizeof(T)));
+        n = count;
+    }
+    void upload(const T* host, size_t count, hipStream_t s = nullptr) {
+        alloc(count);
+        if (count) QG_HIP(hipMemcpyAsync(p, host, count * sizeof(T), hipMemcpyHostToDevice, s));
+        if (count) QG_HIP(hipStreamSynchronize(s));
+    }
+};
+
+}  // namespace qg
+
+// ------------------------------------------------------------------ handles ----
+struct qg_grid {
+    int n[3];
+    double h[3];
+    int64_t ng() const { return (int64_t)n[0] * n[1] * n[2]; }
+    double dvol() const { return h[0] * h[1] * h[2]; }
+};
+
+namespace qg {
+
+// One projector channel as the kernels see it.
+struct DevProj {
+    int l;
+    int kind;          // QG_RADIAL_*
+    int coef_rel;      // coefficient index of this channel's m = first within its atom
+    int table_off;     // offset into the table pool (QG_RADIAL_TABLE)
+    int n_table;
+    double energy;
+    double amp;        // gaussian amplitude
+    double inv_2s2;    // 1/(2 sigma^2)
+    double xi_min, xi_delta, r_min2;
+};
+
+// One periodic-image block of one (atom, cutoff group): the overlap of the atom's
+// GridForSphere box shifted by (sx,sy,sz) grid periods with the grid.
+struct DevImage {
+    int b[3], e[3];     // in-grid range [b, e)
+    int shift[3];       // grid-index shift of the image (multiples of N)
+    int row_base;       // offset of this block's first row in the interval table
+    int coef_base;      // first coefficient of the group (atom-major coefficient order)
+    int proj_begin;     // first DevProj of the group
+    int proj_end;
+    int atom;
+    double R[3];        // nucleus position
+};
+
+// Host-side geometry of the nonlocal operator (built once per UpdatePosition).
+struct NlGeometry {
+    std::vector<DevProj> projs;
+    std::vector<double> tables;
+    std::vector<DevImage> images;
+    std::vector<int2> intervals;   // per row of each image block: [x0, x1) inside the cutoff
+    std::vector<int> atom_img_off; // images of atom a: [atom_img_off[a], atom_img_off[a+1])
+    std::vector<int> atom_coef_off;
+    int ncoef = 0;
+    int natoms = 0;
+    int max_coef_per_atom = 0;
+};
+
+void build_nl_geometry(const qg_grid& g, int natoms, const double* xyz, const int* species,
+                       int nspecies, const int* nproj, const qg_proj_spec* specs, NlGeometry& out);
+
+// Per-(y tile, z plane) lists of images for the fused kernel (CSR).
+void build_tile_lists(const qg_grid& g, const NlGeometry& geo, int ty, std::vector<int>& off,
+                      std::vector<int>& ids);
+
+}  // namespace qg
+
+struct qg_nonlocal {
+    const qg_grid* grid = nullptr;
+    qg_grid gcopy;
+    qg::NlGeometry geo;
+    qg::DevBuf<qg::DevProj> d_projs;
+    qg::DevBuf<double> d_tables;
+    qg::DevBuf<qg::DevImage> d_images;
+    qg::DevBuf<int2> d_intervals;
+    qg::DevBuf<int> d_atom_img_off;
+    qg::DevBuf<int> d_atom_coef_off;
+    qg::DevBuf<double> d_coef;   // scratch coefficients (nbands * ncoef)
+    // tile lists cached for one tile height
+    int list_ty = -1;
+    qg::DevBuf<int> d_list_off, d_list_ids;
+};
+
+struct qg_ham {
+    qg_grid grid;
+    qg_nonlocal* nl = nullptr;
+    qg::DevBuf<double> V64;
+    qg::DevBuf<float> V32;
+    bool timing = false;
+    bool timed_valid = false;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    ~qg_ham() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
