@@ -168,6 +168,20 @@ int qg_lobpcg_d_multi(qg_ham* h, const qg_lobpcg_opts* opts, int nbands,
                       double* eigvals_host, double* X_dev, double* P_dev, int scf_step,
                       qg_lobpcg_stats* stats);
 
+/* Apply a preconditioner in place to nbands residual vectors r (device, ld == Ng):
+ * QG_PRECOND_KINETIC: r <- (T + shift)^-1 r with T = -1/2 Lap8 the periodic 8th-order kinetic
+ * operator, exactly, through FFTs.  No counterpart in the reference (which uses the raw
+ * residual, lobpcg_d_multi_mpi.h:982-990); it is the classic preconditioner stage. */
+int qg_precond_apply(qg_ham* h, int kind, double shift, int dtype, int nbands, void* r, int64_t ld,
+                     void* stream);
+
+/* Gram product C = A^T B on the matrix cores: A (K x M) and B (K x N) FP32 device blocks
+ * (column m at A + m*lda), rounded to BF16, FP32 accumulation, C FP64 device (column
+ * stride ldc).  The BF16 Rayleigh-Ritz product of the mixed-precision LOBPCG; replaces
+ * blas_DGEMM_t (w_dgemm.h:32-37) inside MakeMatrix_d_blas (lobpcg_d_multi_mpi.h:192-250). */
+int qg_gram_bf16(const float* A, const float* B, int64_t K, int M, int N, int64_t lda, int64_t ldb,
+                 double* C, int ldc, void* stream);
+
 /* ------------------------------------------------------- synthetic workloads ---- */
 /* Host-only helpers that build the seeded diamond cells used to exercise this path:
  * ncell^3 diamond supercell of the 8-atom cubic cell with lattice a (bohr) on an

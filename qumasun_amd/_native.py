@@ -74,6 +74,9 @@ SIGNATURES = {
     "qg_lobpcg_default_opts": (None, [P(LobpcgOpts)]),
     "qg_lobpcg_d_multi": (c_int, [c_void_p, P(LobpcgOpts), c_int, P(c_double), c_void_p, c_void_p,
                                   c_int, P(LobpcgStats)]),
+    "qg_precond_apply": (c_int, [c_void_p, c_int, c_double, c_int, c_int, c_void_p, c_int64, c_void_p]),
+    "qg_gram_bf16": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_int, c_int64, c_int64, c_void_p, c_int,
+                             c_void_p]),
     "qg_synth_diamond_atoms": (c_int, [c_int, c_double, P(c_double)]),
     "qg_synth_vloc": (c_int, [c_int, c_int, c_int, c_double, c_double, c_double, c_int, P(c_double),
                               c_double, c_double, c_double, c_uint32, P(c_double)]),

@@ -57,7 +57,7 @@ def build(verbose=False):
     lib = os.path.join(LIBDIR, LIBNAME)
     tmp = lib + ".tmp"
     _run([HIPCC, *DEVICE, "-shared", "-fPIC", "-o", tmp, *objs,
-          "-L/opt/rocm/lib", "-lrocblas", "-lrocsolver", "-Wl,-rpath,/opt/rocm/lib"])
+          "-L/opt/rocm/lib", "-lrocblas", "-lrocsolver", "-lrocfft", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, lib)
     return lib
 

@@ -3,30 +3,30 @@
 // dot-product kernel, and their C-ABI entry points.
 //
 // Data layout in HBM: band-major blocks, x fastest (psi[b*ld + ix + Nx*(iy + Ny*iz)]).
+// Projector values beta_j = p(r) Y_lm live in tables over the sphere points only (the
+// reference's per-block projector and Ylm tables), flattened image by image in (z, y, x)
+// order: 1.51 Ng points x 4 slots for the s+p workloads.
 //
-// Fused kernel design (one pass over psi and Hpsi):
-//   * a workgroup owns TY full x-rows of the grid, a slab of TZ z-planes and NB bands;
-//   * it streams through its slab in z, keeping each point's 9 z-neighbours in registers
-//     (a register queue), and stages the current plane of every band with a 4-row y-halo
-//     in LDS (double-buffered, one barrier per plane); x-neighbours wrap inside the LDS
-//     row because the tile spans the whole periodic x extent, so there is no x-halo;
-//   * V and the projector terms are applied in the same pass: per (y tile, z plane) the
-//     host precomputes the list of projector images that reach it, each image carries
-//     exact per-row [x0,x1) sphere intervals, and beta = p(r) Y_lm is evaluated in-kernel
-//     (no per-point projector tables in HBM);
-//   * workgroup ids are remapped so that consecutive tiles run on the same XCD and share
-//     that XCD's L2 for the halo rows.
-// Projector coefficients come from a separate atom-centric kernel (one workgroup per
-// atom and band chunk, rows of the sphere mapped to half-waves, wavefront reductions).
+// Fused kernel (one pass over psi and H psi):
+//   * a workgroup owns TY full x-rows, a slab of TZ z-planes and NB bands; it streams
+//     through the slab in z with each point's 9 z-neighbours in a register queue and the
+//     current plane (plus a 4-row y-halo and 4 wrapped x-columns) in LDS, double-buffered,
+//     with the next plane's loads issued one plane ahead;
+//   * the projector term of a plane is computed item-parallel: every sphere point of every
+//     segment (image x tile row) reaching the plane is one work item; its table entry is
+//     combined with E*coef (staged per workgroup) and added to a per-point LDS accumulator
+//     with LDS atomics (spheres overlap), so no lane waits on another's sphere;
+//   * workgroup ids are remapped so each XCD gets a contiguous range (y tile fastest), and
+//     neighbouring tiles share that XCD's L2 for halo rows, V and projector tables.
+// Projector coefficients come from a table-driven, atom-centric kernel.  A block of bands is
+// processed in chunks of about 64 MB (projection, then fused apply), so the fused kernel
+// re-reads each chunk from the 256 MB Infinity Cache rather than from HBM.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include "qg_internal.hpp"
 
 namespace qg {
-
-constexpr int kBlock = 256;
-constexpr double kY00 = 0.28209479177387814;     // 1/sqrt(4 pi)
-constexpr double kY1 = 0.48860251190291992;      // sqrt(3/(4 pi))
 
 template <class T>
 struct StencilCoef {
@@ -42,19 +42,43 @@ struct FusedArgs {
     int ntile_y, nslab;
     int nbands;
     int64_t ld_in, ld_out;
-    double h0, h1, h2;
     int accumulate;
-    // nonlocal (MODE_HAM with projectors, MODE_NL)
+    // projector term (MODE_HAM with projectors, MODE_NL)
     int has_nl;
-    const int* list_off;
-    const int* list_ids;
-    const DevImage* images;
-    const DevProj* projs;
-    const double* tables;
-    const int2* intervals;
-    const double* coef;
+    const int* wg_atom_off;
+    const int* wg_atom;
+    const int64_t* item_off;
+    const uint32_t* item_meta;
+    const void* item_beta;     // T, njs values per item
+    const double* coef;        // nbands x ncoef projections of this chunk
+    const double* coef_energy;
+    const int* atom_coef_off;
     int ncoef;
+    int max_atom;
 };
+
+// Buffer-resource access: SGPR base + 32-bit byte offsets (VGPR per point, SGPR per plane).
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+__device__ __forceinline__ rsrc_t make_rsrc(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+template <class T>
+__device__ __forceinline__ T bload(rsrc_t r, int voff, int soff) {
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+    } else {
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+    }
+}
+template <class T>
+__device__ __forceinline__ void bstore(rsrc_t r, T v, int voff, int soff) {
+    if constexpr (sizeof(T) == 4) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
+    } else {
+        using u2 = __attribute__((ext_vector_type(2))) uint32_t;
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, voff, soff, 0);
+    }
+}
 
 // Map the hardware workgroup id (dispatched round-robin over the 8 XCDs) to a logical id
 // such that each XCD receives one contiguous range of logical ids.
@@ -65,82 +89,43 @@ __device__ __forceinline__ int xcd_remap(int i, int n) {
     return start + slot;
 }
 
-// RadialGrid2::GetValueByXi (RadialGrid2.h:81-136) on the device (FP64).
-__device__ double radial_by_xi(double xi, const double* v, int n, double xi_min, double xi_delta) {
-    const double rel = xi - xi_min;
-    const double fl = floor(rel / xi_delta);
-    const int i = (int)fl;
-    if (i >= n - 1) return 0.0;
-    if (i == 0 || i == n - 2) {
-        const double t = rel / xi_delta - fl;
-        return v[i] * (1.0 - t) + v[i + 1] * t;
-    }
-    const double t = rel / xi_delta - (double)i;
-    if (1 < i && i < n - 3) {
-        const double fm2 = v[i - 2], fm1 = v[i - 1], f0 = v[i], f1 = v[i + 1], f2 = v[i + 2], f3 = v[i + 3];
-        const double g0 = 2.0 / 3.0 * (f1 - fm1) - 1.0 / 12.0 * (f2 - fm2);
-        const double g1 = 2.0 / 3.0 * (f2 - f0) - 1.0 / 12.0 * (f3 - fm1);
-        const double h0 = 4.0 / 3.0 * (f1 + fm1) - 1.0 / 12.0 * (f2 + fm2) - 5.0 / 2.0 * f0;
-        const double h1 = 4.0 / 3.0 * (f2 + f0) - 1.0 / 12.0 * (f3 + fm1) - 5.0 / 2.0 * f1;
-        const double e0 = f1 - (f0 + g0 + h0 / 2.0);
-        const double e1 = g1 - (g0 + h0);
-        const double e2 = h1 - h0;
-        const double a3 = 10.0 * e0 - 4.0 * e1 + 0.5 * e2;
-        const double a4 = -15.0 * e0 + 7.0 * e1 - e2;
-        const double a5 = f1 - f0 - g0 - h0 / 2.0 - a3 - a4;
-        return f0 + t * (g0 + t * (h0 / 2.0 + t * (a3 + t * (a4 + t * a5))));
-    }
-    const double v0 = v[i], v1 = v[i + 1];
-    const double s1 = (v[i + 2] - v0) / 2.0, s0 = (v1 - v[i - 1]) / 2.0;
-    const double a = s1 + s0 - 2.0 * (v1 - v0);
-    const double b = -s1 - 2.0 * s0 + 3.0 * (v1 - v0);
-    return a * t * t * t + b * t * t + s0 * t + v0;
-}
+constexpr int kMaxThreads = 512;
+constexpr int kHaloPT = 4;  // halo points per thread and plane (8*nx <= 4*threads)
+constexpr int kItemPT = 4;  // projector work items per thread per round
 
-// Radial projector value p(r) at squared distance rr (inside the cutoff by construction).
-template <class T>
-__device__ __forceinline__ T radial_value(const DevProj& p, const double* tables, double rr) {
-    if (p.kind == QG_RADIAL_GAUSSIAN) {
-        if constexpr (sizeof(T) == 8) {
-            return (T)(p.amp * exp(-rr * p.inv_2s2));
-        } else {
-            return (T)p.amp * expf(-(float)rr * (float)p.inv_2s2);
-        }
-    }
-    const double* t = tables + p.table_off;
-    if (p.r_min2 > rr) return (T)t[0];
-    return (T)radial_by_xi(0.5 * log(rr), t, p.n_table, p.xi_min, p.xi_delta);
-}
-
-// Offsets of grid point (ix,iy,iz) from an image's nucleus (reference arithmetic order).
-__device__ __forceinline__ void image_offset(const DevImage& im, double h0, double h1, double h2,
-                                             int ix, int iy, int iz, double& x, double& y, double& z,
-                                             double& rr) {
-    z = __dsub_rn(__dmul_rn(h2, (double)(iz - im.shift[2])), im.R[2]);
-    y = __dsub_rn(__dmul_rn(h1, (double)(iy - im.shift[1])), im.R[1]);
-    x = __dsub_rn(__dmul_rn(h0, (double)(ix - im.shift[0])), im.R[0]);
-    const double zz = __dmul_rn(z, z);
-    const double yy_zz = __dadd_rn(__dmul_rn(y, y), zz);
-    rr = __dadd_rn(__dmul_rn(x, x), yy_zz);
-}
-
-// ---------------------------------------------------------------------------------------
-// Fused kernel.  T = float/double, NB bands per workgroup, PPT points per thread.
-template <class T, int NB, int PPT, int MODE>
-__global__ __launch_bounds__(kBlock) void fused_hpsi_kernel(FusedArgs a, StencilCoef<T> sc,
-                                                           const T* __restrict__ V,
-                                                           const T* __restrict__ psi,
-                                                           T* __restrict__ out) {
+// Fused kernel.  T = float/double, NB bands per workgroup, PPT points per thread, NJS table
+// slots per projector item.  blockDim.x = threads (multiple of 64, <= 512).
+template <class T, int NB, int PPT, int MODE, int NJS>
+__global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 3 : 4)))
+void fused_hpsi_kernel(FusedArgs a, StencilCoef<T> sc, const T* __restrict__ V,
+                       const T* __restrict__ psi, T* __restrict__ out) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* lds = reinterpret_cast<T*>(smem_raw);
+    constexpr int S = sizeof(T);
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const int lds_band = (a.ty + 8) * nx;
+    const int nthr = blockDim.x;
+    const int W = nx + 8;                 // LDS row: 4 wrapped columns each side
+    const int lds_band = (a.ty + 8) * W;
+    const int tile_pts = a.ty * nx;
+    // LDS carve-up
+    unsigned char* lp = smem_raw;
+    T* planes = reinterpret_cast<T*>(lp);
+    if (MODE != MODE_NL) lp += 2 * (size_t)NB * lds_band * S;
+    T* nl_l = reinterpret_cast<T*>(lp);       // per-point projector term, NB x tile_pts
+    T* ec_l = nullptr;                        // E*coef per (local atom, slot, band)
+    if (a.has_nl) {
+        lp += (size_t)NB * tile_pts * S;
+        ec_l = reinterpret_cast<T*>(lp);
+    }
 
     const int wg = xcd_remap(blockIdx.x, gridDim.x);
-    const int ytile = wg % a.ntile_y;
-    const int rest = wg / a.ntile_y;
-    const int slab = rest % a.nslab;
-    const int chunk = rest / a.nslab;
+    // logical order: band chunk fastest, so the workgroups of one (tile, slab) run back to
+    // back on one XCD and find V and the projector items in its L2; halo rows and the
+    // z warm-up planes of neighbouring tiles come from the Infinity Cache
+    const int nchunk = (a.nbands + NB - 1) / NB;
+    const int chunk = wg % nchunk;
+    const int rest = wg / nchunk;
+    const int ytile = rest % a.ntile_y;
+    const int slab = rest / a.ntile_y;
     const int y0 = ytile * a.ty;
     const int rows = min(a.ty, ny - y0);
     const int z0 = slab * a.tz;
@@ -148,166 +133,192 @@ __global__ __launch_bounds__(kBlock) void fused_hpsi_kernel(FusedArgs a, Stencil
     const int b0 = chunk * NB;
     const int tid = threadIdx.x;
     const int npts = rows * nx;
-    const int64_t plane = (int64_t)nx * ny;
-    const int64_t tile_off = (int64_t)y0 * nx;
+    const int plane = nx * ny;
+    const uint32_t band_bytes = (uint32_t)((int64_t)plane * nz * S);
 
-    int pix[PPT], piy[PPT];
-    bool pv[PPT];
-#pragma unroll
-    for (int k = 0; k < PPT; ++k) {
-        const int p = tid + kBlock * k;
-        pv[k] = p < npts;
-        piy[k] = pv[k] ? p / nx : 0;
-        pix[k] = pv[k] ? p - piy[k] * nx : 0;
-    }
-    const T* src[NB];
-    T* dst[NB];
+    rsrc_t rsrc_in[NB], rsrc_out[NB];
     bool bv[NB];
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
         bv[b] = (b0 + b) < a.nbands;
         const int bb = bv[b] ? b0 + b : a.nbands - 1;
-        src[b] = psi + (int64_t)bb * a.ld_in;
-        dst[b] = out + (int64_t)bb * a.ld_out;
+        rsrc_in[b] = make_rsrc(psi + (int64_t)bb * a.ld_in, band_bytes);
+        rsrc_out[b] = make_rsrc(out + (int64_t)bb * a.ld_out, band_bytes);
+    }
+    const rsrc_t rsrc_v = make_rsrc(V, band_bytes);
+
+    int vo[PPT];    // byte offset of the point inside a plane
+    int lc[PPT];    // LDS index of the point (row iy+4, column ix+4), -1 when idle
+    int ldup[PPT];  // LDS index of its wrapped copy in the x halo, or -1
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        const int p = tid + nthr * k;
+        const bool ok = p < npts;
+        const int iy = ok ? p / nx : 0;
+        const int ix = ok ? p - iy * nx : 0;
+        vo[k] = ok ? (y0 * nx + p) * S : 0;
+        lc[k] = ok ? (iy + 4) * W + ix + 4 : -1;
+        ldup[k] = (ok && ix < 4) ? lc[k] + nx : ((ok && ix >= nx - 4) ? lc[k] - nx : -1);
     }
     // halo elements of this thread: 8 rows x nx (4 above, 4 below the tile), wrapped in y
-    constexpr int HPPT = 4;   // supports nx <= 128
-    int h_lds[HPPT], h_glb[HPPT];
-    bool hv[HPPT];
+    int ho[kHaloPT], hl[kHaloPT];
 #pragma unroll
-    for (int j = 0; j < HPPT; ++j) {
-        const int e = tid + kBlock * j;
-        hv[j] = (MODE != MODE_NL) && e < 8 * nx;
-        const int r8 = hv[j] ? e / nx : 0;
-        const int col = hv[j] ? e - r8 * nx : 0;
+    for (int j = 0; j < kHaloPT; ++j) {
+        const int e = tid + nthr * j;
+        const bool ok = (MODE != MODE_NL) && e < 8 * nx;
+        const int r8 = ok ? e / nx : 0;
+        const int col = ok ? e - r8 * nx : 0;
         int gy = (r8 < 4) ? y0 - 4 + r8 : y0 + rows + (r8 - 4);
         gy = ((gy % ny) + ny) % ny;
         const int ly = (r8 < 4) ? r8 : rows + 4 + (r8 - 4);
-        h_lds[j] = ly * nx + col;
-        h_glb[j] = gy * nx + col;
+        ho[j] = ok ? (gy * nx + col) * S : 0;
+        hl[j] = ok ? ly * W + col + 4 : -1;
     }
-    auto zwrap = [&](int z) { int zz = z % nz; return zz < 0 ? zz + nz : zz; };
+    auto zoff = [&](int z) { int zz = z % nz; zz = zz < 0 ? zz + nz : zz; return zz * plane * S; };
+
+    // ---- projector staging for the workgroup: E*coef per (atom slot, j, band) ----
+    const int wgi = ytile * a.nslab + slab;
+    if (a.has_nl) {
+        const int a0 = a.wg_atom_off[wgi];
+        const int natom = a.wg_atom_off[wgi + 1] - a0;
+        for (int i = tid; i < natom * NJS * NB; i += nthr) {
+            const int as = i / (NJS * NB), j = (i / NB) % NJS, b = i % NB;
+            const int at = a.wg_atom[a0 + as];
+            const int cb = a.atom_coef_off[at];
+            const int cnt = a.atom_coef_off[at + 1] - cb;
+            T v = T(0);
+            if (j < cnt && (b0 + b) < a.nbands)
+                v = (T)(a.coef_energy[cb + j] * a.coef[(int64_t)(b0 + b) * a.ncoef + cb + j]);
+            ec_l[i] = v;
+        }
+        for (int i = tid; i < NB * tile_pts; i += nthr) nl_l[i] = T(0);
+    }
 
     T q[NB][PPT][9];
+    T qn[NB][PPT];
+    T hn[NB][kHaloPT];
+    T vn[PPT];
     if constexpr (MODE != MODE_NL) {
 #pragma unroll
         for (int d = 0; d < 8; ++d) {
-            const int64_t off = (int64_t)zwrap(z0 - 4 + d) * plane + tile_off;
+            const int so = zoff(z0 - 4 + d);
 #pragma unroll
             for (int b = 0; b < NB; ++b)
 #pragma unroll
-                for (int k = 0; k < PPT; ++k) q[b][k][d] = pv[k] ? src[b][off + tid + kBlock * k] : T(0);
+                for (int k = 0; k < PPT; ++k) q[b][k][d] = bload<T>(rsrc_in[b], vo[k], so);
         }
     }
-
-    for (int z = z0; z < z1; ++z) {
-        const int64_t zoff = (int64_t)z * plane;
-        T res[NB][PPT];
+    auto prefetch = [&](int z) {   // stencil inputs of plane z: psi(z+4), halo(z), V(z)
         if constexpr (MODE != MODE_NL) {
-            const int64_t off = (int64_t)zwrap(z + 4) * plane + tile_off;
-#pragma unroll
-            for (int b = 0; b < NB; ++b)
-#pragma unroll
-                for (int k = 0; k < PPT; ++k) q[b][k][8] = pv[k] ? src[b][off + tid + kBlock * k] : T(0);
-            T* L = lds + ((z - z0) & 1) * NB * lds_band;
+            const int so = zoff(z + 4), sh = zoff(z);
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
 #pragma unroll
-                for (int k = 0; k < PPT; ++k)
-                    if (pv[k]) L[b * lds_band + 4 * nx + tid + kBlock * k] = q[b][k][4];
+                for (int k = 0; k < PPT; ++k) qn[b][k] = bload<T>(rsrc_in[b], vo[k], so);
 #pragma unroll
-                for (int j = 0; j < HPPT; ++j)
-                    if (hv[j]) L[b * lds_band + h_lds[j]] = src[b][zoff + h_glb[j]];
+                for (int j = 0; j < kHaloPT; ++j) hn[b][j] = (hl[j] >= 0) ? bload<T>(rsrc_in[b], ho[j], sh) : T(0);
             }
         }
-        // local potential / accumulate base
+        if constexpr (MODE == MODE_HAM) {
+            const int sv = zoff(z);
 #pragma unroll
-        for (int k = 0; k < PPT; ++k) {
-            T v = T(0);
-            if constexpr (MODE == MODE_HAM) v = pv[k] ? V[zoff + tile_off + tid + kBlock * k] : T(0);
+            for (int k = 0; k < PPT; ++k) vn[k] = bload<T>(rsrc_v, vo[k], sv);
+        }
+    };
+    prefetch(z0);
+    const T* ibeta = reinterpret_cast<const T*>(a.item_beta);
+
+    for (int z = z0; z < z1; ++z) {
+        const int buf = (z - z0) & 1;
+        const int sz = z * plane * S;
+        T vcur[PPT];
+        // commit the prefetched plane into registers / LDS
+        if constexpr (MODE != MODE_NL) {
+            T* L = planes + buf * NB * lds_band;
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+#pragma unroll
+                for (int k = 0; k < PPT; ++k) {
+                    q[b][k][8] = qn[b][k];
+                    if (lc[k] >= 0) L[b * lds_band + lc[k]] = q[b][k][4];
+                    if (ldup[k] >= 0) L[b * lds_band + ldup[k]] = q[b][k][4];
+                }
+#pragma unroll
+                for (int j = 0; j < kHaloPT; ++j)
+                    if (hl[j] >= 0) L[b * lds_band + hl[j]] = hn[b][j];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) vcur[k] = (MODE == MODE_HAM) ? vn[k] : T(0);
+        // issue the loads of the next plane
+        if (z + 1 < z1) prefetch(z + 1);
+        __syncthreads();
+
+        // projector term: every (image, sphere point) reaching this plane is one work item
+        if (MODE != MODE_LAP && a.has_nl) {
+            const size_t tp = (size_t)ytile * nz + z;
+            const int64_t io0 = a.item_off[tp];
+            const int nitem = (int)(a.item_off[tp + 1] - io0);
+            for (int base = 0; base < nitem; base += nthr * kItemPT) {
+                uint32_t meta[kItemPT];
+                T bj[kItemPT][NJS];
+#pragma unroll
+                for (int u = 0; u < kItemPT; ++u) {
+                    const int it = base + tid + nthr * u;
+                    meta[u] = 0xffffffffu;
+                    if (it < nitem) {
+                        meta[u] = a.item_meta[io0 + it];
+                        const T* bt = ibeta + (io0 + it) * NJS;
+#pragma unroll
+                        for (int j = 0; j < NJS; ++j) bj[u][j] = bt[j];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kItemPT; ++u) {
+                    if (meta[u] == 0xffffffffu) continue;
+                    const int pt = (int)(meta[u] & 0xffffu);
+                    const T* ec = ec_l + (int)(meta[u] >> 16) * NJS * NB;
+#pragma unroll
+                    for (int b = 0; b < NB; ++b) {
+                        T acc = T(0);
+#pragma unroll
+                        for (int j = 0; j < NJS; ++j) acc += ec[j * NB + b] * bj[u][j];
+                        atomicAdd(&nl_l[b * tile_pts + pt], acc);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+
+        T res[NB][PPT];
+#pragma unroll
+        for (int k = 0; k < PPT; ++k)
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
                 if constexpr (MODE == MODE_HAM) {
-                    res[b][k] = v * q[b][k][4];
+                    res[b][k] = vcur[k] * q[b][k][4];
                 } else {
-                    res[b][k] = (a.accumulate && pv[k] && bv[b]) ? dst[b][zoff + tile_off + tid + kBlock * k] : T(0);
+                    res[b][k] = (a.accumulate && lc[k] >= 0) ? bload<T>(rsrc_out[b], vo[k], sz) : T(0);
+                }
+                if (MODE != MODE_LAP && a.has_nl && lc[k] >= 0) {
+                    T* nlp = &nl_l[b * tile_pts + tid + nthr * k];
+                    res[b][k] += *nlp;
+                    *nlp = T(0);
                 }
             }
-        }
-        // nonlocal projector terms
-        if (MODE != MODE_LAP && a.has_nl) {
-            const int slot = ytile * nz + z;
-            const int lo = a.list_off[slot], hi = a.list_off[slot + 1];
-            for (int e = lo; e < hi; ++e) {
-                const DevImage im = a.images[a.list_ids[e]];
-                const int wy = im.e[1] - im.b[1];
-                const bool zin = (z >= im.b[2]) && (z < im.e[2]);
-                if (!zin) continue;
-#pragma unroll
-                for (int k = 0; k < PPT; ++k) {
-                    const int iy = y0 + piy[k];
-                    const int ix = pix[k];
-                    bool in = pv[k] && iy >= im.b[1] && iy < im.e[1];
-                    if (in) {
-                        const int2 iv = a.intervals[im.row_base + (iy - im.b[1]) + wy * (z - im.b[2])];
-                        in = ix >= iv.x && ix < iv.y;
-                    }
-                    if (!in) continue;
-                    double x, y, zc, rr;
-                    image_offset(im, a.h0, a.h1, a.h2, ix, iy, z, x, y, zc, rr);
-                    for (int pj = im.proj_begin; pj < im.proj_end; ++pj) {
-                        const DevProj pr = a.projs[pj];
-                        const T rad = radial_value<T>(pr, a.tables, rr);
-                        const double* cb = a.coef + im.coef_base + pr.coef_rel;
-                        if (pr.l == 0) {
-                            const T beta = rad * (T)kY00;
-#pragma unroll
-                            for (int b = 0; b < NB; ++b)
-                                res[b][k] += (T)(pr.energy * cb[(int64_t)(b0 + (bv[b] ? b : 0)) * a.ncoef]) * beta;
-                        } else {
-                            T ym1 = T(0), y0v = T(0), yp1 = T(0);
-                            if (!(1.0e-14 > rr)) {
-                                const double ir = 1.0 / sqrt(rr);
-                                ym1 = (T)(kY1 * (y * ir));
-                                y0v = (T)(kY1 * (zc * ir));
-                                yp1 = (T)(kY1 * (x * ir));
-                            }
-#pragma unroll
-                            for (int b = 0; b < NB; ++b) {
-                                const double* c = cb + (int64_t)(b0 + (bv[b] ? b : 0)) * a.ncoef;
-                                res[b][k] += (T)(pr.energy * c[0]) * (rad * ym1) +
-                                             (T)(pr.energy * c[1]) * (rad * y0v) +
-                                             (T)(pr.energy * c[2]) * (rad * yp1);
-                            }
-                        }
-                    }
-                }
-            }
-        }
         if constexpr (MODE != MODE_NL) {
-            __syncthreads();
-            const T* L = lds + ((z - z0) & 1) * NB * lds_band;
+            const T* L = planes + buf * NB * lds_band;
 #pragma unroll
             for (int k = 0; k < PPT; ++k) {
-                if (!pv[k]) continue;
-                const int ix = pix[k];
-                const int c = (piy[k] + 4) * nx + ix;
-                int xm[4], xp[4];
-#pragma unroll
-                for (int m = 0; m < 4; ++m) {
-                    int t = ix - (m + 1);
-                    xm[m] = (t < 0 ? t + nx : t) - ix;
-                    t = ix + (m + 1);
-                    xp[m] = (t >= nx ? t - nx : t) - ix;
-                }
+                if (lc[k] < 0) continue;
 #pragma unroll
                 for (int b = 0; b < NB; ++b) {
-                    const T* Lb = L + b * lds_band + c;
+                    const T* Lb = L + b * lds_band + lc[k];
                     T d = sc.c0 * q[b][k][4];
-                    d += sc.cx[3] * (Lb[xp[3]] + Lb[xm[3]]) + sc.cx[2] * (Lb[xp[2]] + Lb[xm[2]]) +
-                         sc.cx[1] * (Lb[xp[1]] + Lb[xm[1]]) + sc.cx[0] * (Lb[xp[0]] + Lb[xm[0]]);
-                    d += sc.cy[3] * (Lb[4 * nx] + Lb[-4 * nx]) + sc.cy[2] * (Lb[3 * nx] + Lb[-3 * nx]) +
-                         sc.cy[1] * (Lb[2 * nx] + Lb[-2 * nx]) + sc.cy[0] * (Lb[nx] + Lb[-nx]);
+                    d += sc.cx[3] * (Lb[4] + Lb[-4]) + sc.cx[2] * (Lb[3] + Lb[-3]) +
+                         sc.cx[1] * (Lb[2] + Lb[-2]) + sc.cx[0] * (Lb[1] + Lb[-1]);
+                    d += sc.cy[3] * (Lb[4 * W] + Lb[-4 * W]) + sc.cy[2] * (Lb[3 * W] + Lb[-3 * W]) +
+                         sc.cy[1] * (Lb[2 * W] + Lb[-2 * W]) + sc.cy[0] * (Lb[W] + Lb[-W]);
                     d += sc.cz[3] * (q[b][k][8] + q[b][k][0]) + sc.cz[2] * (q[b][k][7] + q[b][k][1]) +
                          sc.cz[1] * (q[b][k][6] + q[b][k][2]) + sc.cz[0] * (q[b][k][5] + q[b][k][3]);
                     res[b][k] += d;
@@ -318,7 +329,7 @@ __global__ __launch_bounds__(kBlock) void fused_hpsi_kernel(FusedArgs a, Stencil
         for (int b = 0; b < NB; ++b)
 #pragma unroll
             for (int k = 0; k < PPT; ++k)
-                if (pv[k] && bv[b]) dst[b][zoff + tile_off + tid + kBlock * k] = res[b][k];
+                if (lc[k] >= 0 && bv[b]) bstore<T>(rsrc_out[b], res[b][k], vo[k], sz);
         if constexpr (MODE != MODE_NL) {
 #pragma unroll
             for (int b = 0; b < NB; ++b)
@@ -331,101 +342,70 @@ __global__ __launch_bounds__(kBlock) void fused_hpsi_kernel(FusedArgs a, Stencil
 }
 
 // ---------------------------------------------------------------------------------------
-// Projector coefficients: coef[b*ncoef + j] = dV * sum_i beta_j(i) psi_b(i).
-// One workgroup per (atom, chunk of NBA bands); the atom's image rows are spread over the
-// 8 half-waves of the block, lanes walk x inside the exact sphere interval.
+// Projector coefficients: coef[b*ncoef + cbase(a) + j] = dV * sum_i beta_j(i) psi_b(i).
+// One workgroup per (atom, chunk of NBA bands), streaming the atom's table entries.
 struct ProjArgs {
-    int nx, ny, nz;
     int natoms;
     int nbands;
     int64_t ld;
-    double h0, h1, h2, dv;
+    double dv;
     const int* atom_img_off;
     const int* atom_coef_off;
-    const DevImage* images;
-    const DevProj* projs;
-    const double* tables;
-    const int2* intervals;
+    const int64_t* img_tbl_off;
+    const int* gidx;
+    const void* beta;
     double* coef;
     int ncoef;
 };
 
-template <int MAXJ, class T>
-__device__ __forceinline__ void eval_beta(const DevImage& im, const DevProj* projs, const double* tables,
-                                          double x, double y, double z, double rr, T beta[MAXJ]) {
-#pragma unroll
-    for (int j = 0; j < MAXJ; ++j) beta[j] = T(0);
-    for (int pj = im.proj_begin; pj < im.proj_end; ++pj) {
-        const DevProj pr = projs[pj];
-        const T rad = radial_value<T>(pr, tables, rr);
-        if (pr.l == 0) {
-#pragma unroll
-            for (int j = 0; j < MAXJ; ++j)
-                if (j == pr.coef_rel) beta[j] = rad * (T)kY00;
-        } else {
-            T ym1 = T(0), y0v = T(0), yp1 = T(0);
-            if (!(1.0e-14 > rr)) {
-                const double ir = 1.0 / sqrt(rr);
-                ym1 = (T)(kY1 * (y * ir));
-                y0v = (T)(kY1 * (z * ir));
-                yp1 = (T)(kY1 * (x * ir));
-            }
-#pragma unroll
-            for (int j = 0; j < MAXJ; ++j) {
-                if (j == pr.coef_rel) beta[j] = rad * ym1;
-                if (j == pr.coef_rel + 1) beta[j] = rad * y0v;
-                if (j == pr.coef_rel + 2) beta[j] = rad * yp1;
-            }
-        }
-    }
-}
+constexpr int kProjBlock = 256;
 
-template <class T, int NBA, int MAXJ>
-__global__ __launch_bounds__(kBlock) void project_kernel(ProjArgs a, const T* __restrict__ psi) {
-    using Acc = T;
-    __shared__ double red[kBlock / 64][MAXJ * NBA];
+template <class T, int NBA, int NJS>
+__global__ __launch_bounds__(kProjBlock) void project_kernel(ProjArgs a, const T* __restrict__ psi) {
+    __shared__ double red[kProjBlock / 64][NJS * NBA];
     const int atom = blockIdx.x % a.natoms;
     const int chunk = blockIdx.x / a.natoms;
     const int b0 = chunk * NBA;
     const int tid = threadIdx.x;
-    const int half = tid >> 5, lane = tid & 31;
-    const int64_t plane = (int64_t)a.nx * a.ny;
+    const T* beta = reinterpret_cast<const T*>(a.beta);
     const T* src[NBA];
 #pragma unroll
     for (int b = 0; b < NBA; ++b) src[b] = psi + (int64_t)min(b0 + b, a.nbands - 1) * a.ld;
-    Acc acc[MAXJ][NBA];
+    T acc[NJS][NBA];
 #pragma unroll
-    for (int j = 0; j < MAXJ; ++j)
+    for (int j = 0; j < NJS; ++j)
 #pragma unroll
-        for (int b = 0; b < NBA; ++b) acc[j][b] = Acc(0);
-
-    for (int ii = a.atom_img_off[atom]; ii < a.atom_img_off[atom + 1]; ++ii) {
-        const DevImage im = a.images[ii];
-        const int wy = im.e[1] - im.b[1];
-        const int nrows = wy * (im.e[2] - im.b[2]);
-        for (int row = half; row < nrows; row += kBlock / 32) {
-            const int iz = im.b[2] + row / wy;
-            const int iy = im.b[1] + row % wy;
-            const int2 iv = a.intervals[im.row_base + row];
-            for (int ix = iv.x + lane; ix < iv.y; ix += 32) {
-                double x, y, z, rr;
-                image_offset(im, a.h0, a.h1, a.h2, ix, iy, iz, x, y, z, rr);
-                T beta[MAXJ];
-                eval_beta<MAXJ, T>(im, a.projs, a.tables, x, y, z, rr, beta);
-                const int64_t gi = (int64_t)iz * plane + (int64_t)iy * a.nx + ix;
+        for (int b = 0; b < NBA; ++b) acc[j][b] = T(0);
+    const int64_t t0 = a.img_tbl_off[a.atom_img_off[atom]];
+    const int64_t t1 = a.img_tbl_off[a.atom_img_off[atom + 1]];
+    constexpr int U = 4;   // independent table points in flight per thread
+    for (int64_t base = t0; base < t1; base += (int64_t)kProjBlock * U) {
+        int g[U];
+        bool ok[U];
+        T bj[U][NJS];
+        T v[U][NBA];
 #pragma unroll
-                for (int b = 0; b < NBA; ++b) {
-                    const T v = src[b][gi];
+        for (int u = 0; u < U; ++u) {
+            const int64_t i = base + tid + (int64_t)kProjBlock * u;
+            ok[u] = i < t1;
+            g[u] = ok[u] ? a.gidx[i] : 0;
 #pragma unroll
-                    for (int j = 0; j < MAXJ; ++j) acc[j][b] += beta[j] * v;
-                }
-            }
+            for (int j = 0; j < NJS; ++j) bj[u][j] = ok[u] ? beta[i * NJS + j] : T(0);
         }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int b = 0; b < NBA; ++b) v[u][b] = ok[u] ? src[b][g[u]] : T(0);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int b = 0; b < NBA; ++b)
+#pragma unroll
+                for (int j = 0; j < NJS; ++j) acc[j][b] += bj[u][j] * v[u][b];
     }
-    // block reduction in FP64
     const int wave = tid >> 6;
 #pragma unroll
-    for (int j = 0; j < MAXJ; ++j)
+    for (int j = 0; j < NJS; ++j)
 #pragma unroll
         for (int b = 0; b < NBA; ++b) {
             double v = (double)acc[j][b];
@@ -434,14 +414,14 @@ __global__ __launch_bounds__(kBlock) void project_kernel(ProjArgs a, const T* __
             if ((tid & 63) == 0) red[wave][j * NBA + b] = v;
         }
     __syncthreads();
-    if (tid < MAXJ * NBA) {
+    if (tid < NJS * NBA) {
         const int j = tid / NBA, b = tid % NBA;
-        double s = 0.0;
+        double sum = 0.0;
 #pragma unroll
-        for (int w = 0; w < kBlock / 64; ++w) s += red[w][tid];
+        for (int w = 0; w < kProjBlock / 64; ++w) sum += red[w][tid];
         const int cbase = a.atom_coef_off[atom];
         const int count = a.atom_coef_off[atom + 1] - cbase;
-        if (b0 + b < a.nbands && j < count) a.coef[(int64_t)(b0 + b) * a.ncoef + cbase + j] = s * a.dv;
+        if (b0 + b < a.nbands && j < count) a.coef[(int64_t)(b0 + b) * a.ncoef + cbase + j] = sum * a.dv;
     }
 }
 
@@ -462,45 +442,80 @@ static StencilCoef<T> make_coef(const qg_grid& g, double coef) {
 }
 
 struct Plan {
-    int ty, tz, ppt, nb;
-    size_t lds;
+    int ty, tz, ppt, nb, nthr;
 };
 
-// Tile plan: TY full rows so that a 256-thread block holds ~4-5 points per thread.
-static Plan make_plan(const qg_grid& g, int dtype, int nbands, int mode) {
+static int round64(int v) { return (v + 63) / 64 * 64; }
+
+static int env_int(const char* name, int dflt) {
+    const char* v = std::getenv(name);
+    return (v && *v) ? std::atoi(v) : dflt;
+}
+
+// Tile plan: a tile is TY full x-rows; PPT points per thread; threads = ceil(TY*Nx/PPT)
+// rounded up to whole wavefronts (<= 512); slabs of TZ planes, deep enough to amortise the
+// 8-plane z warm-up, shallow enough to give >= ~2048 workgroups.  QG_PLAN_NB / _TY / _PPT /
+// _TZ override the choice for tuning sweeps.
+static Plan make_plan(const qg_grid& g, int dtype, int nbands) {
     const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
     QG_REQUIRE(nx <= 128, "the fused kernel supports Nx <= 128");
     Plan p{};
-    p.nb = (dtype == QG_F32) ? 2 : 1;
-    if (nbands < p.nb) p.nb = 1;
-    int ty = std::max(4, std::min(ny, 1280 / nx));
-    int ppt = (ty * nx + kBlock - 1) / kBlock;
-    const int allowed[] = {1, 2, 3, 4, 5, 6, 8};
-    int chosen = 8;
-    for (int v : allowed)
-        if (v >= ppt) { chosen = v; break; }
-    if (ppt > 8) { ty = (8 * kBlock) / nx; ppt = 8; chosen = 8; }
-    p.ty = ty;
-    p.ppt = chosen;
-    const int ntile = (ny + ty - 1) / ty;
+    p.nb = env_int("QG_PLAN_NB", 1);
+    p.nb = (p.nb == 2 && nbands >= 2) ? 2 : 1;
+    p.ty = std::max(1, std::min(ny, env_int("QG_PLAN_TY", 1280 / nx)));
+    p.ppt = env_int("QG_PLAN_PPT", dtype == QG_F64 ? 2 : 4);
+    if (dtype == QG_F64 && !std::getenv("QG_PLAN_TY")) p.ty = std::max(1, std::min(ny, 960 / nx));
+    QG_REQUIRE(p.ppt == 2 || p.ppt == 4 || p.ppt == 5 || p.ppt == 8, "QG_PLAN_PPT must be 2, 4, 5 or 8");
+    p.nthr = round64((p.ty * nx + p.ppt - 1) / p.ppt);
+    while (p.nthr > kMaxThreads) {
+        p.ppt = (p.ppt == 2) ? 4 : (p.ppt == 4) ? 5 : 8;
+        p.nthr = round64((p.ty * nx + p.ppt - 1) / p.ppt);
+        QG_REQUIRE(p.ppt <= 8 && (p.ppt < 8 || p.nthr <= kMaxThreads), "tile too large");
+    }
+    // enough threads to stage the 8 halo rows (kHaloPT points each)
+    p.nthr = std::max(p.nthr, round64((8 * nx + kHaloPT - 1) / kHaloPT));
+    QG_REQUIRE(8 * nx <= kHaloPT * p.nthr && p.nthr <= kMaxThreads, "halo does not fit the thread block");
+    const int ntile = (ny + p.ty - 1) / p.ty;
     const int nchunk = (nbands + p.nb - 1) / p.nb;
-    // slab depth: at least 8 planes, enough workgroups to fill 256 CUs several times
     int tz = nz;
-    while (tz > 8 && (int64_t)ntile * nchunk * ((nz + tz - 1) / tz) < 2048) tz = (tz + 1) / 2;
-    p.tz = std::max(1, tz);
-    const size_t es = (dtype == QG_F32) ? 4 : 8;
-    p.lds = (mode == MODE_NL) ? 0 : 2 * (size_t)p.nb * (ty + 8) * nx * es;
-    QG_REQUIRE(p.lds <= 65536, "tile does not fit in LDS");
+    while (tz > 32 && (int64_t)ntile * nchunk * ((nz + tz - 1) / tz) < 4096) tz = (tz + 1) / 2;
+    p.tz = std::max(1, std::min(nz, env_int("QG_PLAN_TZ", tz)));
     return p;
 }
 
-template <class T, int NB, int MODE>
-static void launch_fused_ppt(const Plan& p, dim3 grid, hipStream_t s, const FusedArgs& a,
+// Upload the per-plan staging lists of the projector term (cached per (ty, tz)).
+static void prepare_tile_nl(qg_nonlocal* nl, const Plan& p) {
+    if (nl->plan_ty == p.ty && nl->plan_tz == p.tz) return;
+    TileNL& t = nl->tile;
+    build_tile_nl(nl->gcopy, nl->geo, p.ty, p.tz, t);
+    auto up = [](auto& buf, auto& vec) {
+        if (vec.empty()) vec.push_back({});
+        buf.upload(vec.data(), vec.size());
+    };
+    up(nl->d_wg_atom_off, t.wg_atom_off);
+    up(nl->d_wg_atom, t.wg_atom);
+    up(nl->d_item_off, t.item_off);
+    up(nl->d_item_meta, t.item_meta);
+    up(nl->d_item_beta64, t.item_beta);
+    std::vector<float> b32(t.item_beta.begin(), t.item_beta.end());
+    up(nl->d_item_beta32, b32);
+    // the host copies are not needed once uploaded
+    std::vector<uint32_t>().swap(t.item_meta);
+    std::vector<double>().swap(t.item_beta);
+    nl->plan_ty = p.ty;
+    nl->plan_tz = p.tz;
+}
+
+template <class T, int NB, int MODE, int NJS>
+static void launch_fused_ppt(const Plan& p, dim3 grid, size_t lds, hipStream_t s, const FusedArgs& a,
                              const StencilCoef<T>& sc, const T* V, const T* psi, T* out) {
     switch (p.ppt) {
-#define QG_CASE(PP) \
-    case PP: hipLaunchKernelGGL((fused_hpsi_kernel<T, NB, PP, MODE>), grid, dim3(kBlock), p.lds, s, a, sc, V, psi, out); break;
-        QG_CASE(1) QG_CASE(2) QG_CASE(3) QG_CASE(4) QG_CASE(5) QG_CASE(6) QG_CASE(8)
+#define QG_CASE(PP)                                                                                 \
+    case PP:                                                                                        \
+        hipLaunchKernelGGL((fused_hpsi_kernel<T, NB, PP, MODE, NJS>), grid, dim3(p.nthr), lds, s, a, sc, V, \
+                           psi, out);                                                               \
+        break;
+        QG_CASE(2) QG_CASE(4) QG_CASE(5) QG_CASE(8)
 #undef QG_CASE
         default: throw Error{QG_ERR_ARG, "unsupported points-per-thread"};
     }
@@ -510,9 +525,10 @@ static void launch_fused_ppt(const Plan& p, dim3 grid, hipStream_t s, const Fuse
 template <class T, int MODE>
 static void launch_fused(const qg_grid& g, int nbands, const T* psi, int64_t ld_in, T* out,
                          int64_t ld_out, double coef, int accumulate, const T* V,
-                         qg_nonlocal* nl, hipStream_t s) {
+                         qg_nonlocal* nl, const double* coefs, hipStream_t s) {
     const int dtype = sizeof(T) == 4 ? QG_F32 : QG_F64;
-    Plan p = make_plan(g, dtype, nbands, MODE);
+    Plan p = make_plan(g, dtype, nbands);
+    QG_REQUIRE(g.ng() * (int64_t)sizeof(T) < (1LL << 31), "grid too large for 32-bit buffer offsets");
     FusedArgs a{};
     a.nx = g.n[0]; a.ny = g.n[1]; a.nz = g.n[2];
     a.ty = p.ty; a.tz = p.tz;
@@ -520,89 +536,102 @@ static void launch_fused(const qg_grid& g, int nbands, const T* psi, int64_t ld_
     a.nslab = (g.n[2] + p.tz - 1) / p.tz;
     a.nbands = nbands;
     a.ld_in = ld_in; a.ld_out = ld_out;
-    a.h0 = g.h[0]; a.h1 = g.h[1]; a.h2 = g.h[2];
     a.accumulate = accumulate;
     a.has_nl = 0;
+    size_t lds = (MODE == MODE_NL) ? 0 : 2 * (size_t)p.nb * (p.ty + 8) * (g.n[0] + 8) * sizeof(T);
+    int njs = 4;
     if (nl && nl->geo.ncoef > 0 && MODE != MODE_LAP) {
-        if (nl->list_ty != p.ty) {
-            std::vector<int> off, ids;
-            build_tile_lists(g, nl->geo, p.ty, off, ids);
-            if (ids.empty()) ids.push_back(0);
-            nl->d_list_off.upload(off.data(), off.size());
-            nl->d_list_ids.upload(ids.data(), ids.size());
-            nl->list_ty = p.ty;
-        }
+        prepare_tile_nl(nl, p);
+        const TileNL& t = nl->tile;
+        njs = nl->geo.njs;
         a.has_nl = 1;
-        a.list_off = nl->d_list_off.p;
-        a.list_ids = nl->d_list_ids.p;
-        a.images = nl->d_images.p;
-        a.projs = nl->d_projs.p;
-        a.tables = nl->d_tables.p;
-        a.intervals = nl->d_intervals.p;
-        a.coef = nl->d_coef.p;
+        a.wg_atom_off = nl->d_wg_atom_off.p;
+        a.wg_atom = nl->d_wg_atom.p;
+        a.item_off = nl->d_item_off.p;
+        a.item_meta = nl->d_item_meta.p;
+        a.item_beta = (dtype == QG_F32) ? (const void*)nl->d_item_beta32.p : (const void*)nl->d_item_beta64.p;
+        a.coef = coefs;
+        a.coef_energy = nl->d_coef_energy.p;
+        a.atom_coef_off = nl->d_atom_coef_off.p;
         a.ncoef = nl->geo.ncoef;
+        a.max_atom = std::max(1, t.max_atom);
+        lds += (size_t)p.nb * p.ty * g.n[0] * sizeof(T) + (size_t)a.max_atom * njs * p.nb * sizeof(T);
     }
+    QG_REQUIRE(lds <= 65536, "tile staging does not fit in 64 KiB of LDS");
     const StencilCoef<T> sc = make_coef<T>(g, coef);
     const int nchunk = (nbands + p.nb - 1) / p.nb;
     const int64_t nwg = (int64_t)a.ntile_y * a.nslab * nchunk;
     QG_REQUIRE(nwg < (1LL << 31), "too many workgroups");
     dim3 grid((unsigned)nwg);
-    if (p.nb == 2)
-        launch_fused_ppt<T, 2, MODE>(p, grid, s, a, sc, V, psi, out);
-    else
-        launch_fused_ppt<T, 1, MODE>(p, grid, s, a, sc, V, psi, out);
+    if (p.nb == 2) {
+        if (njs == 4) launch_fused_ppt<T, 2, MODE, 4>(p, grid, lds, s, a, sc, V, psi, out);
+        else launch_fused_ppt<T, 2, MODE, 8>(p, grid, lds, s, a, sc, V, psi, out);
+    } else {
+        if (njs == 4) launch_fused_ppt<T, 1, MODE, 4>(p, grid, lds, s, a, sc, V, psi, out);
+        else launch_fused_ppt<T, 1, MODE, 8>(p, grid, lds, s, a, sc, V, psi, out);
+    }
 }
 
 template <class T>
 static void launch_project(qg_nonlocal* nl, int nbands, const T* psi, int64_t ld, double* coef,
                            hipStream_t s) {
-    const qg_grid& g = nl->gcopy;
     ProjArgs a{};
-    a.nx = g.n[0]; a.ny = g.n[1]; a.nz = g.n[2];
     a.natoms = nl->geo.natoms;
     a.nbands = nbands;
     a.ld = ld;
-    a.h0 = g.h[0]; a.h1 = g.h[1]; a.h2 = g.h[2];
-    a.dv = g.dvol();
+    a.dv = nl->gcopy.dvol();
     a.atom_img_off = nl->d_atom_img_off.p;
     a.atom_coef_off = nl->d_atom_coef_off.p;
-    a.images = nl->d_images.p;
-    a.projs = nl->d_projs.p;
-    a.tables = nl->d_tables.p;
-    a.intervals = nl->d_intervals.p;
+    a.img_tbl_off = nl->d_img_tbl_off.p;
+    a.gidx = nl->d_gidx.p;
+    a.beta = (sizeof(T) == 4) ? (const void*)nl->d_beta32.p : (const void*)nl->d_beta64.p;
     a.coef = coef;
     a.ncoef = nl->geo.ncoef;
-    constexpr int NBA = 4;
+    constexpr int NBA = 8;
     const int nchunk = (nbands + NBA - 1) / NBA;
     dim3 grid((unsigned)((int64_t)a.natoms * nchunk));
-    QG_REQUIRE(nl->geo.max_coef_per_atom <= 8, "at most 8 projector coefficients per atom");
-    if (nl->geo.max_coef_per_atom <= 4)
-        hipLaunchKernelGGL((project_kernel<T, NBA, 4>), grid, dim3(kBlock), 0, s, a, psi);
+    if (nl->geo.njs == 4)
+        hipLaunchKernelGGL((project_kernel<T, NBA, 4>), grid, dim3(kProjBlock), 0, s, a, psi);
     else
-        hipLaunchKernelGGL((project_kernel<T, NBA, 8>), grid, dim3(kBlock), 0, s, a, psi);
+        hipLaunchKernelGGL((project_kernel<T, 4, 8>), dim3((unsigned)((int64_t)a.natoms * ((nbands + 3) / 4))),
+                           dim3(kProjBlock), 0, s, a, psi);
     QG_HIP(hipGetLastError());
 }
 
-// Public-to-internal helpers used by the LOBPCG driver.
+// Bands per chunk so that one chunk of psi stays resident in the Infinity Cache between
+// the projection and the fused apply (QG_CHUNK_MB, default 64 MB).
+static int chunk_bands(const qg_grid& g, int dtype, int nbands) {
+    const int64_t bytes = g.ng() * (dtype == QG_F32 ? 4 : 8);
+    const int64_t budget = (int64_t)env_int("QG_CHUNK_MB", 64) << 20;
+    int64_t c = std::max<int64_t>(1, budget / std::max<int64_t>(1, bytes));
+    return (int)std::min<int64_t>(nbands, c);
+}
+
+// H psi for a band block, used by qg_hamiltonian_apply and the LOBPCG driver.
 void hamiltonian_apply(qg_ham* h, int dtype, int nbands, const void* psi, int64_t ld, void* hpsi,
                        int64_t ld_out, hipStream_t s) {
     const qg_grid& g = h->grid;
     qg_nonlocal* nl = (h->nl && h->nl->geo.ncoef > 0) ? h->nl : nullptr;
+    const int cb = nl ? chunk_bands(g, dtype, nbands) : nbands;
+    if (nl) nl->d_coef.alloc((size_t)std::min(cb, nbands) * nl->geo.ncoef);
     if (h->timing) QG_HIP(hipEventRecord(h->ev[0], s));
-    if (nl) {
-        nl->d_coef.alloc((size_t)nbands * nl->geo.ncoef);
+    for (int b0 = 0; b0 < nbands; b0 += cb) {
+        const int nb = std::min(cb, nbands - b0);
+        const size_t es = (dtype == QG_F32) ? 4 : 8;
+        const char* pin = (const char*)psi + (size_t)b0 * ld * es;
+        char* pout = (char*)hpsi + (size_t)b0 * ld_out * es;
+        if (nl) {
+            if (dtype == QG_F64) launch_project<double>(nl, nb, (const double*)pin, ld, nl->d_coef.p, s);
+            else launch_project<float>(nl, nb, (const float*)pin, ld, nl->d_coef.p, s);
+        }
+        if (h->timing && b0 == 0) QG_HIP(hipEventRecord(h->ev[1], s));
         if (dtype == QG_F64)
-            launch_project<double>(nl, nbands, (const double*)psi, ld, nl->d_coef.p, s);
+            launch_fused<double, MODE_HAM>(g, nb, (const double*)pin, ld, (double*)pout, ld_out, -0.5, 0,
+                                           h->V64.p, nl, nl ? nl->d_coef.p : nullptr, s);
         else
-            launch_project<float>(nl, nbands, (const float*)psi, ld, nl->d_coef.p, s);
+            launch_fused<float, MODE_HAM>(g, nb, (const float*)pin, ld, (float*)pout, ld_out, -0.5, 0,
+                                          h->V32.p, nl, nl ? nl->d_coef.p : nullptr, s);
     }
-    if (h->timing) QG_HIP(hipEventRecord(h->ev[1], s));
-    if (dtype == QG_F64)
-        launch_fused<double, MODE_HAM>(g, nbands, (const double*)psi, ld, (double*)hpsi, ld_out, -0.5, 0,
-                                       h->V64.p, nl, s);
-    else
-        launch_fused<float, MODE_HAM>(g, nbands, (const float*)psi, ld, (float*)hpsi, ld_out, -0.5, 0,
-                                      h->V32.p, nl, s);
     if (h->timing) {
         QG_HIP(hipEventRecord(h->ev[2], s));
         h->timed_valid = true;
@@ -633,10 +662,10 @@ int qg_laplacian8_apply(const qg_grid* g, int dtype, int nbands, const void* psi
         hipStream_t s = as_stream(stream);
         if (dtype == QG_F64)
             launch_fused<double, MODE_LAP>(*g, nbands, (const double*)psi, ld_psi, (double*)out, ld_out, coef,
-                                           accumulate, nullptr, nullptr, s);
+                                           accumulate, nullptr, nullptr, nullptr, s);
         else
             launch_fused<float, MODE_LAP>(*g, nbands, (const float*)psi, ld_psi, (float*)out, ld_out, coef,
-                                          accumulate, nullptr, nullptr, s);
+                                          accumulate, nullptr, nullptr, nullptr, s);
     });
 }
 
@@ -656,6 +685,14 @@ int qg_nonlocal_create(const qg_grid* g, int natoms, const double* xyz, const in
             if (!geo.tables.empty()) nl->d_tables.upload(geo.tables.data(), geo.tables.size());
             if (!geo.images.empty()) nl->d_images.upload(geo.images.data(), geo.images.size());
             if (!geo.intervals.empty()) nl->d_intervals.upload(geo.intervals.data(), geo.intervals.size());
+            if (!geo.gidx.empty()) {
+                nl->d_gidx.upload(geo.gidx.data(), geo.gidx.size());
+                nl->d_beta64.upload(geo.beta.data(), geo.beta.size());
+                std::vector<float> b32(geo.beta.begin(), geo.beta.end());
+                nl->d_beta32.upload(b32.data(), b32.size());
+            }
+            nl->d_img_tbl_off.upload(geo.img_tbl_off.data(), geo.img_tbl_off.size());
+            if (!geo.coef_energy.empty()) nl->d_coef_energy.upload(geo.coef_energy.data(), geo.coef_energy.size());
             nl->d_atom_img_off.upload(geo.atom_img_off.data(), geo.atom_img_off.size());
             nl->d_atom_coef_off.upload(geo.atom_coef_off.data(), geo.atom_coef_off.size());
         } catch (...) {
@@ -700,21 +737,21 @@ int qg_nonlocal_apply(qg_nonlocal* nl, int dtype, int nbands, const void* psi, i
             // no projectors: out = accumulate ? out : 0, through the same kernel
             if (dtype == QG_F64)
                 launch_fused<double, MODE_NL>(nl->gcopy, nbands, (const double*)psi, ld, (double*)hpsi, ld_out,
-                                              0.0, accumulate, nullptr, nullptr, s);
+                                              0.0, accumulate, nullptr, nullptr, nullptr, s);
             else
                 launch_fused<float, MODE_NL>(nl->gcopy, nbands, (const float*)psi, ld, (float*)hpsi, ld_out, 0.0,
-                                             accumulate, nullptr, nullptr, s);
+                                             accumulate, nullptr, nullptr, nullptr, s);
             return;
         }
         nl->d_coef.alloc((size_t)nbands * nl->geo.ncoef);
         if (dtype == QG_F64) {
             launch_project<double>(nl, nbands, (const double*)psi, ld, nl->d_coef.p, s);
             launch_fused<double, MODE_NL>(nl->gcopy, nbands, (const double*)psi, ld, (double*)hpsi, ld_out, 0.0,
-                                          accumulate, nullptr, nl, s);
+                                          accumulate, nullptr, nl, nl->d_coef.p, s);
         } else {
             launch_project<float>(nl, nbands, (const float*)psi, ld, nl->d_coef.p, s);
             launch_fused<float, MODE_NL>(nl->gcopy, nbands, (const float*)psi, ld, (float*)hpsi, ld_out, 0.0,
-                                         accumulate, nullptr, nl, s);
+                                         accumulate, nullptr, nl, nl->d_coef.p, s);
         }
     });
 }

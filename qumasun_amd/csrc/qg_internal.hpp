@@ -90,6 +90,7 @@ struct DevProj {
     int l;
     int kind;          // QG_RADIAL_*
     int coef_rel;      // coefficient index of this channel's m = first within its atom
+    int spec_index;    // index into the caller's qg_proj_spec array
     int table_off;     // offset into the table pool (QG_RADIAL_TABLE)
     int n_table;
     double energy;
@@ -119,17 +120,38 @@ struct NlGeometry {
     std::vector<int2> intervals;   // per row of each image block: [x0, x1) inside the cutoff
     std::vector<int> atom_img_off; // images of atom a: [atom_img_off[a], atom_img_off[a+1])
     std::vector<int> atom_coef_off;
+    std::vector<double> coef_energy;   // per coefficient: the KB energy of its projector
     int ncoef = 0;
     int natoms = 0;
     int max_coef_per_atom = 0;
+    // projector tables on the sphere points (the reference's per-block projector x Ylm
+    // tables), flattened image by image in (z, y, x) order
+    int njs = 4;                       // table stride: coefficient slots per point (4 or 8)
+    std::vector<int64_t> img_tbl_off;  // points of image i: [img_tbl_off[i], img_tbl_off[i+1])
+    std::vector<int64_t> row_tbl_off;  // per interval row: table index of its first point
+    std::vector<double> beta;          // npts * njs, slot j = coefficient within the atom
+    std::vector<int> gidx;             // grid index of every table point
 };
 
 void build_nl_geometry(const qg_grid& g, int natoms, const double* xyz, const int* species,
                        int nspecies, const int* nproj, const qg_proj_spec* specs, NlGeometry& out);
 
-// Per-(y tile, z plane) lists of images for the fused kernel (CSR).
-void build_tile_lists(const qg_grid& g, const NlGeometry& geo, int ty, std::vector<int>& off,
-                      std::vector<int>& ids);
+// Projector data the fused kernel stages per workgroup: for every (y tile, z slab) the
+// images and atoms that reach it, and for every (y tile, z plane) the sphere segments of
+// each row, packed as x0 | x1 << 10 | local image slot << 21 and sorted by tile row.
+struct TileNL {
+    int ty = 0, tz = 0, ntile = 0, nslab = 0;
+    std::vector<int> wg_img_off, wg_img;     // per (tile, slab): global image ids
+    std::vector<int> wg_img_atom;            // parallel to wg_img: local atom slot
+    std::vector<int> wg_atom_off, wg_atom;   // per (tile, slab): global atom ids
+    // work items of the projector apply: one per (image, sphere point), grouped by
+    // (tile, plane), each with its tile-local point index and the workgroup's atom slot
+    std::vector<int64_t> item_off;           // per (tile, plane): [item_off[tp], item_off[tp+1])
+    std::vector<uint32_t> item_meta;         // tile point | atom slot << 16
+    std::vector<double> item_beta;           // njs table values per item
+    int max_img = 0, max_atom = 0, max_items = 0;
+};
+void build_tile_nl(const qg_grid& g, const NlGeometry& geo, int ty, int tz, TileNL& out);
 
 }  // namespace qg
 
@@ -144,13 +166,33 @@ struct qg_nonlocal {
     qg::DevBuf<int> d_atom_img_off;
     qg::DevBuf<int> d_atom_coef_off;
     qg::DevBuf<double> d_coef;   // scratch coefficients (nbands * ncoef)
-    // tile lists cached for one tile height
-    int list_ty = -1;
-    qg::DevBuf<int> d_list_off, d_list_ids;
+    // per-plan staging lists, cached for one (ty, tz)
+    int plan_ty = -1, plan_tz = -1;
+    qg::TileNL tile;
+    qg::DevBuf<int> d_wg_img_off, d_wg_img, d_wg_img_atom, d_wg_atom_off, d_wg_atom;
+    qg::DevBuf<int64_t> d_item_off;
+    qg::DevBuf<uint32_t> d_item_meta;
+    qg::DevBuf<double> d_item_beta64;
+    qg::DevBuf<float> d_item_beta32;
+    qg::DevBuf<double> d_coef_energy;   // KB energy of every coefficient slot
+    // projector tables
+    qg::DevBuf<double> d_beta64;
+    qg::DevBuf<float> d_beta32;
+    qg::DevBuf<int> d_gidx;
+    qg::DevBuf<int64_t> d_img_tbl_off;
 };
+
+namespace qg {
+struct LobpcgState;
+struct PrecondPlan;
+void lobpcg_state_free(LobpcgState*);
+void precond_state_free(PrecondPlan*);
+}  // namespace qg
 
 struct qg_ham {
     qg_grid grid;
+    qg::LobpcgState* lobpcg = nullptr;
+    qg::PrecondPlan* precond = nullptr;
     qg_nonlocal* nl = nullptr;
     qg::DevBuf<double> V64;
     qg::DevBuf<float> V32;
@@ -160,5 +202,7 @@ struct qg_ham {
     ~qg_ham() {
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
+        if (lobpcg) qg::lobpcg_state_free(lobpcg);
+        if (precond) qg::precond_state_free(precond);
     }
 };

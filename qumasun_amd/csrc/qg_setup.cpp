@@ -3,6 +3,7 @@
 // floating-point contraction so sphere membership is decided by exactly the arithmetic of
 // the reference (x = dx*(ix - shift) - Rx, rr = x*x + (y*y + z*z), inside iff !(rc^2 < rr)).
 #include <algorithm>
+#include <cstdint>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -37,6 +38,120 @@ static void sphere_box(const qg_grid& g, const double R[3], double rc, int b[3],
 
 // Smallest n with n*y > x (integer division toward zero, as GetOverlapPeriodic's helper).
 static int first_multiple_above(int x, int y) { return (x >= 0) ? (x / y) + 1 : ((x + 1) / y); }
+
+// RadialGrid2::GetValueByXi (RadialGrid2.h:81-136): linear in the outer intervals, cubic
+// Hermite next to them, quintic Hermite with 4th-order derivative estimates inside.
+static double radial_by_xi_host(double xi, const double* v, int n, double xi_min, double xi_delta) {
+    const double rel = xi - xi_min;
+    const double fl = std::floor(rel / xi_delta);
+    const int i = (int)fl;
+    if (i >= n - 1) return 0.0;
+    if (i == 0 || i == n - 2) {
+        const double t = rel / xi_delta - fl;
+        return v[i] * (1.0 - t) + v[i + 1] * t;
+    }
+    const double t = rel / xi_delta - (double)i;
+    if (1 < i && i < n - 3) {
+        const double fm2 = v[i - 2], fm1 = v[i - 1], f0 = v[i], f1 = v[i + 1], f2 = v[i + 2], f3 = v[i + 3];
+        const double g0 = 2.0 / 3.0 * (f1 - fm1) - 1.0 / 12.0 * (f2 - fm2);
+        const double g1 = 2.0 / 3.0 * (f2 - f0) - 1.0 / 12.0 * (f3 - fm1);
+        const double h0 = 4.0 / 3.0 * (f1 + fm1) - 1.0 / 12.0 * (f2 + fm2) - 5.0 / 2.0 * f0;
+        const double h1 = 4.0 / 3.0 * (f2 + f0) - 1.0 / 12.0 * (f3 + fm1) - 5.0 / 2.0 * f1;
+        const double e0 = f1 - (f0 + g0 + h0 / 2.0);
+        const double e1 = g1 - (g0 + h0);
+        const double e2 = h1 - h0;
+        const double a3 = 10.0 * e0 - 4.0 * e1 + 0.5 * e2;
+        const double a4 = -15.0 * e0 + 7.0 * e1 - e2;
+        const double a5 = f1 - f0 - g0 - h0 / 2.0 - a3 - a4;
+        return f0 + t * (g0 + t * (h0 / 2.0 + t * (a3 + t * (a4 + t * a5))));
+    }
+    const double v0 = v[i], v1 = v[i + 1];
+    const double s1 = (v[i + 2] - v0) / 2.0, s0 = (v1 - v[i - 1]) / 2.0;
+    const double a = s1 + s0 - 2.0 * (v1 - v0);
+    const double b = -s1 - 2.0 * s0 + 3.0 * (v1 - v0);
+    return a * t * t * t + b * t * t + s0 * t + v0;
+}
+
+// p(r) of a channel at squared distance rr inside its cutoff
+// (PseudoPotOperator_mpi_v2.h:154-166 for tables).
+static double radial_host(const qg_proj_spec& s, double rr) {
+    if (s.radial_kind == QG_RADIAL_GAUSSIAN) return s.amplitude * std::exp(-rr / (2.0 * s.sigma * s.sigma));
+    if (s.r_min * s.r_min > rr) return s.table[0];
+    return radial_by_xi_host(0.5 * std::log(rr), s.table, s.n_table, s.xi_min, s.xi_delta);
+}
+
+// beta_j = p(r) Y_lm on every sphere point, image by image, rows in (z, y) order; real
+// Y_lm of RealSphericalHarmonics.h:149-161 (m = -1: y/r, 0: z/r, +1: x/r), Y_lm(r = 0) = 0
+// for l > 0 (SetBlockYlm.h:41-42).
+static void build_tables(const qg_grid& g, const qg_proj_spec* specs, NlGeometry& out) {
+    out.njs = out.max_coef_per_atom <= 4 ? 4 : 8;
+    const int nimg = (int)out.images.size();
+    out.img_tbl_off.assign(nimg + 1, 0);
+    out.row_tbl_off.assign(out.intervals.size(), 0);
+    int64_t np = 0;
+    for (int i = 0; i < nimg; ++i) {
+        const DevImage& im = out.images[i];
+        out.img_tbl_off[i] = np;
+        const int nrows = (im.e[1] - im.b[1]) * (im.e[2] - im.b[2]);
+        for (int r = 0; r < nrows; ++r) {
+            const int2 iv = out.intervals[im.row_base + r];
+            out.row_tbl_off[im.row_base + r] = np;
+            np += std::max(0, iv.y - iv.x);
+        }
+    }
+    out.img_tbl_off[nimg] = np;
+    QG_REQUIRE(np < (1LL << 31), "projector tables too large");
+    out.beta.assign((size_t)np * out.njs, 0.0);
+    out.gidx.assign((size_t)np, 0);
+    const double Y00 = 1.0 / std::sqrt(4.0 * M_PI);
+    const double c1 = std::sqrt(3.0 * 1.0 / (2.0 * 2.0 * M_PI));   // |m| = 1
+    const double c0 = std::sqrt(3.0 / (4.0 * M_PI));               // m = 0
+    auto work = [&](int i0, int i1) {
+        for (int i = i0; i < i1; ++i) {
+            const DevImage& im = out.images[i];
+            const int wy = im.e[1] - im.b[1];
+            for (int iz = im.b[2]; iz < im.e[2]; ++iz) {
+                const double z = g.h[2] * (double)(iz - im.shift[2]) - im.R[2];
+                const double zz = z * z;
+                for (int iy = im.b[1]; iy < im.e[1]; ++iy) {
+                    const double y = g.h[1] * (double)(iy - im.shift[1]) - im.R[1];
+                    const double yy_zz = y * y + zz;
+                    const int row = im.row_base + (iy - im.b[1]) + wy * (iz - im.b[2]);
+                    const int2 iv = out.intervals[row];
+                    int64_t pt = out.row_tbl_off[row];
+                    for (int ix = iv.x; ix < iv.y; ++ix, ++pt) {
+                        const double x = g.h[0] * (double)(ix - im.shift[0]) - im.R[0];
+                        const double rr = x * x + yy_zz;
+                        out.gidx[pt] = ix + g.n[0] * (iy + g.n[1] * iz);
+                        double* b = &out.beta[(size_t)pt * out.njs];
+                        for (int pj = im.proj_begin; pj < im.proj_end; ++pj) {
+                            const DevProj& d = out.projs[pj];
+                            const double rad = radial_host(specs[d.spec_index], rr);
+                            if (d.l == 0) {
+                                b[d.coef_rel] = rad * Y00;
+                            } else {
+                                double ym = 0.0, y0 = 0.0, yp = 0.0;
+                                if (!(1.0e-14 > rr)) {
+                                    const double r = std::sqrt(rr);
+                                    ym = c1 * 1.0 * (y / r);
+                                    y0 = c0 * (z / r) * 1.0;
+                                    yp = c1 * 1.0 * (x / r);
+                                }
+                                b[d.coef_rel] = rad * ym;
+                                b[d.coef_rel + 1] = rad * y0;
+                                b[d.coef_rel + 2] = rad * yp;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    };
+    const int nt = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t) th.emplace_back(work, (int)((int64_t)nimg * t / nt), (int)((int64_t)nimg * (t + 1) / nt));
+    for (auto& t : th) t.join();
+}
 
 void build_nl_geometry(const qg_grid& g, int natoms, const double* xyz, const int* species,
                        int nspecies, const int* nproj, const qg_proj_spec* specs, NlGeometry& out) {
@@ -85,6 +200,7 @@ void build_nl_geometry(const qg_grid& g, int natoms, const double* xyz, const in
                 d.l = p.l;
                 d.kind = p.radial_kind;
                 d.coef_rel = coef_rel[k];
+                d.spec_index = first[s] + k;
                 d.energy = p.energy;
                 d.amp = p.amplitude;
                 d.inv_2s2 = (p.radial_kind == QG_RADIAL_GAUSSIAN) ? 1.0 / (2.0 * p.sigma * p.sigma) : 0.0;
@@ -162,27 +278,86 @@ void build_nl_geometry(const qg_grid& g, int natoms, const double* xyz, const in
         out.atom_img_off[a + 1] = (int)out.images.size();
     }
     out.ncoef = out.atom_coef_off[natoms];
+    out.coef_energy.assign(out.ncoef, 0.0);
+    for (int a = 0; a < natoms; ++a) {
+        const int s = species[a];
+        int c = out.atom_coef_off[a];
+        for (int k = 0; k < nproj[s]; ++k) {
+            const qg_proj_spec& p = specs[first[s] + k];
+            for (int m = 0; m < 2 * p.l + 1; ++m) out.coef_energy[c++] = p.energy;
+        }
+    }
+    build_tables(g, specs, out);
 }
 
-void build_tile_lists(const qg_grid& g, const NlGeometry& geo, int ty, std::vector<int>& off,
-                      std::vector<int>& ids) {
-    const int ntile = (g.n[1] + ty - 1) / ty;
-    const int nz = g.n[2];
-    std::vector<int> cnt((size_t)ntile * nz + 1, 0);
-    auto visit = [&](auto&& f) {
-        for (int i = 0; i < (int)geo.images.size(); ++i) {
-            const DevImage& im = geo.images[i];
-            const int t0 = im.b[1] / ty, t1 = (im.e[1] - 1) / ty;
-            for (int t = t0; t <= t1; ++t)
-                for (int z = im.b[2]; z < im.e[2]; ++z) f((size_t)t * nz + z, i);
+void build_tile_nl(const qg_grid& g, const NlGeometry& geo, int ty, int tz, TileNL& out) {
+    out = TileNL{};
+    const int ny = g.n[1], nz = g.n[2];
+    out.ty = ty;
+    out.tz = tz;
+    out.ntile = (ny + ty - 1) / ty;
+    out.nslab = (nz + tz - 1) / tz;
+    const int nwg = out.ntile * out.nslab;
+    // images / atoms per (tile, slab)
+    std::vector<std::vector<int>> wimg(nwg), watom(nwg);
+    for (int i = 0; i < (int)geo.images.size(); ++i) {
+        const DevImage& im = geo.images[i];
+        const int t0 = im.b[1] / ty, t1 = (im.e[1] - 1) / ty;
+        const int s0 = im.b[2] / tz, s1 = (im.e[2] - 1) / tz;
+        for (int t = t0; t <= t1; ++t)
+            for (int sl = s0; sl <= s1; ++sl) wimg[t * out.nslab + sl].push_back(i);
+    }
+    out.wg_img_off.assign(nwg + 1, 0);
+    out.wg_atom_off.assign(nwg + 1, 0);
+    std::vector<std::vector<int>> local_slot(nwg);
+    for (int w = 0; w < nwg; ++w) {
+        std::vector<int>& at = watom[w];
+        for (int i : wimg[w]) {
+            const int a = geo.images[i].atom;
+            auto it = std::find(at.begin(), at.end(), a);
+            int slot = (int)(it - at.begin());
+            if (it == at.end()) at.push_back(a);
+            out.wg_img.push_back(i);
+            out.wg_img_atom.push_back(slot);
         }
-    };
-    visit([&](size_t slot, int) { ++cnt[slot + 1]; });
-    for (size_t i = 1; i < cnt.size(); ++i) cnt[i] += cnt[i - 1];
-    off = cnt;
-    ids.assign(off.back(), 0);
-    std::vector<int> fill(off.begin(), off.end() - 1);
-    visit([&](size_t slot, int i) { ids[fill[slot]++] = i; });
+        out.wg_img_off[w + 1] = (int)out.wg_img.size();
+        out.wg_atom.insert(out.wg_atom.end(), at.begin(), at.end());
+        out.wg_atom_off[w + 1] = (int)out.wg_atom.size();
+        out.max_img = std::max(out.max_img, (int)wimg[w].size());
+        out.max_atom = std::max(out.max_atom, (int)at.size());
+    }
+    // work items per (tile, plane): every sphere point of every image reaching the plane
+    const int njs = geo.njs;
+    out.item_off.assign((size_t)out.ntile * nz + 1, 0);
+    for (int t = 0; t < out.ntile; ++t) {
+        const int y0 = t * ty, rows = std::min(ty, ny - y0);
+        QG_REQUIRE((int64_t)ty * g.n[0] <= 65535, "tile too large for 16-bit point indices");
+        for (int z = 0; z < nz; ++z) {
+            const int w = t * out.nslab + z / tz;
+            const size_t tp = (size_t)t * nz + z;
+            int64_t items = 0;
+            for (int li = 0; li < (int)wimg[w].size(); ++li) {
+                const DevImage& im = geo.images[wimg[w][li]];
+                if (z < im.b[2] || z >= im.e[2]) continue;
+                const int wy = im.e[1] - im.b[1];
+                const uint32_t slot = (uint32_t)out.wg_img_atom[out.wg_img_off[w] + li];
+                for (int y = std::max(y0, im.b[1]); y < std::min(y0 + rows, im.e[1]); ++y) {
+                    const int row = im.row_base + (y - im.b[1]) + wy * (z - im.b[2]);
+                    const int2 iv = geo.intervals[row];
+                    for (int ix = iv.x; ix < iv.y; ++ix) {
+                        const int64_t ti = geo.row_tbl_off[row] + (ix - iv.x);
+                        out.item_meta.push_back((uint32_t)((y - y0) * g.n[0] + ix) | (slot << 16));
+                        out.item_beta.insert(out.item_beta.end(), geo.beta.begin() + ti * njs,
+                                             geo.beta.begin() + (ti + 1) * njs);
+                        ++items;
+                    }
+                }
+            }
+            out.item_off[tp + 1] = out.item_off[tp] + items;
+            out.max_items = std::max<int>(out.max_items, (int)items);
+        }
+    }
+    QG_REQUIRE(out.max_atom < 65536, "too many atoms per workgroup");
 }
 
 }  // namespace qg
