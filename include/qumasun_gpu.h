@@ -144,6 +144,7 @@ typedef struct qg_lobpcg_opts {
     int precond;             /* QG_PRECOND_* */
     double precond_shift;    /* shift of the kinetic preconditioner (Ha) */
     int verbose;
+    double switch_tol;       /* FP32 modes: residual at which FP64 refinement sweeps take over */
 } qg_lobpcg_opts;
 
 typedef struct qg_lobpcg_stats {
@@ -152,6 +153,7 @@ typedef struct qg_lobpcg_stats {
     int hpsi_calls;          /* band applications of H */
     double max_residual;     /* final max_k ||r_k|| (dVol-normalised) */
     double t_hpsi, t_gram, t_update, t_eig, t_other, t_precond;   /* seconds per sector */
+    int refine_iterations;   /* FP64 refinement iterations of an FP32-mode run */
 } qg_lobpcg_stats;
 
 void qg_lobpcg_default_opts(qg_lobpcg_opts* o);
@@ -176,11 +178,12 @@ int qg_precond_apply(qg_ham* h, int kind, double shift, int dtype, int nbands, v
                      void* stream);
 
 /* Gram product C = A^T B on the matrix cores: A (K x M) and B (K x N) FP32 device blocks
- * (column m at A + m*lda), rounded to BF16, FP32 accumulation, C FP64 device (column
- * stride ldc).  The BF16 Rayleigh-Ritz product of the mixed-precision LOBPCG; replaces
- * blas_DGEMM_t (w_dgemm.h:32-37) inside MakeMatrix_d_blas (lobpcg_d_multi_mpi.h:192-250). */
+ * (column m at A + m*lda), rounded to BF16 (split != 0: x = hi + lo in two BF16 terms,
+ * three MFMAs per step), FP32 accumulation, C FP64 device (column stride ldc).  The
+ * Rayleigh-Ritz product of the mixed-precision LOBPCG; replaces blas_DGEMM_t
+ * (w_dgemm.h:32-37) inside MakeMatrix_d_blas (lobpcg_d_multi_mpi.h:192-250). */
 int qg_gram_bf16(const float* A, const float* B, int64_t K, int M, int N, int64_t lda, int64_t ldb,
-                 double* C, int ldc, void* stream);
+                 double* C, int ldc, int split, void* stream);
 
 /* ------------------------------------------------------- synthetic workloads ---- */
 /* Host-only helpers that build the seeded diamond cells used to exercise this path:

@@ -37,7 +37,7 @@ class LobpcgOpts(ctypes.Structure):
     """qg_lobpcg_opts"""
     _fields_ = [("precision", c_int), ("max_iter", c_int), ("tol", c_double),
                 ("refresh_period", c_int), ("rollback_limit", c_double), ("precond", c_int),
-                ("precond_shift", c_double), ("verbose", c_int)]
+                ("precond_shift", c_double), ("verbose", c_int), ("switch_tol", c_double)]
 
 
 class LobpcgStats(ctypes.Structure):
@@ -45,7 +45,7 @@ class LobpcgStats(ctypes.Structure):
     _fields_ = [("iterations", c_int), ("rollbacks", c_int), ("hpsi_calls", c_int),
                 ("max_residual", c_double), ("t_hpsi", c_double), ("t_gram", c_double),
                 ("t_update", c_double), ("t_eig", c_double), ("t_other", c_double),
-                ("t_precond", c_double)]
+                ("t_precond", c_double), ("refine_iterations", c_int)]
 
 
 # name -> (restype, argtypes)
@@ -76,7 +76,7 @@ SIGNATURES = {
                                   c_int, P(LobpcgStats)]),
     "qg_precond_apply": (c_int, [c_void_p, c_int, c_double, c_int, c_int, c_void_p, c_int64, c_void_p]),
     "qg_gram_bf16": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_int, c_int64, c_int64, c_void_p, c_int,
-                             c_void_p]),
+                             c_int, c_void_p]),
     "qg_synth_diamond_atoms": (c_int, [c_int, c_double, P(c_double)]),
     "qg_synth_vloc": (c_int, [c_int, c_int, c_int, c_double, c_double, c_double, c_int, P(c_double),
                               c_double, c_double, c_double, c_uint32, P(c_double)]),

@@ -2,7 +2,8 @@
 // M, N = band counts (<= a few thousand), on the CDNA4 matrix cores.
 //
 // Inputs are FP32 band-major blocks (column m of A is A + m*lda, K contiguous values); they are
-// rounded to BF16 in registers and multiplied with v_mfma_f32_32x32x16_bf16 (FP32 accumulate).
+// rounded to BF16 in registers and multiplied with v_mfma_f32_32x32x16_bf16 (FP32 accumulate);
+// the split variant carries x = hi + lo (two BF16 terms) through three MFMAs per step.
 // Each wavefront owns a 64 x 64 output tile (2 x 2 MFMA blocks, 64 accumulators per lane) and
 // one K-chunk (split-K); a lane loads 8 consecutive K values of one row of A / column of B per
 // MFMA (32 B of FP32, fully contiguous along K).  Partial tiles go to an FP32 workspace and a
@@ -26,7 +27,32 @@ __device__ __forceinline__ bf16x8 to_bf16x8(f32x4 a, f32x4 b) {
     return r;
 }
 
+// split x = hi + lo with hi = bf16(x), lo = bf16(x - hi): products of (hi + lo) keep ~16
+// mantissa bits, hi*hi + hi*lo + lo*hi drops only the lo*lo term (2^-16 relative)
+__device__ __forceinline__ void split_bf16x8(f32x4 a, f32x4 b, bf16x8& hi, bf16x8& lo) {
+    hi = to_bf16x8(a, b);
+    f32x4 ra, rb;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        ra[j] = a[j] - (float)hi[j];
+        rb[j] = b[j] - (float)hi[j + 4];
+    }
+    lo = to_bf16x8(ra, rb);
+}
+
+template <bool SPLIT>
+__device__ __forceinline__ f32x16 mfma_acc(const bf16x8& ah, const bf16x8& al, const bf16x8& bh, const bf16x8& bl,
+                                           f32x16 c) {
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, c, 0, 0, 0);
+    if constexpr (SPLIT) {
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, c, 0, 0, 0);
+    }
+    return c;
+}
+
 // one wavefront per (64 x 64 tile, K chunk); partial[chunk][n][m] (column-major tiles)
+template <bool SPLIT>
 __global__ __launch_bounds__(64) void gram_bf16_kernel(const float* __restrict__ A, const float* __restrict__ B,
                                                        int64_t K, int M, int N, int64_t lda, int64_t ldb,
                                                        int64_t kchunk, float* __restrict__ partial) {
@@ -59,14 +85,15 @@ __global__ __launch_bounds__(64) void gram_bf16_kernel(const float* __restrict__
             const f32x4* pa1 = reinterpret_cast<const f32x4*>(a1 + kk);
             const f32x4* pb0 = reinterpret_cast<const f32x4*>(b0 + kk);
             const f32x4* pb1 = reinterpret_cast<const f32x4*>(b1 + kk);
-            const bf16x8 fa0 = va0 ? to_bf16x8(pa0[0], pa0[1]) : to_bf16x8(z4, z4);
-            const bf16x8 fa1 = va1 ? to_bf16x8(pa1[0], pa1[1]) : to_bf16x8(z4, z4);
-            const bf16x8 fb0 = vb0 ? to_bf16x8(pb0[0], pb0[1]) : to_bf16x8(z4, z4);
-            const bf16x8 fb1 = vb1 ? to_bf16x8(pb1[0], pb1[1]) : to_bf16x8(z4, z4);
-            c00 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa0, fb0, c00, 0, 0, 0);
-            c01 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa0, fb1, c01, 0, 0, 0);
-            c10 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa1, fb0, c10, 0, 0, 0);
-            c11 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa1, fb1, c11, 0, 0, 0);
+            bf16x8 ha0, la0, ha1, la1, hb0, lb0, hb1, lb1;
+            split_bf16x8(va0 ? pa0[0] : z4, va0 ? pa0[1] : z4, ha0, la0);
+            split_bf16x8(va1 ? pa1[0] : z4, va1 ? pa1[1] : z4, ha1, la1);
+            split_bf16x8(vb0 ? pb0[0] : z4, vb0 ? pb0[1] : z4, hb0, lb0);
+            split_bf16x8(vb1 ? pb1[0] : z4, vb1 ? pb1[1] : z4, hb1, lb1);
+            c00 = mfma_acc<SPLIT>(ha0, la0, hb0, lb0, c00);
+            c01 = mfma_acc<SPLIT>(ha0, la0, hb1, lb1, c01);
+            c10 = mfma_acc<SPLIT>(ha1, la1, hb0, lb0, c10);
+            c11 = mfma_acc<SPLIT>(ha1, la1, hb1, lb1, c11);
         }
     }
     // remainder (and unaligned inputs): element-wise loads with K bounds
@@ -85,12 +112,15 @@ __global__ __launch_bounds__(64) void gram_bf16_kernel(const float* __restrict__
         const f32x4 y0 = {ta1[0], ta1[1], ta1[2], ta1[3]}, y1 = {ta1[4], ta1[5], ta1[6], ta1[7]};
         const f32x4 u0 = {tb0[0], tb0[1], tb0[2], tb0[3]}, u1 = {tb0[4], tb0[5], tb0[6], tb0[7]};
         const f32x4 w0 = {tb1[0], tb1[1], tb1[2], tb1[3]}, w1 = {tb1[4], tb1[5], tb1[6], tb1[7]};
-        const bf16x8 fa0 = to_bf16x8(x0, x1), fa1 = to_bf16x8(y0, y1);
-        const bf16x8 fb0 = to_bf16x8(u0, u1), fb1 = to_bf16x8(w0, w1);
-        c00 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa0, fb0, c00, 0, 0, 0);
-        c01 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa0, fb1, c01, 0, 0, 0);
-        c10 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa1, fb0, c10, 0, 0, 0);
-        c11 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa1, fb1, c11, 0, 0, 0);
+        bf16x8 ha0, la0, ha1, la1, hb0, lb0, hb1, lb1;
+        split_bf16x8(x0, x1, ha0, la0);
+        split_bf16x8(y0, y1, ha1, la1);
+        split_bf16x8(u0, u1, hb0, lb0);
+        split_bf16x8(w0, w1, hb1, lb1);
+        c00 = mfma_acc<SPLIT>(ha0, la0, hb0, lb0, c00);
+        c01 = mfma_acc<SPLIT>(ha0, la0, hb1, lb1, c01);
+        c10 = mfma_acc<SPLIT>(ha1, la1, hb0, lb0, c10);
+        c11 = mfma_acc<SPLIT>(ha1, la1, hb1, lb1, c11);
     }
     // 32x32 accumulator layout: value r of lane l is (row 8*(r/4) + 4*(l/32) + r%4, column l%32)
     float* P = partial + (int64_t)chunk * M * N;
@@ -129,13 +159,17 @@ static int64_t gram_chunks(int64_t K, int M, int N) {
 size_t gram_bf16_workspace(int64_t K, int M, int N) { return (size_t)gram_chunks(K, M, N) * M * N; }
 
 void gram_bf16(const float* A, const float* B, int64_t K, int M, int N, int64_t lda, int64_t ldb, double* C,
-               int ldc, float* partial, hipStream_t s) {
+               int ldc, float* partial, hipStream_t s, bool split) {
     const int64_t nch = gram_chunks(K, M, N);
     int64_t kchunk = (K + nch - 1) / nch;
     kchunk = (kchunk + kGramKStep - 1) / kGramKStep * kGramKStep;
     const int64_t tiles = (int64_t)((M + kGramTile - 1) / kGramTile) * ((N + kGramTile - 1) / kGramTile);
-    hipLaunchKernelGGL(gram_bf16_kernel, dim3((unsigned)(tiles * nch)), dim3(64), 0, s, A, B, K, M, N, lda, ldb,
-                       kchunk, partial);
+    if (split)
+        hipLaunchKernelGGL(gram_bf16_kernel<true>, dim3((unsigned)(tiles * nch)), dim3(64), 0, s, A, B, K, M, N, lda,
+                           ldb, kchunk, partial);
+    else
+        hipLaunchKernelGGL(gram_bf16_kernel<false>, dim3((unsigned)(tiles * nch)), dim3(64), 0, s, A, B, K, M, N, lda,
+                           ldb, kchunk, partial);
     QG_HIP(hipGetLastError());
     const int64_t mn = (int64_t)M * N;
     hipLaunchKernelGGL(gram_reduce_kernel, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, s, partial, (int)nch, M,
@@ -148,12 +182,12 @@ void gram_bf16(const float* A, const float* B, int64_t K, int M, int N, int64_t 
 using namespace qg;
 
 extern "C" int qg_gram_bf16(const float* A, const float* B, int64_t K, int M, int N, int64_t lda, int64_t ldb,
-                            double* C, int ldc, void* stream) {
+                            double* C, int ldc, int split, void* stream) {
     return guarded([&] {
         QG_REQUIRE(A && B && C && K > 0 && M > 0 && N > 0 && lda >= K && ldb >= K && ldc >= M, "bad arguments");
         DevBuf<float> ws;
         ws.alloc(gram_bf16_workspace(K, M, N));
-        gram_bf16(A, B, K, M, N, lda, ldb, C, ldc, ws.p, as_stream(stream));
+        gram_bf16(A, B, K, M, N, lda, ldb, C, ldc, ws.p, as_stream(stream), split != 0);
         QG_HIP(hipStreamSynchronize(as_stream(stream)));
     });
 }

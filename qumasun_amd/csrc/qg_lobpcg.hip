@@ -15,8 +15,8 @@
 // Arithmetic: the vector blocks (X, R, P, AX, AR, AP, T) are FP64 in QG_PREC_F64 and FP32
 // otherwise; every N x N matrix, the Cholesky factorisation and the generalized eigensolve
 // (rocSOLVER dsygvd on the device) are FP64.  Tall-skinny products are plain library GEMMs
-// (rocBLAS) except the QG_PREC_MIXED Gram products, which use this library's BF16 MFMA
-// kernel with FP32 accumulation.
+// (rocBLAS) except the QG_PREC_MIXED H-matrix Gram products, which use this library's
+// split-BF16 MFMA kernel with FP32 accumulation (the overlap matrix stays FP32).
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 #include <algorithm>
@@ -33,7 +33,7 @@ void hamiltonian_apply(qg_ham* h, int dtype, int nbands, const void* psi, int64_
 void precond_apply(qg_ham* h, int kind, double shift, int dtype, int nbands, void* r, int64_t ld,
                    hipStream_t s);
 void gram_bf16(const float* A, const float* B, int64_t K, int M, int N, int64_t lda, int64_t ldb,
-               double* C, int ldc, float* partial, hipStream_t s);
+               double* C, int ldc, float* partial, hipStream_t s, bool split);
 size_t gram_bf16_workspace(int64_t K, int M, int N);
 
 #define QG_BLAS(call)                                                                        \
@@ -100,6 +100,30 @@ __global__ __launch_bounds__(256) void residual_kernel(const T* ax, const T* x, 
     if (threadIdx.x == 0) norm2[k] = s;
 }
 
+// C[m + ldc n] = sum_c W[c][m + N n] (split-K partial Gram products)
+template <class T>
+__global__ void split_reduce_kernel(const T* W, int nch, int N, double* C, int ldc) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t nn = (int64_t)N * N;
+    if (i >= nn) return;
+    double s = 0.0;
+    for (int c = 0; c < nch; ++c) s += (double)W[(int64_t)c * nn + i];
+    C[(i % N) + (int64_t)ldc * (i / N)] = s;
+}
+
+// x_k *= 1 / sqrt(n2[k]) for the columns with n2[k] > 0 (also applied to a partner block y)
+template <class T>
+__global__ void colscale_kernel(T* x, T* y, const double* n2, int64_t ld, int64_t n) {
+    const int k = blockIdx.y;
+    const double v = n2[k];
+    if (!(v > 0.0)) return;
+    const double c = 1.0 / sqrt(v);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        x[(int64_t)k * ld + i] = (T)(x[(int64_t)k * ld + i] * c);
+        if (y) y[(int64_t)k * ld + i] = (T)(y[(int64_t)k * ld + i] * c);
+    }
+}
+
 // C[:, j] (column stride ldc, FP64) = G[:, j] (N x N, FP32)
 __global__ void widen_kernel(const float* G, double* C, int N, int ldc) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -134,6 +158,7 @@ struct LobpcgState {
     DevBuf<int> info;
     DevBuf<double> eig, norm_a, norm_b, res2;
     DevBuf<float> C32, G32, gram_ws;
+    DevBuf<unsigned char> split_ws;
     hipEvent_t ev[2] = {nullptr, nullptr};
     ~LobpcgState() {
         if (blas) rocblas_destroy_handle(blas);
@@ -153,23 +178,46 @@ struct Ops {
     int prec;
     T* blk(DevBuf<unsigned char>& b) { return reinterpret_cast<T*>(b.p); }
 
-    // C (ld ldc, FP64) = A^T B over K rows, A and B are N columns of length K (ld K)
-    void gram(const T* A, const T* B, double* C, int ldc) {
-        const double one = 1.0, zero = 0.0;
-        if constexpr (sizeof(T) == 8) {
-            QG_BLAS(rocblas_dgemm(st.blas, rocblas_operation_transpose, rocblas_operation_none, N, N, (int)K, &one,
-                                  A, (int)K, B, (int)K, &zero, C, ldc));
-        } else {
-            if (prec == QG_PREC_MIXED) {
-                gram_bf16(A, B, K, N, N, K, K, C, ldc, st.gram_ws.p, s);
-            } else {
-                const float onef = 1.f, zerof = 0.f;
-                QG_BLAS(rocblas_sgemm(st.blas, rocblas_operation_transpose, rocblas_operation_none, N, N, (int)K,
-                                      &onef, A, (int)K, B, (int)K, &zerof, st.G32.p, N));
-                // widen into the FP64 block (column stride ldc)
-                hipLaunchKernelGGL(widen_kernel, dim3((N * N + 255) / 256), dim3(256), 0, s, st.G32.p, C, N, ldc);
+    // C (ld ldc, FP64) = A^T B over K rows, A and B are N columns of length K (ld K).
+    // Split-K: the K range is cut into chunks computed as one strided-batched library GEMM
+    // (enough parallel work even for small N), then summed in FP64.
+    // In QG_PREC_MIXED the H blocks (s_matrix = false) use split-BF16 MFMA; the overlap blocks
+    // stay FP32, since S must remain positive definite when R and P become nearly collinear.
+    void gram(const T* A, const T* B, double* C, int ldc, bool s_matrix = false) {
+        if constexpr (sizeof(T) == 4) {
+            if (prec == QG_PREC_MIXED && !s_matrix) {
+                gram_bf16(A, B, K, N, N, K, K, C, ldc, st.gram_ws.p, s, true);
+                return;
             }
         }
+        const int64_t nch = std::max<int64_t>(1, std::min<int64_t>(256, K / 2048));
+        const int64_t kc = K / nch;
+        const int64_t rem = K - nch * kc;
+        const int64_t nn = (int64_t)N * N;
+        T* W = reinterpret_cast<T*>(st.split_ws.p);
+        if constexpr (sizeof(T) == 8) {
+            const double one = 1.0, zero = 0.0;
+            QG_BLAS(rocblas_dgemm_strided_batched(st.blas, rocblas_operation_transpose, rocblas_operation_none, N, N,
+                                                  (int)kc, &one, A, (int)K, kc, B, (int)K, kc, &zero, W, N, nn,
+                                                  (int)nch));
+            if (rem)
+                QG_BLAS(rocblas_dgemm(st.blas, rocblas_operation_transpose, rocblas_operation_none, N, N, (int)rem,
+                                      &one, A + nch * kc, (int)K, B + nch * kc, (int)K, &zero, W + nch * nn, N));
+        } else {
+            const float one = 1.f, zero = 0.f;
+            QG_BLAS(rocblas_sgemm_strided_batched(st.blas, rocblas_operation_transpose, rocblas_operation_none, N, N,
+                                                  (int)kc, &one, A, (int)K, kc, B, (int)K, kc, &zero, W, N, nn,
+                                                  (int)nch));
+            if (rem)
+                QG_BLAS(rocblas_sgemm(st.blas, rocblas_operation_transpose, rocblas_operation_none, N, N, (int)rem,
+                                      &one, A + nch * kc, (int)K, B + nch * kc, (int)K, &zero, W + nch * nn, N));
+        }
+        hipLaunchKernelGGL((split_reduce_kernel<T>), dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, W,
+                           (int)(nch + (rem ? 1 : 0)), N, C, ldc);
+    }
+    static size_t split_workspace(int64_t K, int N) {
+        const int64_t nch = std::max<int64_t>(1, std::min<int64_t>(256, K / 2048));
+        return (size_t)(nch + 1) * N * N * sizeof(T);
     }
     // Out = A * Cm + beta * Out, A: K x N (ld K), Cm: N x N FP64 (ld ldc) / FP32 copy for T = float
     void update(const T* A, const double* Cm, const float* Cm32, int ldc, double beta, T* Out) {
@@ -262,6 +310,7 @@ static void run_lobpcg(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_ho
     st.eig.alloc(N); st.norm_a.alloc(N); st.norm_b.alloc(N); st.res2.alloc(N);
     if (sizeof(T) != 8) { st.C32.alloc(nn9); st.G32.alloc((size_t)N * N); }
     if (o.precision == QG_PREC_MIXED) st.gram_ws.alloc(gram_bf16_workspace(K, N, N));
+    st.split_ws.alloc(Ops<T>::split_workspace(K, N));
     QG_HIP(hipMemsetAsync(st.Sa.p, 0, (nn9 + 4 * (int64_t)N * N) * 8, s));
     QG_HIP(hipMemsetAsync(st.Sb.p, 0, (nn9 + 4 * (int64_t)N * N) * 8, s));
 
@@ -313,7 +362,12 @@ static void run_lobpcg(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_ho
     stats.max_residual = maxres;
 
     const bool precond = o.precond != QG_PRECOND_NONE;
-    const bool direct_xr = !precond;   // DIRECT_S_MATRIX_XR_RP needs R = AX - XE
+    // FP32 working precision: columns of R, P (and AP) are normalised before every Gram
+    // product and X is renormalised after every update, all blocks are recomputed each
+    // iteration, and a norm drift is absorbed instead of rolling back (the reference's
+    // recurrences and rollback assume FP64 arithmetic)
+    const bool robust = sizeof(T) != 8;
+    const bool direct_xr = !precond && !robust;   // DIRECT_S_MATRIX_XR_RP needs R = AX - XE
     auto blkS = [&](DevBuf<double>& S, int bi, int bj) { return S.p + (int64_t)N * bi + (int64_t)N * n3 * bj; };
     bool large_diff = false;
     int iter = 0;
@@ -325,33 +379,43 @@ static void run_lobpcg(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_ho
             sync();
             stats.t_precond += secs(t0, now());
         }
+        const bool use_p = !(first_time && iter == 0);
+        if (robust) {
+            hipLaunchKernelGGL((coldot_kernel<T>), dim3(N), dim3(256), 0, s, R, R, K, K, st.norm_a.p);
+            hipLaunchKernelGGL((colscale_kernel<T>), dim3(grid_for(K) / 4 + 1, N), dim3(256), 0, s, R, (T*)nullptr,
+                               st.norm_a.p, K, K);
+            if (use_p) {
+                hipLaunchKernelGGL((coldot_kernel<T>), dim3(N), dim3(256), 0, s, P, P, K, K, st.norm_b.p);
+                hipLaunchKernelGGL((colscale_kernel<T>), dim3(grid_for(K) / 4 + 1, N), dim3(256), 0, s, P, AP,
+                                   st.norm_b.p, K, K);
+            }
+        }
         hpsi(R, AR);
         // MakeMatrix_d_blas (:192-250)
         t0 = now();
-        const bool full = (iter == 0) || (o.refresh_period > 0 && iter % o.refresh_period == 0) || large_diff;
+        const bool full = robust || (iter == 0) || (o.refresh_period > 0 && iter % o.refresh_period == 0) || large_diff;
         if (full) {
             op.gram(X, AX, blkS(st.Sa, 0, 0), n3);
             op.gram(X, AP, blkS(st.Sa, 0, 2), n3);
             op.gram(P, AP, blkS(st.Sa, 2, 2), n3);
-            op.gram(X, X, blkS(st.Sb, 0, 0), n3);
-            op.gram(X, P, blkS(st.Sb, 0, 2), n3);
-            op.gram(P, P, blkS(st.Sb, 2, 2), n3);
+            op.gram(X, X, blkS(st.Sb, 0, 0), n3, true);
+            op.gram(X, P, blkS(st.Sb, 0, 2), n3, true);
+            op.gram(P, P, blkS(st.Sb, 2, 2), n3, true);
             large_diff = false;
         }
         if (full || !direct_xr) {
-            op.gram(X, R, blkS(st.Sb, 0, 1), n3);
-            op.gram(R, P, blkS(st.Sb, 1, 2), n3);
+            op.gram(X, R, blkS(st.Sb, 0, 1), n3, true);
+            op.gram(R, P, blkS(st.Sb, 1, 2), n3, true);
         }
         op.gram(X, AR, blkS(st.Sa, 0, 1), n3);
         op.gram(R, AR, blkS(st.Sa, 1, 1), n3);
         op.gram(R, AP, blkS(st.Sa, 1, 2), n3);
-        op.gram(R, R, blkS(st.Sb, 1, 1), n3);
+        op.gram(R, R, blkS(st.Sb, 1, 1), n3, true);
         sync();
         stats.t_gram += secs(t0, now());
 
         // Rayleigh-Ritz: dsygvd on the upper nb x nb problem (lapack_DSYGVX, w_dsygvx.h:25-103)
         t0 = now();
-        const bool use_p = !(first_time && iter == 0);
         const int nb = use_p ? n3 : 2 * N;
         copy_sub(nb, nb, st.A3.p, nb, st.Sa.p, n3, s);
         copy_sub(nb, nb, st.B3.p, nb, st.Sb.p, n3, s);
@@ -389,7 +453,7 @@ static void run_lobpcg(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_ho
         stats.t_update += secs(t0, now());
         double maxd = 0.0;
         for (int k = 0; k < N; ++k) maxd = std::max(maxd, std::fabs(1.0 - nx[k]));
-        large_diff = maxd > o.rollback_limit;
+        large_diff = !robust && maxd > o.rollback_limit;
         if (large_diff) {
             // rollback (:1322-1346): P = R, X keeps the previous vectors, leave the loop
             QG_HIP(hipMemcpyAsync(P, R, blk, hipMemcpyDeviceToDevice, s));
@@ -416,6 +480,9 @@ static void run_lobpcg(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_ho
             QG_HIP(hipMemcpyAsync(AP, Tm, blk, hipMemcpyDeviceToDevice, s));
             op.update(AX, Cx, Cx32, nb, 1.0, Tm);
             QG_HIP(hipMemcpyAsync(AX, Tm, blk, hipMemcpyDeviceToDevice, s));
+            if (robust)   // renormalise X (and AX with it); norm_a holds ||x'||^2
+                hipLaunchKernelGGL((colscale_kernel<T>), dim3(grid_for(K) / 4 + 1, N), dim3(256), 0, s, X, AX,
+                                   st.norm_a.p, K, K);
             hipLaunchKernelGGL((residual_kernel<T>), dim3(N), dim3(256), 0, s, AX, X, st.eig.p, R, K, K, st.res2.p);
             sync();
             stats.t_update += secs(t0, now());
@@ -444,7 +511,7 @@ static void run_lobpcg(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_ho
         if (stop) {
             // final normalisation x_j /= sqrt(||x_j||^2 dVol) (:1391-1394)
             std::vector<double> c(N);
-            for (int j = 0; j < N; ++j) c[j] = 1.0 / std::sqrt(nx[j] * dvol);
+            for (int j = 0; j < N; ++j) c[j] = 1.0 / std::sqrt((robust && !final_step ? 1.0 : nx[j]) * dvol);
             DevBuf<double> dc;
             dc.upload(c.data(), N, s);
             for (int j = 0; j < N; ++j) {
@@ -490,6 +557,7 @@ void qg_lobpcg_default_opts(qg_lobpcg_opts* o) {
     o->precond = QG_PRECOND_NONE;
     o->precond_shift = 1.0;
     o->verbose = 0;
+    o->switch_tol = 1.0e-3;
 }
 
 int qg_lobpcg_d_multi(qg_ham* h, const qg_lobpcg_opts* opts, int nbands, double* eig, double* X, double* P,
@@ -507,10 +575,29 @@ int qg_lobpcg_d_multi(qg_ham* h, const qg_lobpcg_opts* opts, int nbands, double*
         QG_REQUIRE(o.max_iter >= 0 && o.rollback_limit > 0.0, "bad iteration options");
         QG_REQUIRE(h->grid.ng() < (1LL << 31), "grid too large for 32-bit BLAS dimensions");
         qg_lobpcg_stats st{};
-        if (o.precision == QG_PREC_F64)
+        if (o.precision == QG_PREC_F64) {
             run_lobpcg<double>(h, o, nbands, eig, X, P, scf_step, st);
-        else
-            run_lobpcg<float>(h, o, nbands, eig, X, P, scf_step, st);
+        } else {
+            // FP32 phase down to switch_tol, then FP64 refinement sweeps down to tol
+            const bool refine = o.tol > 0.0 && o.tol < o.switch_tol;
+            qg_lobpcg_opts o1 = o;
+            if (refine) o1.tol = o.switch_tol;
+            run_lobpcg<float>(h, o1, nbands, eig, X, P, scf_step, st);
+            if (refine && st.max_residual > o.tol && st.iterations < o.max_iter) {
+                qg_lobpcg_opts o2 = o;
+                o2.precision = QG_PREC_F64;
+                o2.max_iter = o.max_iter - st.iterations;
+                qg_lobpcg_stats s2{};
+                run_lobpcg<double>(h, o2, nbands, eig, X, P, 0, s2);
+                st.iterations += s2.iterations;
+                st.rollbacks += s2.rollbacks;
+                st.hpsi_calls += s2.hpsi_calls;
+                st.max_residual = s2.max_residual;
+                st.t_hpsi += s2.t_hpsi; st.t_gram += s2.t_gram; st.t_update += s2.t_update;
+                st.t_eig += s2.t_eig; st.t_other += s2.t_other; st.t_precond += s2.t_precond;
+                st.refine_iterations = s2.iterations;
+            }
+        }
         if (stats) *stats = st;
     });
 }
