@@ -15,7 +15,7 @@ c_int, c_double, c_void_p, c_int64, c_uint32 = (ctypes.c_int, ctypes.c_double, c
 P = ctypes.POINTER
 DP = P(c_double)
 
-__all__ = ["build", "olib", "ref_lib", "OracleNL", "laplacian8", "hamiltonian_block",
+__all__ = ["REF", "REF_SRC", "_d", "build", "olib", "ref_lib", "OracleNL", "laplacian8", "hamiltonian_block",
            "lobpcg_hamiltonian", "lobpcg_dense", "dsygvx", "random_states", "radial_by_xi"]
 
 
