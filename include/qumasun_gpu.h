@@ -134,6 +134,9 @@ int qg_hamiltonian_last_times(qg_ham* h, double* t_project_ms, double* t_fused_m
 /* preconditioners (the reference uses the raw residual, lobpcg_d_multi_mpi.h:982-990) */
 #define QG_PRECOND_NONE 0
 #define QG_PRECOND_KINETIC 1   /* (T + shift)^-1 applied exactly in Fourier space */
+#define QG_PRECOND_LEARNED 2   /* K + K N^T N K with the learned network attached to the
+                                  Hamiltonian (qg_hamiltonian_set_cnn); falls back to
+                                  QG_PRECOND_KINETIC when the residual stalls */
 
 typedef struct qg_lobpcg_opts {
     int precision;           /* QG_PREC_* */
@@ -145,6 +148,8 @@ typedef struct qg_lobpcg_opts {
     double precond_shift;    /* shift of the kinetic preconditioner (Ha) */
     int verbose;
     double switch_tol;       /* FP32 modes: residual at which FP64 refinement sweeps take over */
+    int stall_window;        /* QG_PRECOND_LEARNED: iterations without a new minimum of the
+                                max residual before the kinetic preconditioner takes over */
 } qg_lobpcg_opts;
 
 typedef struct qg_lobpcg_stats {
@@ -154,6 +159,8 @@ typedef struct qg_lobpcg_stats {
     double max_residual;     /* final max_k ||r_k|| (dVol-normalised) */
     double t_hpsi, t_gram, t_update, t_eig, t_other, t_precond;   /* seconds per sector */
     int refine_iterations;   /* FP64 refinement iterations of an FP32-mode run */
+    int fallback_iter;       /* iteration at which a stalled learned preconditioner was
+                                replaced by the kinetic one (-1: never) */
 } qg_lobpcg_stats;
 
 void qg_lobpcg_default_opts(qg_lobpcg_opts* o);
@@ -184,6 +191,44 @@ int qg_precond_apply(qg_ham* h, int kind, double shift, int dtype, int nbands, v
  * (w_dgemm.h:32-37) inside MakeMatrix_d_blas (lobpcg_d_multi_mpi.h:192-250). */
 int qg_gram_bf16(const float* A, const float* B, int64_t K, int M, int N, int64_t lda, int64_t ldb,
                  double* C, int ldc, int split, void* stream);
+
+/* ------------------------------------------------- learned preconditioner ---- */
+/* A 3-layer 3x3x3 convolution network N on the periodic grid (8 hidden channels, inputs: the
+ * residual and the local-potential patch, V entering through tanh gates), used as
+ *   M r = K r + K N^T N K r,   K = (T + shift)^-1,
+ * which is symmetric positive definite for any weights.  Trained on the device by this
+ * library's forward/backward kernels on pairs r -> (T + V - sigma)^-1 r that the solver's own
+ * residual blocks and a preconditioned CG produce.  No counterpart in the reference (which
+ * has no preconditioner, lobpcg_d_multi_mpi.h:982-990). */
+typedef struct qg_cnn qg_cnn;
+int qg_cnn_num_params(void);
+/* Network for Hamiltonian h (its V is the gate input), kinetic shift s, room for max_pairs
+ * training pairs, weights drawn from a seeded normal distribution. */
+int qg_cnn_create(qg_ham* h, double shift, int max_pairs, uint64_t seed, qg_cnn** out);
+int qg_cnn_destroy(qg_cnn* c);
+int qg_cnn_get_params(qg_cnn* c, float* host, int n);
+int qg_cnn_set_params(qg_cnn* c, const float* host, int n);
+/* out = N in (adjoint == 0) or N^T in (adjoint != 0) for nbands FP32 device vectors of Ng;
+ * bf16 != 0 uses the BF16-rounded weights (the preconditioner's), else the FP32 master copy. */
+int qg_cnn_network(qg_cnn* c, int adjoint, int bf16, int nbands, const float* in, float* out,
+                   void* stream);
+/* Loss and gradient for one batch: u, t are nbands x Ng FP32 device vectors, znorm the
+ * normaliser; loss = sum |K N^T N u - t|^2 / znorm, gradient (qg_cnn_num_params floats) to
+ * grad_host. */
+int qg_cnn_loss_grad(qg_cnn* c, int nbands, const float* u, const float* t, double znorm,
+                     double* loss, float* grad_host);
+/* Add training pairs from solver iterates X (nbands x Ng FP64 device) with Ritz values
+ * eig_host: r = H x - e x, z = (T + V - sigma)^-1 r by PCG to relative tolerance tol. */
+int qg_cnn_collect(qg_cnn* c, int nbands, const double* X_dev, const double* eig_host,
+                   double sigma, double tol, int maxit, int* cg_iters);
+int qg_cnn_num_pairs(const qg_cnn* c);
+/* Adam on random batches of the stored pairs; mean loss of the first and last 10% steps. */
+int qg_cnn_train(qg_cnn* c, int steps, int batch, double lr, uint64_t seed,
+                 double* loss_first, double* loss_last);
+/* r <- M r in place (nbands device vectors, ld == Ng, FP32 or FP64). */
+int qg_cnn_apply(qg_cnn* c, int dtype, int nbands, void* r, int64_t ld, void* stream);
+/* Attach (or detach with NULL) the network used by QG_PRECOND_LEARNED. */
+int qg_hamiltonian_set_cnn(qg_ham* h, qg_cnn* c);
 
 /* ------------------------------------------------------- synthetic workloads ---- */
 /* Host-only helpers that build the seeded diamond cells used to exercise this path:

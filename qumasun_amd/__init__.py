@@ -18,10 +18,10 @@ import numpy as np
 
 from . import _native as N
 from ._native import (QG_F32, QG_F64, QG_PREC_F32, QG_PREC_F64, QG_PREC_MIXED, QG_PRECOND_KINETIC,
-                      QG_PRECOND_NONE, QGError, check, dptr, lib)
+                      QG_PRECOND_LEARNED, QG_PRECOND_NONE, QGError, check, dptr, lib)
 
 __all__ = ["Grid", "Laplasian8th", "PseudoPotIntegrator", "Hamiltonian", "Eigen_d_multi",
-           "gaussian_projector", "diamond_system", "QGError"]
+           "LearnedPreconditioner", "gaussian_projector", "diamond_system", "QGError"]
 
 
 def _dtype_code(t):
@@ -250,6 +250,112 @@ def Eigen_d_multi(ham, X, P, scf_step=0, **opts):
                                   ctypes.byref(st)))
     stats = {f: getattr(st, f) for f, _ in N.LobpcgStats._fields_}
     return eig, stats
+
+
+class LearnedPreconditioner:
+    """The learned preconditioner M = K + K N^T N K of a Hamiltonian (qg_cnn_*): a 3-layer
+    3x3x3 convolution network N (8 channels) with the local potential as gate input, trained
+    on the device on r -> (T + V - sigma)^-1 r pairs built from the solver's own iterates."""
+
+    def __init__(self, ham, shift, max_pairs=512, seed=2024):
+        self.ham = ham
+        self.shift = float(shift)
+        h = ctypes.c_void_p()
+        check(lib().qg_cnn_create(ham._h, self.shift, int(max_pairs), int(seed), ctypes.byref(h)))
+        self._h = h
+
+    @staticmethod
+    def num_params():
+        return lib().qg_cnn_num_params()
+
+    def get_params(self):
+        n = self.num_params()
+        a = np.zeros(n, np.float32)
+        check(lib().qg_cnn_get_params(self._h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n))
+        return a
+
+    def set_params(self, a):
+        a = np.ascontiguousarray(a, np.float32)
+        check(lib().qg_cnn_set_params(self._h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), a.size))
+
+    def network(self, x, out, adjoint=False, bf16=False):
+        """out = N x (or N^T x) for float32 device blocks (nbands, Ng)."""
+        import torch
+        torch.cuda.synchronize()
+        xp, nb, _ = _block(x, self.ham.grid.size)
+        op, nb2, _ = _block(out, self.ham.grid.size)
+        if nb != nb2 or x.dtype != torch.float32 or out.dtype != torch.float32:
+            raise QGError("float32 blocks of the same size required")
+        check(lib().qg_cnn_network(self._h, int(adjoint), int(bf16), nb, xp, op, None))
+
+    def loss_grad(self, u, t, znorm):
+        import torch
+        torch.cuda.synchronize()
+        up, nb, _ = _block(u, self.ham.grid.size)
+        tp, nb2, _ = _block(t, self.ham.grid.size)
+        loss = ctypes.c_double()
+        g = np.zeros(self.num_params(), np.float32)
+        check(lib().qg_cnn_loss_grad(self._h, nb, up, tp, float(znorm), ctypes.byref(loss),
+                                     g.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return loss.value, g
+
+    def collect(self, X, eig, sigma, tol=1e-5, maxit=200):
+        """Add training pairs from FP64 iterates X (nbands, Ng) with Ritz values eig."""
+        import torch
+        torch.cuda.synchronize()
+        xp, nb, _ = _block(X, self.ham.grid.size)
+        e = np.ascontiguousarray(eig, np.float64)
+        it = ctypes.c_int()
+        check(lib().qg_cnn_collect(self._h, nb, xp, dptr(e), float(sigma), float(tol), int(maxit),
+                                   ctypes.byref(it)))
+        return it.value
+
+    @property
+    def num_pairs(self):
+        return lib().qg_cnn_num_pairs(self._h)
+
+    def train(self, steps, batch=8, lr=3e-3, seed=7):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        check(lib().qg_cnn_train(self._h, int(steps), int(batch), float(lr), int(seed), ctypes.byref(a),
+                                 ctypes.byref(b)))
+        return a.value, b.value
+
+    def apply(self, r):
+        """r <- M r in place (float32 or float64 device block)."""
+        rp, nb, ld = _block(r, self.ham.grid.size)
+        check(lib().qg_cnn_apply(self._h, _dtype_code(r), nb, rp, ld, _stream()))
+
+    def attach(self):
+        check(lib().qg_hamiltonian_set_cnn(self.ham._h, self._h))
+
+    def detach(self):
+        check(lib().qg_hamiltonian_set_cnn(self.ham._h, None))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            try:
+                lib().qg_cnn_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+
+def fit_learned_preconditioner(ham, nbands, sigma, shift, snapshots=(2, 5, 10, 16, 24), steps=600,
+                               batch=8, lr=3e-3, seed=2024):
+    """Train a LearnedPreconditioner on pairs from this Hamiltonian's own solver iterates:
+    kinetic-preconditioned LOBPCG from a seeded random start, stopped after each number of
+    iterations in `snapshots`; every snapshot's Ritz vectors give nbands residuals."""
+    import torch
+    net = LearnedPreconditioner(ham, shift, max_pairs=nbands * len(snapshots))
+    X0 = random_states(ham.grid.size, nbands, ham.grid.dvol, seed=SEED_STATES + 1)
+    cg = []
+    for k in snapshots:
+        X = torch.from_numpy(X0.copy()).cuda()
+        P = torch.zeros_like(X)
+        e, _ = Eigen_d_multi(ham, X, P, 0, max_iter=int(k), precond=QG_PRECOND_KINETIC, precond_shift=shift)
+        cg.append(net.collect(X, e, sigma))
+    first, last = net.train(steps, batch=batch, lr=lr, seed=seed)
+    return net, dict(pairs=net.num_pairs, cg_iters=cg, loss_first=first, loss_last=last)
 
 
 # ---------------------------------------------------------------- synthetic cells ----

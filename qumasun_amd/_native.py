@@ -19,9 +19,10 @@ QG_PREC_MIXED = 1
 QG_PREC_F32 = 2
 QG_PRECOND_NONE = 0
 QG_PRECOND_KINETIC = 1
+QG_PRECOND_LEARNED = 2
 
-c_int, c_double, c_void_p, c_int64, c_uint32 = (ctypes.c_int, ctypes.c_double, ctypes.c_void_p,
-                                                 ctypes.c_int64, ctypes.c_uint32)
+c_int, c_double, c_void_p, c_int64, c_uint32, c_uint64 = (ctypes.c_int, ctypes.c_double, ctypes.c_void_p,
+                                                           ctypes.c_int64, ctypes.c_uint32, ctypes.c_uint64)
 P = ctypes.POINTER
 
 
@@ -37,7 +38,8 @@ class LobpcgOpts(ctypes.Structure):
     """qg_lobpcg_opts"""
     _fields_ = [("precision", c_int), ("max_iter", c_int), ("tol", c_double),
                 ("refresh_period", c_int), ("rollback_limit", c_double), ("precond", c_int),
-                ("precond_shift", c_double), ("verbose", c_int), ("switch_tol", c_double)]
+                ("precond_shift", c_double), ("verbose", c_int), ("switch_tol", c_double),
+                ("stall_window", c_int)]
 
 
 class LobpcgStats(ctypes.Structure):
@@ -45,7 +47,7 @@ class LobpcgStats(ctypes.Structure):
     _fields_ = [("iterations", c_int), ("rollbacks", c_int), ("hpsi_calls", c_int),
                 ("max_residual", c_double), ("t_hpsi", c_double), ("t_gram", c_double),
                 ("t_update", c_double), ("t_eig", c_double), ("t_other", c_double),
-                ("t_precond", c_double), ("refine_iterations", c_int)]
+                ("t_precond", c_double), ("refine_iterations", c_int), ("fallback_iter", c_int)]
 
 
 # name -> (restype, argtypes)
@@ -77,6 +79,19 @@ SIGNATURES = {
     "qg_precond_apply": (c_int, [c_void_p, c_int, c_double, c_int, c_int, c_void_p, c_int64, c_void_p]),
     "qg_gram_bf16": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_int, c_int64, c_int64, c_void_p, c_int,
                              c_int, c_void_p]),
+    "qg_cnn_num_params": (c_int, []),
+    "qg_cnn_create": (c_int, [c_void_p, c_double, c_int, c_uint64, P(c_void_p)]),
+    "qg_cnn_destroy": (c_int, [c_void_p]),
+    "qg_cnn_get_params": (c_int, [c_void_p, P(ctypes.c_float), c_int]),
+    "qg_cnn_set_params": (c_int, [c_void_p, P(ctypes.c_float), c_int]),
+    "qg_cnn_network": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    "qg_cnn_loss_grad": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_double, P(c_double),
+                                 P(ctypes.c_float)]),
+    "qg_cnn_collect": (c_int, [c_void_p, c_int, c_void_p, P(c_double), c_double, c_double, c_int, P(c_int)]),
+    "qg_cnn_num_pairs": (c_int, [c_void_p]),
+    "qg_cnn_train": (c_int, [c_void_p, c_int, c_int, c_double, c_uint64, P(c_double), P(c_double)]),
+    "qg_cnn_apply": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, c_void_p]),
+    "qg_hamiltonian_set_cnn": (c_int, [c_void_p, c_void_p]),
     "qg_synth_diamond_atoms": (c_int, [c_int, c_double, P(c_double)]),
     "qg_synth_vloc": (c_int, [c_int, c_int, c_int, c_double, c_double, c_double, c_int, P(c_double),
                               c_double, c_double, c_double, c_uint32, P(c_double)]),

@@ -638,6 +638,19 @@ void hamiltonian_apply(qg_ham* h, int dtype, int nbands, const void* psi, int64_
     }
 }
 
+// (T + V) psi: the local part of H (no projectors), used by the learned preconditioner's
+// training targets.
+void local_hamiltonian_apply(qg_ham* h, int dtype, int nbands, const void* psi, int64_t ld, void* out,
+                             int64_t ld_out, hipStream_t s) {
+    const qg_grid& g = h->grid;
+    if (dtype == QG_F64)
+        launch_fused<double, MODE_HAM>(g, nbands, (const double*)psi, ld, (double*)out, ld_out, -0.5, 0, h->V64.p,
+                                       nullptr, nullptr, s);
+    else
+        launch_fused<float, MODE_HAM>(g, nbands, (const float*)psi, ld, (float*)out, ld_out, -0.5, 0, h->V32.p,
+                                      nullptr, nullptr, s);
+}
+
 }  // namespace qg
 
 using namespace qg;

@@ -191,6 +191,7 @@ void precond_state_free(PrecondPlan*);
 
 struct qg_ham {
     qg_grid grid;
+    qg_cnn* cnn = nullptr;   // learned preconditioner (not owned)
     qg::LobpcgState* lobpcg = nullptr;
     qg::PrecondPlan* precond = nullptr;
     qg_nonlocal* nl = nullptr;

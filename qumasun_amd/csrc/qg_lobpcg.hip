@@ -32,6 +32,7 @@ void hamiltonian_apply(qg_ham* h, int dtype, int nbands, const void* psi, int64_
                        int64_t ld_out, hipStream_t s);
 void precond_apply(qg_ham* h, int kind, double shift, int dtype, int nbands, void* r, int64_t ld,
                    hipStream_t s);
+void cnn_precond_apply(qg_cnn* c, int dtype, int nbands, void* r, int64_t ld, hipStream_t s);
 void gram_bf16(const float* A, const float* B, int64_t K, int M, int N, int64_t lda, int64_t ldb,
                double* C, int ldc, float* partial, hipStream_t s, bool split);
 size_t gram_bf16_workspace(int64_t K, int M, int N);
@@ -285,7 +286,7 @@ static void next_matrix(rocblas_handle h, hipStream_t s, int N, double* S, const
 
 template <class T>
 static void run_lobpcg(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_host, double* X64, double* P64,
-                       int scf_step, qg_lobpcg_stats& stats) {
+                       int scf_step, qg_lobpcg_stats& stats, int iter_offset = 0) {
     const qg_grid& g = h->grid;
     const int64_t K = g.ng();
     const double dvol = g.dvol();
@@ -372,10 +373,18 @@ static void run_lobpcg(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_ho
     bool large_diff = false;
     int iter = 0;
     bool finished = false;
+    // learned preconditioner: replaced by the kinetic one when the max residual has not
+    // reached a new minimum for stall_window iterations
+    bool learned = o.precond == QG_PRECOND_LEARNED && h->cnn && stats.fallback_iter < 0;
+    double best_res = maxres;
+    int best_iter = 0;
     for (iter = 0; iter < o.max_iter && !finished; ++iter) {
         auto t0 = now();
         if (precond) {
-            precond_apply(h, o.precond, o.precond_shift, dtype, N, R, K, s);
+            if (learned)
+                cnn_precond_apply(h->cnn, dtype, N, R, K, s);
+            else
+                precond_apply(h, QG_PRECOND_KINETIC, o.precond_shift, dtype, N, R, K, s);
             sync();
             stats.t_precond += secs(t0, now());
         }
@@ -507,6 +516,15 @@ static void run_lobpcg(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_ho
                 std::fprintf(stderr, "lobpcg it %d: e0 %.10f e%d %.10f max|r| %.3e\n", iter, ev[0], N - 1, ev[N - 1],
                              maxres);
             if (o.tol > 0.0 && maxres <= o.tol) stop = true;
+            if (maxres < best_res) {
+                best_res = maxres;
+                best_iter = iter;
+            } else if (learned && o.stall_window > 0 && iter - best_iter >= o.stall_window) {
+                learned = false;
+                stats.fallback_iter = iter_offset + iter + 1;
+                if (o.verbose) std::fprintf(stderr, "lobpcg: learned preconditioner stalled, kinetic from iteration %d\n",
+                                            stats.fallback_iter);
+            }
         }
         if (stop) {
             // final normalisation x_j /= sqrt(||x_j||^2 dVol) (:1391-1394)
@@ -558,6 +576,7 @@ void qg_lobpcg_default_opts(qg_lobpcg_opts* o) {
     o->precond_shift = 1.0;
     o->verbose = 0;
     o->switch_tol = 1.0e-3;
+    o->stall_window = 5;
 }
 
 int qg_lobpcg_d_multi(qg_ham* h, const qg_lobpcg_opts* opts, int nbands, double* eig, double* X, double* P,
@@ -571,10 +590,13 @@ int qg_lobpcg_d_multi(qg_ham* h, const qg_lobpcg_opts* opts, int nbands, double*
         if (opts) o = *opts;
         QG_REQUIRE(o.precision == QG_PREC_F64 || o.precision == QG_PREC_F32 || o.precision == QG_PREC_MIXED,
                    "unknown precision mode");
-        QG_REQUIRE(o.precond == QG_PRECOND_NONE || o.precond == QG_PRECOND_KINETIC, "unknown preconditioner");
+        QG_REQUIRE(o.precond == QG_PRECOND_NONE || o.precond == QG_PRECOND_KINETIC || o.precond == QG_PRECOND_LEARNED,
+                   "unknown preconditioner");
+        QG_REQUIRE(o.precond != QG_PRECOND_LEARNED || h->cnn, "QG_PRECOND_LEARNED needs a network (qg_hamiltonian_set_cnn)");
         QG_REQUIRE(o.max_iter >= 0 && o.rollback_limit > 0.0, "bad iteration options");
         QG_REQUIRE(h->grid.ng() < (1LL << 31), "grid too large for 32-bit BLAS dimensions");
         qg_lobpcg_stats st{};
+        st.fallback_iter = -1;
         if (o.precision == QG_PREC_F64) {
             run_lobpcg<double>(h, o, nbands, eig, X, P, scf_step, st);
         } else {
@@ -588,7 +610,9 @@ int qg_lobpcg_d_multi(qg_ham* h, const qg_lobpcg_opts* opts, int nbands, double*
                 o2.precision = QG_PREC_F64;
                 o2.max_iter = o.max_iter - st.iterations;
                 qg_lobpcg_stats s2{};
-                run_lobpcg<double>(h, o2, nbands, eig, X, P, 0, s2);
+                s2.fallback_iter = st.fallback_iter;
+                run_lobpcg<double>(h, o2, nbands, eig, X, P, 0, s2, st.iterations);
+                st.fallback_iter = s2.fallback_iter;
                 st.iterations += s2.iterations;
                 st.rollbacks += s2.rollbacks;
                 st.hpsi_calls += s2.hpsi_calls;
