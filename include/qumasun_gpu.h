@@ -230,6 +230,29 @@ int qg_cnn_apply(qg_cnn* c, int dtype, int nbands, void* r, int64_t ld, void* st
 /* Attach (or detach with NULL) the network used by QG_PRECOND_LEARNED. */
 int qg_hamiltonian_set_cnn(qg_ham* h, qg_cnn* c);
 
+/* ------------------------------------------------- batches of uneven cells ---- */
+/* H psi for ncells independent cells (each its own Hamiltonian, grid, atoms and band count;
+ * psi[c], hpsi[c] band-major device blocks with ld == Ng of that cell) in persistent launches:
+ * one projection launch and one fused stencil/V/projector launch per round of Infinity-Cache
+ * sized band chunks, covering every cell.  Same results as qg_hamiltonian_apply per cell.
+ * The cells must share the projector slot count and the points-per-thread plan.  No
+ * counterpart in the reference (which runs one cell per process). */
+int qg_batch_hamiltonian_apply(int ncells, qg_ham* const* hams, int dtype, const int* nbands,
+                               const void* const* psi, void* const* hpsi, void* stream);
+/* Autotuner: times the fused H psi of h on nbands zero bands for candidate tile shapes (rows
+ * per tile x points per thread; fixed_ppt != 0 pins the latter) and records the fastest for
+ * this grid shape and dtype; later applications on grids of that shape use it (unless the
+ * QG_PLAN_TY / QG_PLAN_PPT environment overrides are set). */
+int qg_autotune_hamiltonian(qg_ham* h, int dtype, int nbands, int fixed_ppt, int* ty, int* ppt,
+                            double* best_ms);
+/* LOBPCG for ncells independent cells (arguments per cell as in qg_lobpcg_d_multi).  batched
+ * != 0: the cells are solved together, each on its own host thread and stream, and the H psi
+ * requests of all cells meet and run as one qg_batch_hamiltonian_apply per round; batched ==
+ * 0: one cell after another.  stats (may be NULL) receives ncells records. */
+int qg_batch_lobpcg(int ncells, qg_ham* const* hams, const qg_lobpcg_opts* opts, const int* nbands,
+                    double* const* eigvals_host, double* const* X_dev, double* const* P_dev,
+                    int scf_step, qg_lobpcg_stats* stats, int batched);
+
 /* ------------------------------------------------------- synthetic workloads ---- */
 /* Host-only helpers that build the seeded diamond cells used to exercise this path:
  * ncell^3 diamond supercell of the 8-atom cubic cell with lattice a (bohr) on an

@@ -358,6 +358,63 @@ def fit_learned_preconditioner(ham, nbands, sigma, shift, snapshots=(2, 5, 10, 1
     return net, dict(pairs=net.num_pairs, cg_iters=cg, loss_first=first, loss_last=last)
 
 
+def batch_apply(hams, psis, outs):
+    """H psi for independent cells in persistent batched launches (qg_batch_hamiltonian_apply)."""
+    import torch
+    n = len(hams)
+    if not (len(psis) == len(outs) == n) or n == 0:
+        raise QGError("one psi and one out block per Hamiltonian")
+    dt = _dtype_code(psis[0])
+    H = (ctypes.c_void_p * n)(*[h._h.value for h in hams])
+    nb = (ctypes.c_int * n)()
+    pi = (ctypes.c_void_p * n)()
+    po = (ctypes.c_void_p * n)()
+    for c, (h, x, y) in enumerate(zip(hams, psis, outs)):
+        xp, b, _ = _block(x, h.grid.size)
+        yp, b2, _ = _block(y, h.grid.size)
+        if b != b2 or _dtype_code(x) != dt or _dtype_code(y) != dt:
+            raise QGError("block mismatch in batch")
+        nb[c], pi[c], po[c] = b, xp.value, yp.value
+    check(lib().qg_batch_hamiltonian_apply(n, H, dt, nb, pi, po, _stream()))
+    return outs
+
+
+def autotune(ham, dtype="f32", nbands=32, fixed_ppt=0):
+    """Pick the fused kernel's tile shape for this grid by timing candidates on the device;
+    returns (rows per tile, points per thread, ms of the best application)."""
+    ty, ppt, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+    check(lib().qg_autotune_hamiltonian(ham._h, QG_F32 if dtype == "f32" else QG_F64, int(nbands), int(fixed_ppt),
+                                        ctypes.byref(ty), ctypes.byref(ppt), ctypes.byref(ms)))
+    return ty.value, ppt.value, ms.value
+
+
+def batch_eigen(hams, Xs, Ps, scf_step=0, batched=True, **opts):
+    """LOBPCG for independent cells: batched=True runs them together with their H psi applied
+    in persistent batched launches; False runs one cell after another."""
+    import torch
+    torch.cuda.synchronize()
+    n = len(hams)
+    H = (ctypes.c_void_p * n)(*[h._h.value for h in hams])
+    nb = (ctypes.c_int * n)()
+    xs = (ctypes.c_void_p * n)()
+    ps = (ctypes.c_void_p * n)()
+    eigs = [None] * n
+    ep = (ctypes.POINTER(ctypes.c_double) * n)()
+    for c, (h, X, P) in enumerate(zip(hams, Xs, Ps)):
+        xp, b, _ = _block(X, h.grid.size)
+        pp, b2, _ = _block(P, h.grid.size)
+        if b != b2 or X.dtype != torch.float64 or P.dtype != torch.float64:
+            raise QGError("X and P must be float64 blocks of the same size")
+        nb[c], xs[c], ps[c] = b, xp.value, pp.value
+        eigs[c] = np.zeros(b)
+        ep[c] = dptr(eigs[c])
+    o = lobpcg_opts(**opts)
+    st = (N.LobpcgStats * n)()
+    check(lib().qg_batch_lobpcg(n, H, ctypes.byref(o), nb, ep, xs, ps, int(scf_step), st, int(bool(batched))))
+    stats = [{f: getattr(st[c], f) for f, _ in N.LobpcgStats._fields_} for c in range(n)]
+    return eigs, stats
+
+
 # ---------------------------------------------------------------- synthetic cells ----
 DIAMOND_A = 6.74          # bohr, cubic diamond cell edge of the workloads
 SEED_STATES = 123456789   # qumasun_mpi.initialize.h:199
@@ -377,6 +434,29 @@ def diamond_system(ncell, n_per_cell, a=DIAMOND_A, zion=4.0, rc=0.8, noise=0.05,
     check(lib().qg_synth_vloc(n, n, n, L, L, L, nat, dptr(xyz), zion, rc, noise, seed, dptr(V)))
     projs = [[gaussian_projector(0, e_s, sigma, cutoff), gaussian_projector(1, e_p, sigma, cutoff)]]
     return dict(n=n, L=L, h=L / n, positions=xyz.reshape(nat, 3), V=V, projectors=projs)
+
+
+def diamond_cell(dims, n_per_cell=40, a=DIAMOND_A, zion=4.0, rc=0.8, noise=0.05, seed=SEED_VLOC, sigma=0.6,
+                 cutoff=2.4, e_s=1.0, e_p=-0.5):
+    """A (dims[0] x dims[1] x dims[2]) diamond supercell (8 atoms per cubic cell) on an
+    (n_per_cell * dims) grid, with the same potential and projectors as diamond_system."""
+    base = np.zeros(24)
+    check(lib().qg_synth_diamond_atoms(1, a, dptr(base)))
+    base = base.reshape(8, 3)
+    pos = []
+    for k in range(dims[2]):
+        for j in range(dims[1]):
+            for i in range(dims[0]):
+                pos.append(base + a * np.array([i, j, k]))
+    xyz = np.ascontiguousarray(np.concatenate(pos).reshape(-1))
+    L = [a * d for d in dims]
+    n = [n_per_cell * d for d in dims]
+    V = np.zeros(n[0] * n[1] * n[2])
+    check(lib().qg_synth_vloc(n[0], n[1], n[2], L[0], L[1], L[2], len(xyz) // 3, dptr(xyz), zion, rc, noise, seed,
+                              dptr(V)))
+    projs = [[gaussian_projector(0, e_s, sigma, cutoff), gaussian_projector(1, e_p, sigma, cutoff)]]
+    return dict(n=tuple(n), L=tuple(L), h=tuple(L[d] / n[d] for d in range(3)), positions=xyz.reshape(-1, 3), V=V,
+                projectors=projs)
 
 
 def random_states(ng, nbands, dvol, seed=SEED_STATES):

@@ -92,6 +92,11 @@ SIGNATURES = {
     "qg_cnn_train": (c_int, [c_void_p, c_int, c_int, c_double, c_uint64, P(c_double), P(c_double)]),
     "qg_cnn_apply": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, c_void_p]),
     "qg_hamiltonian_set_cnn": (c_int, [c_void_p, c_void_p]),
+    "qg_batch_hamiltonian_apply": (c_int, [c_int, P(c_void_p), c_int, P(c_int), P(c_void_p), P(c_void_p),
+                                           c_void_p]),
+    "qg_autotune_hamiltonian": (c_int, [c_void_p, c_int, c_int, c_int, P(c_int), P(c_int), P(c_double)]),
+    "qg_batch_lobpcg": (c_int, [c_int, P(c_void_p), P(LobpcgOpts), P(c_int), P(P(c_double)), P(c_void_p),
+                                P(c_void_p), c_int, P(LobpcgStats), c_int]),
     "qg_synth_diamond_atoms": (c_int, [c_int, c_double, P(c_double)]),
     "qg_synth_vloc": (c_int, [c_int, c_int, c_int, c_double, c_double, c_double, c_int, P(c_double),
                               c_double, c_double, c_double, c_uint32, P(c_double)]),

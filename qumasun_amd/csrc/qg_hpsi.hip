@@ -24,6 +24,12 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+#include <type_traits>
 #include "qg_internal.hpp"
 
 namespace qg {
@@ -96,10 +102,9 @@ constexpr int kItemPT = 4;  // projector work items per thread per round
 // Fused kernel.  T = float/double, NB bands per workgroup, PPT points per thread, NJS table
 // slots per projector item.  blockDim.x = threads (multiple of 64, <= 512).
 template <class T, int NB, int PPT, int MODE, int NJS>
-__global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 3 : 4)))
-void fused_hpsi_kernel(FusedArgs a, StencilCoef<T> sc, const T* __restrict__ V,
-                       const T* __restrict__ psi, T* __restrict__ out) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
+__device__ __forceinline__ void fused_body(const FusedArgs& a, const StencilCoef<T>& sc, const T* __restrict__ V,
+                                           const T* __restrict__ psi, T* __restrict__ out, const int wg,
+                                           unsigned char* smem_raw) {
     constexpr int S = sizeof(T);
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int nthr = blockDim.x;
@@ -117,7 +122,6 @@ void fused_hpsi_kernel(FusedArgs a, StencilCoef<T> sc, const T* __restrict__ V,
         ec_l = reinterpret_cast<T*>(lp);
     }
 
-    const int wg = xcd_remap(blockIdx.x, gridDim.x);
     // logical order: band chunk fastest, so the workgroups of one (tile, slab) run back to
     // back on one XCD and find V and the projector items in its L2; halo rows and the
     // z warm-up planes of neighbouring tiles come from the Infinity Cache
@@ -341,6 +345,45 @@ void fused_hpsi_kernel(FusedArgs a, StencilCoef<T> sc, const T* __restrict__ V,
     }
 }
 
+template <class T, int NB, int PPT, int MODE, int NJS>
+__global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 3 : 4)))
+void fused_hpsi_kernel(FusedArgs a, StencilCoef<T> sc, const T* __restrict__ V, const T* __restrict__ psi,
+                       T* __restrict__ out) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    fused_body<T, NB, PPT, MODE, NJS>(a, sc, V, psi, out, xcd_remap(blockIdx.x, gridDim.x), smem_raw);
+}
+
+// One cell of a batched application (independent cells of different sizes in one launch).
+template <class T>
+struct BatchCell {
+    FusedArgs a;
+    StencilCoef<T> sc;
+    const T* V;
+    const T* psi;
+    T* out;
+};
+
+// Persistent form over a batch of cells: the workgroups of all cells form one logical range
+// (cell c owns [prefix[c], prefix[c+1])); each hardware workgroup walks it with a stride of
+// the grid size, starting from its XCD-contiguous logical id.
+template <class T, int NB, int PPT, int MODE, int NJS>
+__global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 3 : 4)))
+void fused_batch_kernel(const BatchCell<T>* __restrict__ cells, const int* __restrict__ prefix, int ncells) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int total = prefix[ncells];
+    for (int w = xcd_remap(blockIdx.x, gridDim.x); w < total; w += gridDim.x) {
+        int lo = 0, hi = ncells - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (prefix[mid] <= w) lo = mid;
+            else hi = mid - 1;
+        }
+        const BatchCell<T>& c = cells[lo];
+        fused_body<T, NB, PPT, MODE, NJS>(c.a, c.sc, c.V, c.psi, c.out, w - prefix[lo], smem_raw);
+        __syncthreads();   // LDS is reused by the next item
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // Projector coefficients: coef[b*ncoef + cbase(a) + j] = dV * sum_i beta_j(i) psi_b(i).
 // One workgroup per (atom, chunk of NBA bands), streaming the atom's table entries.
@@ -361,10 +404,10 @@ struct ProjArgs {
 constexpr int kProjBlock = 256;
 
 template <class T, int NBA, int NJS>
-__global__ __launch_bounds__(kProjBlock) void project_kernel(ProjArgs a, const T* __restrict__ psi) {
-    __shared__ double red[kProjBlock / 64][NJS * NBA];
-    const int atom = blockIdx.x % a.natoms;
-    const int chunk = blockIdx.x / a.natoms;
+__device__ __forceinline__ void project_body(const ProjArgs& a, const T* __restrict__ psi, const int item,
+                                             double (*red)[NJS * NBA]) {
+    const int atom = item % a.natoms;
+    const int chunk = item / a.natoms;
     const int b0 = chunk * NBA;
     const int tid = threadIdx.x;
     const T* beta = reinterpret_cast<const T*>(a.beta);
@@ -425,6 +468,36 @@ __global__ __launch_bounds__(kProjBlock) void project_kernel(ProjArgs a, const T
     }
 }
 
+template <class T, int NBA, int NJS>
+__global__ __launch_bounds__(kProjBlock) void project_kernel(ProjArgs a, const T* __restrict__ psi) {
+    __shared__ double red[kProjBlock / 64][NJS * NBA];
+    project_body<T, NBA, NJS>(a, psi, blockIdx.x, red);
+}
+
+template <class T>
+struct ProjCell {
+    ProjArgs a;
+    const T* psi;
+};
+
+// Persistent projection over a batch of cells: items (cell, atom, band chunk) in one range.
+template <class T, int NBA, int NJS>
+__global__ __launch_bounds__(kProjBlock) void project_batch_kernel(const ProjCell<T>* __restrict__ cells,
+                                                                   const int* __restrict__ prefix, int ncells) {
+    __shared__ double red[kProjBlock / 64][NJS * NBA];
+    const int total = prefix[ncells];
+    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+        int lo = 0, hi = ncells - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (prefix[mid] <= w) lo = mid;
+            else hi = mid - 1;
+        }
+        project_body<T, NBA, NJS>(cells[lo].a, cells[lo].psi, w - prefix[lo], red);
+        __syncthreads();   // red is reused by the next item
+    }
+}
+
 // ------------------------------------------------------------------ host launchers ----
 template <class T>
 static StencilCoef<T> make_coef(const qg_grid& g, double coef) {
@@ -452,19 +525,67 @@ static int env_int(const char* name, int dflt) {
     return (v && *v) ? std::atoi(v) : dflt;
 }
 
+// Tile shapes chosen by the autotuner (qg_autotune_hamiltonian), per grid shape and dtype;
+// make_plan uses them for the stencil modes unless QG_PLAN_* overrides are set.
+struct TunedShape {
+    int ty, ppt;
+};
+static std::mutex g_tune_mu;
+static std::map<std::tuple<int, int, int, int>, TunedShape> g_tuned;
+static thread_local const TunedShape* t_trial = nullptr;   // shape under trial by the autotuner
+
+static bool lookup_tuned(const qg_grid& g, int dtype, TunedShape& out) {
+    if (t_trial) {
+        out = *t_trial;
+        return true;
+    }
+    if (std::getenv("QG_PLAN_TY") || std::getenv("QG_PLAN_PPT")) return false;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(std::make_tuple(g.n[0], g.n[1], g.n[2], dtype));
+    if (it == g_tuned.end()) return false;
+    out = it->second;
+    return true;
+}
+
 // Tile plan: a tile is TY full x-rows; PPT points per thread; threads = ceil(TY*Nx/PPT)
 // rounded up to whole wavefronts (<= 512); slabs of TZ planes, deep enough to amortise the
 // 8-plane z warm-up, shallow enough to give >= ~2048 workgroups.  QG_PLAN_NB / _TY / _PPT /
 // _TZ override the choice for tuning sweeps.
-static Plan make_plan(const qg_grid& g, int dtype, int nbands) {
+static Plan make_plan(const qg_grid& g, int dtype, int nbands, int mode) {
     const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
     QG_REQUIRE(nx <= 128, "the fused kernel supports Nx <= 128");
     Plan p{};
+    if (mode == MODE_NL) {
+        // projector-only pass: no stencil state, so many bands share each projector item
+        const int want = env_int("QG_PLAN_NBNL", dtype == QG_F64 ? 8 : 16);
+        p.nb = 1;
+        for (int c : {16, 8, 4, 2})
+            if (c <= want && c <= nbands) { p.nb = c; break; }
+        p.ty = std::max(1, std::min(ny, env_int("QG_PLAN_TYNL", std::max(1, 640 / nx))));
+        p.ppt = 4;
+        p.nthr = round64((p.ty * nx + p.ppt - 1) / p.ppt);
+        while (p.nthr > kMaxThreads) {
+            p.ppt = (p.ppt == 4) ? 5 : 8;
+            p.nthr = round64((p.ty * nx + p.ppt - 1) / p.ppt);
+            QG_REQUIRE(p.ppt <= 8 && (p.ppt < 8 || p.nthr <= kMaxThreads), "tile too large");
+        }
+        const int ntile = (ny + p.ty - 1) / p.ty;
+        const int nchunk = (nbands + p.nb - 1) / p.nb;
+        int tz = nz;
+        while (tz > 16 && (int64_t)ntile * nchunk * ((nz + tz - 1) / tz) < 4096) tz = (tz + 1) / 2;
+        p.tz = std::max(1, std::min(nz, env_int("QG_PLAN_TZNL", tz)));
+        return p;
+    }
     p.nb = env_int("QG_PLAN_NB", 1);
     p.nb = (p.nb == 2 && nbands >= 2) ? 2 : 1;
     p.ty = std::max(1, std::min(ny, env_int("QG_PLAN_TY", 1280 / nx)));
     p.ppt = env_int("QG_PLAN_PPT", dtype == QG_F64 ? 2 : 4);
     if (dtype == QG_F64 && !std::getenv("QG_PLAN_TY")) p.ty = std::max(1, std::min(ny, 960 / nx));
+    TunedShape ts;
+    if (lookup_tuned(g, dtype, ts)) {
+        p.ty = std::max(1, std::min(ny, ts.ty));
+        p.ppt = ts.ppt;
+    }
     QG_REQUIRE(p.ppt == 2 || p.ppt == 4 || p.ppt == 5 || p.ppt == 8, "QG_PLAN_PPT must be 2, 4, 5 or 8");
     p.nthr = round64((p.ty * nx + p.ppt - 1) / p.ppt);
     while (p.nthr > kMaxThreads) {
@@ -522,12 +643,22 @@ static void launch_fused_ppt(const Plan& p, dim3 grid, size_t lds, hipStream_t s
     QG_HIP(hipGetLastError());
 }
 
+template <class T>
+struct FusedSetup {
+    Plan p;
+    FusedArgs a;
+    StencilCoef<T> sc;
+    size_t lds;
+    int njs;
+    int64_t nwg;
+};
+
+// Plan, kernel arguments and LDS size of one fused launch (also used per cell by the batch).
 template <class T, int MODE>
-static void launch_fused(const qg_grid& g, int nbands, const T* psi, int64_t ld_in, T* out,
-                         int64_t ld_out, double coef, int accumulate, const T* V,
-                         qg_nonlocal* nl, const double* coefs, hipStream_t s) {
+static FusedSetup<T> setup_fused(const qg_grid& g, int nbands, int64_t ld_in, int64_t ld_out, double coef,
+                                 int accumulate, qg_nonlocal* nl, const double* coefs) {
     const int dtype = sizeof(T) == 4 ? QG_F32 : QG_F64;
-    Plan p = make_plan(g, dtype, nbands);
+    Plan p = make_plan(g, dtype, nbands, MODE);
     QG_REQUIRE(g.ng() * (int64_t)sizeof(T) < (1LL << 31), "grid too large for 32-bit buffer offsets");
     FusedArgs a{};
     a.nx = g.n[0]; a.ny = g.n[1]; a.nz = g.n[2];
@@ -562,7 +693,39 @@ static void launch_fused(const qg_grid& g, int nbands, const T* psi, int64_t ld_
     const int nchunk = (nbands + p.nb - 1) / p.nb;
     const int64_t nwg = (int64_t)a.ntile_y * a.nslab * nchunk;
     QG_REQUIRE(nwg < (1LL << 31), "too many workgroups");
-    dim3 grid((unsigned)nwg);
+    return FusedSetup<T>{p, a, sc, lds, njs, nwg};
+}
+
+template <class T, int MODE>
+static void launch_fused(const qg_grid& g, int nbands, const T* psi, int64_t ld_in, T* out,
+                         int64_t ld_out, double coef, int accumulate, const T* V,
+                         qg_nonlocal* nl, const double* coefs, hipStream_t s) {
+    const FusedSetup<T> fs = setup_fused<T, MODE>(g, nbands, ld_in, ld_out, coef, accumulate, nl, coefs);
+    const Plan& p = fs.p;
+    const FusedArgs& a = fs.a;
+    const StencilCoef<T>& sc = fs.sc;
+    const size_t lds = fs.lds;
+    const int njs = fs.njs;
+    dim3 grid((unsigned)fs.nwg);
+    if constexpr (MODE == MODE_NL) {
+        if (p.nb >= 4) {
+            switch (p.nb) {
+                case 4:
+                    if (njs == 4) launch_fused_ppt<T, 4, MODE, 4>(p, grid, lds, s, a, sc, V, psi, out);
+                    else launch_fused_ppt<T, 4, MODE, 8>(p, grid, lds, s, a, sc, V, psi, out);
+                    break;
+                case 8:
+                    if (njs == 4) launch_fused_ppt<T, 8, MODE, 4>(p, grid, lds, s, a, sc, V, psi, out);
+                    else launch_fused_ppt<T, 8, MODE, 8>(p, grid, lds, s, a, sc, V, psi, out);
+                    break;
+                default:
+                    if (njs == 4) launch_fused_ppt<T, 16, MODE, 4>(p, grid, lds, s, a, sc, V, psi, out);
+                    else launch_fused_ppt<T, 16, MODE, 8>(p, grid, lds, s, a, sc, V, psi, out);
+                    break;
+            }
+            return;
+        }
+    }
     if (p.nb == 2) {
         if (njs == 4) launch_fused_ppt<T, 2, MODE, 4>(p, grid, lds, s, a, sc, V, psi, out);
         else launch_fused_ppt<T, 2, MODE, 8>(p, grid, lds, s, a, sc, V, psi, out);
@@ -587,12 +750,18 @@ static void launch_project(qg_nonlocal* nl, int nbands, const T* psi, int64_t ld
     a.beta = (sizeof(T) == 4) ? (const void*)nl->d_beta32.p : (const void*)nl->d_beta64.p;
     a.coef = coef;
     a.ncoef = nl->geo.ncoef;
-    constexpr int NBA = 8;
-    const int nchunk = (nbands + NBA - 1) / NBA;
-    dim3 grid((unsigned)((int64_t)a.natoms * nchunk));
-    if (nl->geo.njs == 4)
-        hipLaunchKernelGGL((project_kernel<T, NBA, 4>), grid, dim3(kProjBlock), 0, s, a, psi);
-    else
+    const int nba = env_int("QG_PROJ_NBA", 8);
+    if (nl->geo.njs == 4) {
+        auto go = [&](auto nba_c) {
+            constexpr int NBA = decltype(nba_c)::value;
+            const int nchunk = (nbands + NBA - 1) / NBA;
+            hipLaunchKernelGGL((project_kernel<T, NBA, 4>), dim3((unsigned)((int64_t)a.natoms * nchunk)),
+                               dim3(kProjBlock), 0, s, a, psi);
+        };
+        if (nba >= 16 && nbands >= 16) go(std::integral_constant<int, 16>{});
+        else if (nba >= 8) go(std::integral_constant<int, 8>{});
+        else go(std::integral_constant<int, 4>{});
+    } else
         hipLaunchKernelGGL((project_kernel<T, 4, 8>), dim3((unsigned)((int64_t)a.natoms * ((nbands + 3) / 4))),
                            dim3(kProjBlock), 0, s, a, psi);
     QG_HIP(hipGetLastError());
@@ -613,6 +782,7 @@ void hamiltonian_apply(qg_ham* h, int dtype, int nbands, const void* psi, int64_
     const qg_grid& g = h->grid;
     qg_nonlocal* nl = (h->nl && h->nl->geo.ncoef > 0) ? h->nl : nullptr;
     const int cb = nl ? chunk_bands(g, dtype, nbands) : nbands;
+    const bool split = env_int("QG_SPLIT_NL", 0) != 0;
     if (nl) nl->d_coef.alloc((size_t)std::min(cb, nbands) * nl->geo.ncoef);
     if (h->timing) QG_HIP(hipEventRecord(h->ev[0], s));
     for (int b0 = 0; b0 < nbands; b0 += cb) {
@@ -625,6 +795,22 @@ void hamiltonian_apply(qg_ham* h, int dtype, int nbands, const void* psi, int64_
             else launch_project<float>(nl, nb, (const float*)pin, ld, nl->d_coef.p, s);
         }
         if (h->timing && b0 == 0) QG_HIP(hipEventRecord(h->ev[1], s));
+        if (nl && split) {
+            // stencil + V pass, then the projector term as a read-modify-write pass over the
+            // chunk (still resident in the Infinity Cache) with many bands per item
+            if (dtype == QG_F64) {
+                launch_fused<double, MODE_HAM>(g, nb, (const double*)pin, ld, (double*)pout, ld_out, -0.5, 0,
+                                               h->V64.p, nullptr, nullptr, s);
+                launch_fused<double, MODE_NL>(g, nb, (const double*)pin, ld, (double*)pout, ld_out, 0.0, 1,
+                                              nullptr, nl, nl->d_coef.p, s);
+            } else {
+                launch_fused<float, MODE_HAM>(g, nb, (const float*)pin, ld, (float*)pout, ld_out, -0.5, 0,
+                                              h->V32.p, nullptr, nullptr, s);
+                launch_fused<float, MODE_NL>(g, nb, (const float*)pin, ld, (float*)pout, ld_out, 0.0, 1, nullptr,
+                                             nl, nl->d_coef.p, s);
+            }
+            continue;
+        }
         if (dtype == QG_F64)
             launch_fused<double, MODE_HAM>(g, nb, (const double*)pin, ld, (double*)pout, ld_out, -0.5, 0,
                                            h->V64.p, nl, nl ? nl->d_coef.p : nullptr, s);
@@ -636,6 +822,219 @@ void hamiltonian_apply(qg_ham* h, int dtype, int nbands, const void* psi, int64_
         QG_HIP(hipEventRecord(h->ev[2], s));
         h->timed_valid = true;
     }
+}
+
+// ------------------------------------------------------------ batches of cells ----
+// H psi for a batch of independent cells (different grids, atoms and band counts) in
+// persistent launches: per round, one projection launch over every (cell, atom, band chunk)
+// and one fused launch over every (cell, tile, slab, band) workgroup.  Rounds follow each
+// cell's Infinity-Cache chunking of its band block.
+struct BatchReq {
+    qg_ham* h;
+    int nbands;
+    const void* psi;
+    void* out;
+};
+
+template <class T, int NJS>
+static void launch_fused_batch(int ppt, int nthr, size_t lds, int total, const BatchCell<T>* cells,
+                               const int* prefix, int ncells, hipStream_t s) {
+    const int grid = std::max(1, std::min(total, 2048));
+    switch (ppt) {
+#define QG_CASE(PP)                                                                                          \
+    case PP:                                                                                                 \
+        hipLaunchKernelGGL((fused_batch_kernel<T, 1, PP, MODE_HAM, NJS>), dim3(grid), dim3(nthr), lds, s, cells, \
+                           prefix, ncells);                                                                  \
+        break;
+        QG_CASE(2) QG_CASE(4) QG_CASE(5) QG_CASE(8)
+#undef QG_CASE
+        default: throw Error{QG_ERR_ARG, "unsupported points-per-thread"};
+    }
+    QG_HIP(hipGetLastError());
+}
+
+template <class T>
+static void batch_apply_t(const std::vector<BatchReq>& reqs, hipStream_t s, DevBuf<unsigned char>& scratch) {
+    const int dtype = sizeof(T) == 4 ? QG_F32 : QG_F64;
+    const int n = (int)reqs.size();
+    std::vector<int> cb(n);
+    int rounds = 0;
+    int njs = 0;
+    for (int c = 0; c < n; ++c) {
+        const BatchReq& r = reqs[c];
+        qg_nonlocal* nl = (r.h->nl && r.h->nl->geo.ncoef > 0) ? r.h->nl : nullptr;
+        cb[c] = nl ? chunk_bands(r.h->grid, dtype, r.nbands) : std::max(1, r.nbands);
+        if (nl) {
+            nl->d_coef.alloc((size_t)std::min(cb[c], r.nbands) * nl->geo.ncoef);
+            QG_REQUIRE(njs == 0 || njs == nl->geo.njs, "cells of one batch must share the projector slot count");
+            njs = nl->geo.njs;
+        }
+        rounds = std::max(rounds, (r.nbands + cb[c] - 1) / cb[c]);
+    }
+    if (njs == 0) njs = 4;
+    const int NBA = njs == 4 ? 8 : 4;
+    // descriptors of every round, packed into one upload
+    struct RoundDesc {
+        size_t pc, pp, fc, fp;
+        int nproj, nfused, proj_total, fused_total, ppt, nthr;
+        size_t lds;
+    };
+    std::vector<RoundDesc> rd(rounds);
+    std::vector<unsigned char> host;
+    auto push = [&](const void* src, size_t bytes) {
+        const size_t off = (host.size() + 255) & ~size_t(255);
+        host.resize(off + bytes);
+        std::memcpy(host.data() + off, src, bytes);
+        return off;
+    };
+    for (int r = 0; r < rounds; ++r) {
+        std::vector<ProjCell<T>> pcs;
+        std::vector<int> ppre{0};
+        std::vector<BatchCell<T>> fcs;
+        std::vector<int> fpre{0};
+        int ppt = -1, nthr = 0;
+        size_t lds = 0;
+        for (int c = 0; c < n; ++c) {
+            const BatchReq& q = reqs[c];
+            const int b0 = r * cb[c];
+            if (b0 >= q.nbands) continue;
+            const int nb = std::min(cb[c], q.nbands - b0);
+            const qg_grid& g = q.h->grid;
+            const int64_t ng = g.ng();
+            const T* pin = (const T*)q.psi + (int64_t)b0 * ng;
+            T* pout = (T*)q.out + (int64_t)b0 * ng;
+            qg_nonlocal* nl = (q.h->nl && q.h->nl->geo.ncoef > 0) ? q.h->nl : nullptr;
+            if (nl) {
+                ProjArgs a{};
+                a.natoms = nl->geo.natoms;
+                a.nbands = nb;
+                a.ld = ng;
+                a.dv = nl->gcopy.dvol();
+                a.atom_img_off = nl->d_atom_img_off.p;
+                a.atom_coef_off = nl->d_atom_coef_off.p;
+                a.img_tbl_off = nl->d_img_tbl_off.p;
+                a.gidx = nl->d_gidx.p;
+                a.beta = (sizeof(T) == 4) ? (const void*)nl->d_beta32.p : (const void*)nl->d_beta64.p;
+                a.coef = nl->d_coef.p;
+                a.ncoef = nl->geo.ncoef;
+                pcs.push_back(ProjCell<T>{a, pin});
+                ppre.push_back(ppre.back() + a.natoms * ((nb + NBA - 1) / NBA));
+            }
+            const FusedSetup<T> fs = setup_fused<T, MODE_HAM>(g, nb, ng, ng, -0.5, 0, nl, nl ? nl->d_coef.p : nullptr);
+            QG_REQUIRE(fs.p.nb == 1, "batched cells use one band per workgroup");
+            QG_REQUIRE(ppt < 0 || ppt == fs.p.ppt, "cells of one batch must share the points-per-thread plan");
+            ppt = fs.p.ppt;
+            nthr = std::max(nthr, fs.p.nthr);
+            lds = std::max(lds, fs.lds);
+            const T* V = (sizeof(T) == 4) ? (const T*)q.h->V32.p : (const T*)q.h->V64.p;
+            fcs.push_back(BatchCell<T>{fs.a, fs.sc, V, pin, pout});
+            QG_REQUIRE((int64_t)fpre.back() + fs.nwg < (1LL << 31), "batch too large");
+            fpre.push_back(fpre.back() + (int)fs.nwg);
+        }
+        RoundDesc& d = rd[r];
+        d.nproj = (int)pcs.size();
+        d.nfused = (int)fcs.size();
+        d.proj_total = ppre.back();
+        d.fused_total = fpre.back();
+        d.ppt = ppt;
+        d.nthr = nthr;
+        d.lds = lds;
+        d.pc = pcs.empty() ? 0 : push(pcs.data(), pcs.size() * sizeof(ProjCell<T>));
+        d.pp = push(ppre.data(), ppre.size() * sizeof(int));
+        d.fc = push(fcs.data(), fcs.size() * sizeof(BatchCell<T>));
+        d.fp = push(fpre.data(), fpre.size() * sizeof(int));
+    }
+    scratch.alloc(host.size());
+    QG_HIP(hipMemcpyAsync(scratch.p, host.data(), host.size(), hipMemcpyHostToDevice, s));
+    for (int r = 0; r < rounds; ++r) {
+        const RoundDesc& d = rd[r];
+        const unsigned char* b = scratch.p;
+        if (d.nproj > 0 && d.proj_total > 0) {
+            const int grid = std::max(1, std::min(d.proj_total, 4096));
+            if (njs == 4)
+                hipLaunchKernelGGL((project_batch_kernel<T, 8, 4>), dim3(grid), dim3(kProjBlock), 0, s,
+                                   (const ProjCell<T>*)(b + d.pc), (const int*)(b + d.pp), d.nproj);
+            else
+                hipLaunchKernelGGL((project_batch_kernel<T, 4, 8>), dim3(grid), dim3(kProjBlock), 0, s,
+                                   (const ProjCell<T>*)(b + d.pc), (const int*)(b + d.pp), d.nproj);
+            QG_HIP(hipGetLastError());
+        }
+        if (d.nfused > 0 && d.fused_total > 0) {
+            if (njs == 4)
+                launch_fused_batch<T, 4>(d.ppt, d.nthr, d.lds, d.fused_total, (const BatchCell<T>*)(b + d.fc),
+                                         (const int*)(b + d.fp), d.nfused, s);
+            else
+                launch_fused_batch<T, 8>(d.ppt, d.nthr, d.lds, d.fused_total, (const BatchCell<T>*)(b + d.fc),
+                                         (const int*)(b + d.fp), d.nfused, s);
+        }
+    }
+    // the host descriptors must outlive the upload
+    QG_HIP(hipStreamSynchronize(s));
+}
+
+void batch_hamiltonian_apply(const std::vector<BatchReq>& reqs, int dtype, hipStream_t s,
+                             DevBuf<unsigned char>& scratch) {
+    if (reqs.empty()) return;
+    if (dtype == QG_F64) batch_apply_t<double>(reqs, s, scratch);
+    else batch_apply_t<float>(reqs, s, scratch);
+}
+
+// ----------------------------------------------------------------- autotuner ----
+// Times the fused H psi of this Hamiltonian for candidate tile shapes (rows per tile and
+// points per thread; slab depth by the planner's rule) on zero-filled blocks of nbands bands
+// and records the fastest for the grid shape and dtype.
+static void autotune(qg_ham* h, int dtype, int nbands, int fixed_ppt, TunedShape& best, double& best_ms,
+                     hipStream_t s) {
+    const qg_grid& g = h->grid;
+    const int nx = g.n[0], ny = g.n[1];
+    const size_t es = dtype == QG_F32 ? 4 : 8;
+    const int64_t ng = g.ng();
+    DevBuf<unsigned char> in, out;
+    in.alloc((size_t)nbands * ng * es);
+    out.alloc((size_t)nbands * ng * es);
+    QG_HIP(hipMemsetAsync(in.p, 0, (size_t)nbands * ng * es, s));
+    hipEvent_t e0, e1;
+    QG_HIP(hipEventCreate(&e0));
+    QG_HIP(hipEventCreate(&e1));
+    std::vector<int> ppts;
+    if (fixed_ppt) ppts = {fixed_ppt};
+    else if (dtype == QG_F32) ppts = {2, 4};
+    else ppts = {2, 4};
+    best_ms = 1e30;
+    for (int ppt : ppts)
+        for (int ty : {4, 6, 8, 12, 16, 20, 24, 32}) {
+            if (ty > ny) continue;
+            const int nthr = round64((ty * nx + ppt - 1) / ppt);
+            if (nthr > kMaxThreads || std::max(nthr, round64((8 * nx + kHaloPT - 1) / kHaloPT)) > kMaxThreads) continue;
+            TunedShape trial{ty, ppt};
+            t_trial = &trial;
+            try {
+                hamiltonian_apply(h, dtype, nbands, in.p, ng, out.p, ng, s);   // warm-up (+ tables)
+                QG_HIP(hipEventRecord(e0, s));
+                for (int r = 0; r < 3; ++r) hamiltonian_apply(h, dtype, nbands, in.p, ng, out.p, ng, s);
+                QG_HIP(hipEventRecord(e1, s));
+                QG_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                QG_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (ms / 3 < best_ms) {
+                    best_ms = ms / 3;
+                    best = trial;
+                }
+            } catch (const Error& e) {
+                if (e.code != QG_ERR_ARG) {
+                    t_trial = nullptr;
+                    (void)hipEventDestroy(e0);
+                    (void)hipEventDestroy(e1);
+                    throw;
+                }
+            }
+            t_trial = nullptr;
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    QG_REQUIRE(best_ms < 1e30, "no feasible tile shape");
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[std::make_tuple(g.n[0], g.n[1], g.n[2], dtype)] = best;
 }
 
 // (T + V) psi: the local part of H (no projectors), used by the learned preconditioner's
@@ -813,6 +1212,42 @@ int qg_hamiltonian_apply(qg_ham* h, int dtype, int nbands, const void* psi, int6
         check_block(&h->grid, dtype, nbands, psi, ld, hpsi, ld_out);
         if (nbands == 0) return;
         hamiltonian_apply(h, dtype, nbands, psi, ld, hpsi, ld_out, as_stream(stream));
+    });
+}
+
+int qg_batch_hamiltonian_apply(int ncells, qg_ham* const* hams, int dtype, const int* nbands,
+                               const void* const* psi, void* const* hpsi, void* stream) {
+    return guarded([&] {
+        QG_REQUIRE(ncells >= 0 && (ncells == 0 || (hams && nbands && psi && hpsi)), "bad arguments");
+        QG_REQUIRE(dtype == QG_F64 || dtype == QG_F32, "bad dtype");
+        std::vector<BatchReq> reqs;
+        for (int c = 0; c < ncells; ++c) {
+            QG_REQUIRE(hams[c] && nbands[c] >= 0, "bad cell");
+            if (nbands[c] == 0) continue;
+            QG_REQUIRE(psi[c] && hpsi[c] && psi[c] != hpsi[c], "bad cell block");
+            reqs.push_back(BatchReq{hams[c], nbands[c], psi[c], hpsi[c]});
+        }
+        // descriptor upload area of this entry point (kept for the life of the process)
+        static DevBuf<unsigned char>* scratch = new DevBuf<unsigned char>();
+        static std::mutex mu;
+        std::lock_guard<std::mutex> lk(mu);
+        batch_hamiltonian_apply(reqs, dtype, as_stream(stream), *scratch);
+    });
+}
+
+int qg_autotune_hamiltonian(qg_ham* h, int dtype, int nbands, int fixed_ppt, int* ty, int* ppt,
+                            double* best_ms) {
+    return guarded([&] {
+        QG_REQUIRE(h && nbands >= 1, "bad arguments");
+        QG_REQUIRE(dtype == QG_F64 || dtype == QG_F32, "bad dtype");
+        QG_REQUIRE(fixed_ppt == 0 || fixed_ppt == 2 || fixed_ppt == 4 || fixed_ppt == 5 || fixed_ppt == 8,
+                   "fixed_ppt must be 0, 2, 4, 5 or 8");
+        TunedShape best{};
+        double ms = 0.0;
+        autotune(h, dtype, nbands, fixed_ppt, best, ms, nullptr);
+        if (ty) *ty = best.ty;
+        if (ppt) *ppt = best.ppt;
+        if (best_ms) *best_ms = ms;
     });
 }
 

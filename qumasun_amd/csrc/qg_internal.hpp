@@ -185,6 +185,7 @@ struct qg_nonlocal {
 namespace qg {
 struct LobpcgState;
 struct PrecondPlan;
+struct BatchService;
 void lobpcg_state_free(LobpcgState*);
 void precond_state_free(PrecondPlan*);
 }  // namespace qg
@@ -192,6 +193,7 @@ void precond_state_free(PrecondPlan*);
 struct qg_ham {
     qg_grid grid;
     qg_cnn* cnn = nullptr;   // learned preconditioner (not owned)
+    qg::BatchService* batch = nullptr;   // set while a batched solve runs this cell
     qg::LobpcgState* lobpcg = nullptr;
     qg::PrecondPlan* precond = nullptr;
     qg_nonlocal* nl = nullptr;

@@ -21,6 +21,10 @@
 #include <rocsolver/rocsolver.h>
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
+#include <mutex>
+#include <string>
+#include <thread>
 #include <cmath>
 #include <cstdio>
 #include <vector>
@@ -33,6 +37,71 @@ void hamiltonian_apply(qg_ham* h, int dtype, int nbands, const void* psi, int64_
 void precond_apply(qg_ham* h, int kind, double shift, int dtype, int nbands, void* r, int64_t ld,
                    hipStream_t s);
 void cnn_precond_apply(qg_cnn* c, int dtype, int nbands, void* r, int64_t ld, hipStream_t s);
+
+struct BatchReq {
+    qg_ham* h;
+    int nbands;
+    const void* psi;
+    void* out;
+};
+void batch_hamiltonian_apply(const std::vector<BatchReq>& reqs, int dtype, hipStream_t s,
+                             DevBuf<unsigned char>& scratch);
+
+// Rendezvous of the cells of a batched solve: each cell's solver thread posts its H psi
+// request and waits; the last thread to arrive runs every pending request as one batched
+// (persistent-kernel) application on its own stream, then wakes the others.  Cells whose
+// solve has finished leave the rendezvous.
+struct BatchService {
+    std::mutex mu;
+    std::condition_variable cv;
+    int active = 0;
+    uint64_t gen = 0;
+    struct Req {
+        BatchReq r;
+        int dtype;
+    };
+    std::vector<Req> pending;
+    std::string error;
+    DevBuf<unsigned char> scratch;
+    int launches = 0;
+
+    // runs with mu held; every active thread is waiting, so pending is stable
+    void execute(hipStream_t s) {
+        try {
+            for (int dt : {QG_F32, QG_F64}) {
+                std::vector<BatchReq> v;
+                for (auto& q : pending)
+                    if (q.dtype == dt) v.push_back(q.r);
+                if (!v.empty()) {
+                    batch_hamiltonian_apply(v, dt, s, scratch);
+                    ++launches;
+                }
+            }
+        } catch (const Error& e) {
+            error = e.msg;
+        }
+        pending.clear();
+        ++gen;
+        cv.notify_all();
+    }
+};
+
+static void batch_service_apply(BatchService* b, qg_ham* h, int dtype, int nb, const void* in, void* out,
+                                hipStream_t s) {
+    QG_HIP(hipStreamSynchronize(s));   // the request's input is complete on this cell's stream
+    std::unique_lock<std::mutex> lk(b->mu);
+    b->pending.push_back(BatchService::Req{BatchReq{h, nb, in, out}, dtype});
+    const uint64_t my = b->gen;
+    if ((int)b->pending.size() == b->active) b->execute(s);
+    else b->cv.wait(lk, [&] { return b->gen != my; });
+    if (!b->error.empty()) throw Error{QG_ERR_HIP, "batched H psi failed: " + b->error};
+}
+
+static void batch_service_leave(BatchService* b, hipStream_t s) {
+    std::unique_lock<std::mutex> lk(b->mu);
+    --b->active;
+    if (b->active > 0 && (int)b->pending.size() == b->active) b->execute(s);
+}
 void gram_bf16(const float* A, const float* B, int64_t K, int M, int N, int64_t lda, int64_t ldb,
                double* C, int ldc, float* partial, hipStream_t s, bool split);
 size_t gram_bf16_workspace(int64_t K, int M, int N);
@@ -330,7 +399,10 @@ static void run_lobpcg(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_ho
     auto sync = [&] { QG_HIP(hipStreamSynchronize(s)); };
     auto hpsi = [&](const T* in, T* outp) {
         auto t0 = now();
-        hamiltonian_apply(h, dtype, N, in, K, outp, K, s);
+        if (h->batch)
+            batch_service_apply(h->batch, h, dtype, N, in, outp, s);
+        else
+            hamiltonian_apply(h, dtype, N, in, K, outp, K, s);
         sync();
         stats.t_hpsi += secs(t0, now());
         stats.hpsi_calls += N;
@@ -623,6 +695,46 @@ int qg_lobpcg_d_multi(qg_ham* h, const qg_lobpcg_opts* opts, int nbands, double*
             }
         }
         if (stats) *stats = st;
+    });
+}
+
+int qg_batch_lobpcg(int ncells, qg_ham* const* hams, const qg_lobpcg_opts* opts, const int* nbands,
+                    double* const* eig, double* const* X, double* const* P, int scf_step, qg_lobpcg_stats* stats,
+                    int batched) {
+    return guarded([&] {
+        QG_REQUIRE(ncells >= 1 && hams && nbands && eig && X && P, "bad arguments");
+        for (int c = 0; c < ncells; ++c) {
+            QG_REQUIRE(hams[c] && eig[c] && X[c] && P[c] && nbands[c] >= 1, "bad cell");
+            for (int d = 0; d < c; ++d) QG_REQUIRE(hams[d] != hams[c], "a Hamiltonian appears twice in the batch");
+        }
+        if (!batched) {
+            // one cell after another, each with its own launches
+            for (int c = 0; c < ncells; ++c) {
+                const int rc = qg_lobpcg_d_multi(hams[c], opts, nbands[c], eig[c], X[c], P[c], scf_step,
+                                                 stats ? &stats[c] : nullptr);
+                if (rc != QG_OK) throw Error{rc, std::string("cell ") + std::to_string(c) + ": " + qg_last_error()};
+            }
+            return;
+        }
+        BatchService svc;
+        svc.active = ncells;
+        for (int c = 0; c < ncells; ++c) hams[c]->batch = &svc;
+        std::vector<std::string> errs(ncells);
+        std::vector<int> codes(ncells, QG_OK);
+        std::vector<std::thread> th;
+        for (int c = 0; c < ncells; ++c)
+            th.emplace_back([&, c] {
+                qg_lobpcg_stats st{};
+                codes[c] = qg_lobpcg_d_multi(hams[c], opts, nbands[c], eig[c], X[c], P[c], scf_step, &st);
+                if (codes[c] != QG_OK) errs[c] = qg_last_error();
+                if (stats) stats[c] = st;
+                hipStream_t s = hams[c]->lobpcg ? hams[c]->lobpcg->stream : nullptr;
+                batch_service_leave(&svc, s);
+            });
+        for (auto& t : th) t.join();
+        for (int c = 0; c < ncells; ++c) hams[c]->batch = nullptr;
+        for (int c = 0; c < ncells; ++c)
+            if (codes[c] != QG_OK) throw Error{codes[c], std::string("cell ") + std::to_string(c) + ": " + errs[c]};
     });
 }
 
