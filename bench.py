@@ -1,14 +1,24 @@
 #!/usr/bin/env python3
-"""Benchmark of the H·psi hot path on one MI355X.
+"""Benchmark of the H·psi hot path (and, optionally, time to eigenpairs) on one MI355X.
 
-Workload (the configuration the headline H·psi metric is quoted on): the 64-atom 2x2x2
-diamond supercell, L = 13.48 bohr, periodic 80^3 grid, Vloc = sum -4 erf(r/0.8)/r + U(+-0.05),
-Gaussian s+p KB projectors (sigma 0.6, cutoff 2.4, E = +1.0 / -0.5), and a block of 128
-seeded random real bands.  A step is one application of H (projector dots + the fused
-stencil/V/projector kernel) to the whole 128-band block, inputs resident in HBM.
+H·psi workload (the configuration the H·psi bandwidth metric is quoted on): the 64-atom
+2x2x2 diamond supercell, L = 13.48 bohr, periodic 80^3 grid, Vloc = sum -4 erf(r/0.8)/r +
+U(+-0.05), Gaussian s+p KB projectors (sigma 0.6, cutoff 2.4, E = +1.0 / -0.5), and a block of
+128 seeded random real bands in FP32.  A step is one application of H to the whole 128-band
+block with inputs resident in HBM: per band chunk of about 64 MB, the projector-dot kernel and
+the fused stencil + V + projector-apply kernel.
 
-value = algorithmic bytes of the step / step time, algorithmic bytes = 2*B*Ng*s + Ng*s
-(psi read once, H psi written once, V read once; analytic projectors, no tables).
+value     = algorithmic bytes of a step / step time, algorithmic bytes = 2*B*Ng*s + Ng*s (psi
+            read once, H psi written once, V read once; the projector tables are reported
+            separately as table_bytes).
+roofline  = the fused kernel: algorithmic bytes of one launch (one band chunk) / its average
+            duration (hipEvents around each launch, on the launch stream); traffic = HBM bytes
+            per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this
+            workload, scaled by the copy-kernel calibration factor recorded there.
+cpu_baseline = the FP64 oracle (a C++ restatement of the reference's CPU path) on 1 host
+            thread, one pass over a bounded sample of the same block.
+--eigen cfg0|cfg2: also time the LOBPCG solve to eigenvalues within 1e-6 Ha of the FP64
+            reference: plain FP64 (the reference's algorithm) vs FP32 + learned preconditioner.
 """
 import argparse
 import json
@@ -24,6 +34,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "fused_hpsi_80cube_128bands_f32_pmc.json")
 
 
 def parse():
@@ -35,7 +46,9 @@ def parse():
     ap.add_argument("--bands", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-bands", type=int, default=128,
-                    help="bands of the CPU baseline sample (FP64 oracle, 1 core)")
+                    help="bands of the CPU baseline sample (FP64 oracle, 1 thread)")
+    ap.add_argument("--eigen", choices=["none", "cfg0", "cfg2"], default="none",
+                    help="also time LOBPCG to 1e-6 Ha: plain FP64 vs FP32 + learned preconditioner")
     return ap.parse_args()
 
 
@@ -51,9 +64,9 @@ def cpu_model():
 
 
 def cpu_baseline(sysd, X, nbands):
-    """The FP64 oracle (a C++ restatement of the reference's CPU path, 1 thread) on a bounded
-    sample of the same workload: nbands bands of the 80^3 block."""
+    """The FP64 oracle on 1 host thread, one pass over nbands bands of the 80^3 block."""
     import oracle as O
+    O.olib().or_set_threads(1)
     n = (sysd["n"],) * 3
     h = (sysd["h"],) * 3
     nat = len(sysd["positions"])
@@ -64,9 +77,77 @@ def cpu_baseline(sysd, X, nbands):
     dt = time.perf_counter() - t0
     ng = n[0] * n[1] * n[2]
     bytes_ = 2 * nbands * ng * 8 + ng * 8
-    return dict(value=bytes_ / dt / 1e9, unit="GB/s", cores=1, kind="port",
-                sample=f"FP64 oracle H·psi, {nbands} bands of the 80^3/64-atom block, one pass, "
-                       f"{dt:.2f} s on 1 thread of {cpu_model()}; bytes counted as 2*B*Ng*8+Ng*8")
+    return dict(value=round(bytes_ / dt / 1e9, 4), unit="GB/s", cores=1, kind="port",
+                sample=f"FP64 oracle H·psi (C++ restatement of the reference's loops), {nbands} bands of "
+                       f"the 80^3/64-atom block, one pass, {dt:.2f} s on 1 thread of {cpu_model()}; "
+                       f"bytes counted as 2*B*Ng*8+Ng*8")
+
+
+def traffic_per_launch(dtype, chunk_bands):
+    """HBM bytes per fused launch from the committed rocprofv3 counter summary, if it matches."""
+    try:
+        d = json.load(open(PMC_SUMMARY))
+    except Exception:
+        return None
+    if d.get("dtype") != dtype or d.get("chunk_bands") != chunk_bands:
+        return None
+    return int(d["hbm_bytes_per_launch"])
+
+
+def eigen_leg(cfg):
+    """Seconds to eigenvalues within 1e-6 Ha of the FP64 reference: plain FP64 LOBPCG (the
+    reference's algorithm, no preconditioner) vs FP32 with the learned preconditioner."""
+    import torch
+    import qumasun_amd as Q
+    from qumasun_amd import _native as N
+    ncell, nb = (1, 16) if cfg == "cfg0" else (3, 512)
+    s = Q.diamond_system(ncell, 40)
+    n, h = s["n"], s["h"]
+    g = Q.Grid(n, n, n, h, h, h)
+    integ = Q.PseudoPotIntegrator(g, s["projectors"])
+    integ.UpdatePosition(s["positions"])
+    H = Q.Hamiltonian(g, s["V"], integ)
+    X0 = Q.random_states(g.size, nb, g.dvol)
+
+    def solve(chunk, ref=None, budget=600.0, **kw):
+        X = torch.from_numpy(X0.copy()).cuda()
+        P = torch.zeros_like(X)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        it, step = 0, 0
+        while True:
+            e, st = Q.Eigen_d_multi(H, X, P, step, max_iter=chunk, **kw)
+            step += 1
+            it += st["iterations"]
+            el = time.perf_counter() - t0
+            if ref is None and st["max_residual"] <= kw.get("tol", 0.0):
+                return e, el, it, True
+            if ref is not None and float(np.abs(e - ref).max()) <= 1e-6:
+                return e, el, it, True
+            if el > budget or st["iterations"] == 0:
+                return e, el, it, False
+
+    ref, t_ref, it_ref, _ = solve(400, precision=N.QG_PREC_F64, precond=N.QG_PRECOND_KINETIC, precond_shift=0.5,
+                                  tol=1e-9)
+    _, t_plain, it_plain, ok_plain = solve(10, ref=ref, precision=N.QG_PREC_F64)
+    t1 = time.perf_counter()
+    sigma = float(ref[0]) - 0.7
+    net, info = Q.fit_learned_preconditioner(H, min(nb, 32), sigma, 2.0, snapshots=(2, 5, 10), steps=300)
+    t_train = time.perf_counter() - t1
+    net.attach()
+    _, t_l, it_l, ok_l = solve(10, ref=ref, precision=N.QG_PREC_F32, precond=N.QG_PRECOND_LEARNED,
+                               precond_shift=2.0, tol=1e-5, switch_tol=1e-3)
+    _, t_k, it_k, ok_k = solve(10, ref=ref, precision=N.QG_PREC_F32, precond=N.QG_PRECOND_KINETIC,
+                               precond_shift=0.5, tol=1e-5, switch_tol=1e-3)
+    net.detach()
+    return {"workload": f"{len(s['positions'])} atoms, {n}^3 grid, {nb} bands",
+            "criterion": "max |e - e_FP64| <= 1e-6 Ha",
+            "plain_fp64": {"seconds": round(t_plain, 3), "iterations": it_plain, "reached": ok_plain},
+            "fp32_learned": {"seconds": round(t_l, 3), "iterations": it_l, "reached": ok_l,
+                             "train_seconds": round(t_train, 3), "train": info},
+            "fp32_kinetic": {"seconds": round(t_k, 3), "iterations": it_k, "reached": ok_k},
+            "speedup_learned_vs_plain": round(t_plain / t_l, 3) if ok_l else None,
+            "reference_fp64_kinetic": {"seconds": round(t_ref, 3), "iterations": it_ref}}
 
 
 def main():
@@ -111,7 +192,7 @@ def main():
     wall = time.perf_counter() - t0
     ms_step = e0.elapsed_time(e1) / args.steps
 
-    # per-kernel durations of the same launches (sector hipEvents inside the library)
+    # per-launch durations of the same kernels (hipEvents around each launch inside the library)
     H.set_timing(True)
     tp, tf = [], []
     for _ in range(args.steps):
@@ -119,13 +200,18 @@ def main():
         a, b = H.last_times()
         tp.append(a)
         tf.append(b)
+    nchunks = H.last_chunks()
     H.set_timing(False)
     t_proj = float(np.mean(tp))
     t_fused = float(np.mean(tf))
+    chunk_bands = (args.bands + nchunks - 1) // nchunks
 
     bytes_step = 2 * args.bands * ng * s + ng * s
+    bytes_launch = 2 * chunk_bands * ng * s + ng * s
     value = bytes_step / (ms_step * 1e-3) / 1e9
-    achieved = bytes_step / (t_fused * 1e-3) / 1e9
+    fused_launch_ms = t_fused / nchunks
+    achieved = bytes_launch / (fused_launch_ms * 1e-3) / 1e9
+    table_bytes = int(round(1.51 * ng)) * 4 * s   # beta tables, 4 slots per sphere point
 
     res = {
         "metric": "Hpsi HBM GB/s (algorithmic bytes / step time)",
@@ -142,16 +228,20 @@ def main():
         "data": "synthetic (seeded diamond cell, seeded random bands)",
         "config": {"workload": "Hpsi 64-atom diamond 2x2x2, L=13.48 bohr, 80^3 grid, 128 bands",
                    "grid": [n, n, n], "atoms": len(sysd["positions"]), "bands": args.bands,
-                   "projectors": "gaussian s+p, sigma 0.6, rc 2.4"},
+                   "projectors": "gaussian s+p, sigma 0.6, rc 2.4", "band_chunks": nchunks},
         "roofline": {"bound": "hbm", "kernel": "fused_hpsi_kernel",
                      "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
-                     "kernel_ms": round(t_fused, 5), "project_ms": round(t_proj, 5),
-                     "algorithmic_bytes_per_launch": bytes_step},
+                     "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "traffic": traffic_per_launch(args.dtype, chunk_bands),
+                     "algorithmic_bytes_per_launch": bytes_launch, "launch_ms": round(fused_launch_ms, 5),
+                     "project_ms_per_step": round(t_proj, 5), "fused_ms_per_step": round(t_fused, 5),
+                     "table_bytes": table_bytes},
         "wall_s_timed": round(wall, 4),
     }
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(sysd, X, min(args.cpu_bands, args.bands))
+    if args.eigen != "none":
+        res["eigen"] = eigen_leg(args.eigen)
     print(json.dumps(res))
 
 

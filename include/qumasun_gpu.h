@@ -119,11 +119,13 @@ int qg_hamiltonian_apply(qg_ham* h, int dtype, int nbands, const void* psi, int6
                          void* hpsi, int64_t ld_out, void* stream);
 /* Sector timers of qg_hamiltonian_apply on this handle (hipEvents recorded on the apply's
  * stream), the counterpart of the reference's StopWatch sectors 13 (PP nonlocal) and
- * 11/12 (kinetic, V) (qumasun_mpi.matrix.h:30-41): t_project_ms is the projector-dot
- * kernel, t_fused_ms the fused stencil + V + projector-apply kernel of the last apply.
- * last_times waits for the recorded events. */
+ * 11/12 (kinetic, V) (qumasun_mpi.matrix.h:30-41): t_project_ms is the total time of the
+ * projector-dot launches, t_fused_ms that of the fused stencil + V + projector-apply launches
+ * of the last apply (one of each per band chunk).  last_times waits for the recorded events. */
 int qg_hamiltonian_set_timing(qg_ham* h, int enable);
 int qg_hamiltonian_last_times(qg_ham* h, double* t_project_ms, double* t_fused_ms);
+/* Number of band chunks (= projection launches = fused launches) of the last timed apply. */
+int qg_hamiltonian_last_chunks(qg_ham* h, int* nchunks);
 
 /* ------------------------------------------------------------- block LOBPCG ---- */
 /* precision modes */

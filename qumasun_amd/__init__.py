@@ -212,10 +212,17 @@ class Hamiltonian:
         check(lib().qg_hamiltonian_set_timing(self._h, int(bool(enable))))
 
     def last_times(self):
-        """(projector-dot ms, fused-kernel ms) of the last apply (hipEvent sector timers)."""
+        """(projector-dot ms, fused-kernel ms) of the last apply, summed over its band chunks
+        (hipEvent sector timers)."""
         a, b = ctypes.c_double(), ctypes.c_double()
         check(lib().qg_hamiltonian_last_times(self._h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+    def last_chunks(self):
+        """Band chunks (fused launches) of the last timed apply."""
+        n = ctypes.c_int()
+        check(lib().qg_hamiltonian_last_chunks(self._h, ctypes.byref(n)))
+        return n.value
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -457,6 +464,22 @@ def diamond_cell(dims, n_per_cell=40, a=DIAMOND_A, zion=4.0, rc=0.8, noise=0.05,
     projs = [[gaussian_projector(0, e_s, sigma, cutoff), gaussian_projector(1, e_p, sigma, cutoff)]]
     return dict(n=tuple(n), L=tuple(L), h=tuple(L[d] / n[d] for d in range(3)), positions=xyz.reshape(-1, 3), V=V,
                 projectors=projs)
+
+
+SEED_CELLS = 123456791   # stream of the batched uneven-cell workload
+
+
+def uneven_cells(ncells=32, seed=SEED_CELLS, n_per_cell=40):
+    """The batched workload: ncells independent diamond supercells with dimensions drawn from
+    {1,2}^3 (8-64 atoms, grids 40^3-80^3) and band counts from {16, 32, 64, 128}, all from one
+    seeded stream; each cell's potential noise uses its own seed."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for c in range(ncells):
+        dims = tuple(int(x) for x in rng.integers(1, 3, size=3))
+        nb = int(rng.choice([16, 32, 64, 128]))
+        out.append(dict(dims=dims, nbands=nb, seed=int(SEED_VLOC + 1 + c)))
+    return out
 
 
 def random_states(ng, nbands, dvol, seed=SEED_STATES):

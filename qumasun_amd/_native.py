@@ -73,6 +73,7 @@ SIGNATURES = {
                                      c_void_p]),
     "qg_hamiltonian_set_timing": (c_int, [c_void_p, c_int]),
     "qg_hamiltonian_last_times": (c_int, [c_void_p, P(c_double), P(c_double)]),
+    "qg_hamiltonian_last_chunks": (c_int, [c_void_p, P(c_int)]),
     "qg_lobpcg_default_opts": (None, [P(LobpcgOpts)]),
     "qg_lobpcg_d_multi": (c_int, [c_void_p, P(LobpcgOpts), c_int, P(c_double), c_void_p, c_void_p,
                                   c_int, P(LobpcgStats)]),

@@ -784,8 +784,18 @@ void hamiltonian_apply(qg_ham* h, int dtype, int nbands, const void* psi, int64_
     const int cb = nl ? chunk_bands(g, dtype, nbands) : nbands;
     const bool split = env_int("QG_SPLIT_NL", 0) != 0;
     if (nl) nl->d_coef.alloc((size_t)std::min(cb, nbands) * nl->geo.ncoef);
-    if (h->timing) QG_HIP(hipEventRecord(h->ev[0], s));
+    const int nchunks = (nbands + cb - 1) / cb;
+    if (h->timing) {
+        while ((int)h->tev.size() < 3 * nchunks) {
+            hipEvent_t e;
+            QG_HIP(hipEventCreate(&e));
+            h->tev.push_back(e);
+        }
+        h->tchunks = nchunks;
+    }
     for (int b0 = 0; b0 < nbands; b0 += cb) {
+        const int ci = b0 / cb;
+        if (h->timing) QG_HIP(hipEventRecord(h->tev[3 * ci], s));
         const int nb = std::min(cb, nbands - b0);
         const size_t es = (dtype == QG_F32) ? 4 : 8;
         const char* pin = (const char*)psi + (size_t)b0 * ld * es;
@@ -794,7 +804,7 @@ void hamiltonian_apply(qg_ham* h, int dtype, int nbands, const void* psi, int64_
             if (dtype == QG_F64) launch_project<double>(nl, nb, (const double*)pin, ld, nl->d_coef.p, s);
             else launch_project<float>(nl, nb, (const float*)pin, ld, nl->d_coef.p, s);
         }
-        if (h->timing && b0 == 0) QG_HIP(hipEventRecord(h->ev[1], s));
+        if (h->timing) QG_HIP(hipEventRecord(h->tev[3 * ci + 1], s));
         if (nl && split) {
             // stencil + V pass, then the projector term as a read-modify-write pass over the
             // chunk (still resident in the Infinity Cache) with many bands per item
@@ -809,19 +819,16 @@ void hamiltonian_apply(qg_ham* h, int dtype, int nbands, const void* psi, int64_
                 launch_fused<float, MODE_NL>(g, nb, (const float*)pin, ld, (float*)pout, ld_out, 0.0, 1, nullptr,
                                              nl, nl->d_coef.p, s);
             }
-            continue;
-        }
-        if (dtype == QG_F64)
+        } else if (dtype == QG_F64) {
             launch_fused<double, MODE_HAM>(g, nb, (const double*)pin, ld, (double*)pout, ld_out, -0.5, 0,
                                            h->V64.p, nl, nl ? nl->d_coef.p : nullptr, s);
-        else
+        } else {
             launch_fused<float, MODE_HAM>(g, nb, (const float*)pin, ld, (float*)pout, ld_out, -0.5, 0,
                                           h->V32.p, nl, nl ? nl->d_coef.p : nullptr, s);
+        }
+        if (h->timing) QG_HIP(hipEventRecord(h->tev[3 * ci + 2], s));
     }
-    if (h->timing) {
-        QG_HIP(hipEventRecord(h->ev[2], s));
-        h->timed_valid = true;
-    }
+    if (h->timing) h->timed_valid = true;
 }
 
 // ------------------------------------------------------------ batches of cells ----
@@ -1254,9 +1261,6 @@ int qg_autotune_hamiltonian(qg_ham* h, int dtype, int nbands, int fixed_ppt, int
 int qg_hamiltonian_set_timing(qg_ham* h, int enable) {
     return guarded([&] {
         QG_REQUIRE(h, "null hamiltonian");
-        if (enable)
-            for (auto& e : h->ev)
-                if (!e) QG_HIP(hipEventCreate(&e));
         h->timing = enable != 0;
         h->timed_valid = false;
     });
@@ -1266,12 +1270,25 @@ int qg_hamiltonian_last_times(qg_ham* h, double* t_project_ms, double* t_fused_m
     return guarded([&] {
         QG_REQUIRE(h && t_project_ms && t_fused_ms, "null argument");
         QG_REQUIRE(h->timing && h->timed_valid, "no timed apply recorded (enable timing first)");
-        QG_HIP(hipEventSynchronize(h->ev[2]));
-        float a = 0.f, b = 0.f;
-        QG_HIP(hipEventElapsedTime(&a, h->ev[0], h->ev[1]));
-        QG_HIP(hipEventElapsedTime(&b, h->ev[1], h->ev[2]));
-        *t_project_ms = a;
-        *t_fused_ms = b;
+        double ta = 0.0, tb = 0.0;
+        for (int c = 0; c < h->tchunks; ++c) {
+            QG_HIP(hipEventSynchronize(h->tev[3 * c + 2]));
+            float a = 0.f, b = 0.f;
+            QG_HIP(hipEventElapsedTime(&a, h->tev[3 * c], h->tev[3 * c + 1]));
+            QG_HIP(hipEventElapsedTime(&b, h->tev[3 * c + 1], h->tev[3 * c + 2]));
+            ta += a;
+            tb += b;
+        }
+        *t_project_ms = ta;
+        *t_fused_ms = tb;
+    });
+}
+
+int qg_hamiltonian_last_chunks(qg_ham* h, int* nchunks) {
+    return guarded([&] {
+        QG_REQUIRE(h && nchunks, "null argument");
+        QG_REQUIRE(h->timing && h->timed_valid, "no timed apply recorded (enable timing first)");
+        *nchunks = h->tchunks;
     });
 }
 

@@ -201,9 +201,10 @@ struct qg_ham {
     qg::DevBuf<float> V32;
     bool timing = false;
     bool timed_valid = false;
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    std::vector<hipEvent_t> tev;   // per band chunk: before projection, before fused, after fused
+    int tchunks = 0;
     ~qg_ham() {
-        for (auto& e : ev)
+        for (auto& e : tev)
             if (e) (void)hipEventDestroy(e);
         if (lobpcg) qg::lobpcg_state_free(lobpcg);
         if (precond) qg::precond_state_free(precond);

@@ -234,6 +234,7 @@ struct LobpcgState {
         if (blas) rocblas_destroy_handle(blas);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
     }
 };
 
