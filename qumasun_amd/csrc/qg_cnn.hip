@@ -24,6 +24,7 @@
 // 4 z-points x COUT channels in registers.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <random>
@@ -144,6 +145,110 @@ __global__ __launch_bounds__(kConvThreads) void conv3_kernel(ConvArgs a, const W
                     val *= 1.f + tanhf(a.ga[c] * vv + a.gb[c]);
                 }
                 out[(int64_t)b * a.out_bs + c * a.ng + gi] = val;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------ MFMA convolution ----
+// Inference form of conv3_kernel on the matrix cores (BF16 activations and weights, FP32
+// accumulation): one wavefront computes 32 x-points x 4 z-planes of one y-row for up to 8
+// output channels as v_mfma_f32_32x32x16_bf16 products, summed over the 9 (dy, dx) taps:
+//   A (32 points x K),  K = (input plane zin of 6) x (input channel of 8),
+//   B (K x 32),         column n = co * 4 + zo, B[(zin, ci)][n] = w[co][ci][zin - zo][dy][dx]
+//                       (zero outside the 3-tap band),
+// so 27 MFMAs per wavefront and tile.  The input tile (6 planes x 6 rows x 34 columns x 8
+// channels, BF16, channel-fastest so a lane's 8 K-values are one 16-byte LDS read) is staged
+// with the periodic wrap; the B fragments of all 27 (tap, K-step) pairs stay in registers.
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+constexpr int kMX = 32, kMY = 4, kMZ = 4;
+constexpr int kMLX = kMX + 2, kMLY = kMY + 2, kMLZ = kMZ + 2;
+constexpr int kMThreads = 64 * kMY;
+
+__device__ __forceinline__ int modi(int i, int n) { return ((i % n) + n) % n; }
+
+template <int CIN, int COUT, int IN_MODE, int OUT_MODE>
+__global__ __launch_bounds__(kMThreads) void conv3_mfma_kernel(ConvArgs a, const float* __restrict__ w,
+                                                               const float* __restrict__ in, float* __restrict__ out) {
+    static_assert(CIN <= 8 && COUT <= 8, "at most 8 channels");
+    __shared__ __align__(16) __bf16 tile[kMLZ * kMLY * kMLX * 8];
+    const int tiles_x = (a.nx + kMX - 1) / kMX;
+    const int x0 = (blockIdx.x % tiles_x) * kMX, y0 = (blockIdx.x / tiles_x) * kMY;
+    const int z0 = blockIdx.y * kMZ;
+    const int b = blockIdx.z;
+    const float* inb = in + (int64_t)b * a.in_bs;
+    for (int i = threadIdx.x; i < kMLZ * kMLY * kMLX; i += kMThreads) {
+        const int xx = i % kMLX, yy = (i / kMLX) % kMLY, zz = i / (kMLX * kMLY);
+        const int gx = modi(x0 + xx - 1, a.nx), gy = modi(y0 + yy - 1, a.ny), gz = modi(z0 + zz - 1, a.nz);
+        const int64_t gi = gx + (int64_t)a.nx * (gy + (int64_t)a.ny * gz);
+        float vals[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) vals[c] = 0.f;
+        if constexpr (IN_MODE == IN_UV) {
+            const float u = inb[gi];
+            vals[0] = u;
+            vals[1] = a.v[gi] * u;
+        } else {
+#pragma unroll
+            for (int c = 0; c < CIN; ++c) vals[c] = inb[c * a.ng + gi];
+        }
+        bf16x8_t pk;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) pk[c] = (__bf16)vals[c];
+        *reinterpret_cast<bf16x8_t*>(&tile[i * 8]) = pk;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n = lane & 31, kg = lane >> 5;
+    const int co = n >> 2, zo = n & 3;
+    bf16x8_t bw[27];
+#pragma unroll
+    for (int sh = 0; sh < 9; ++sh) {
+        const int dy = sh / 3, dx = sh % 3;
+#pragma unroll
+        for (int kb = 0; kb < 3; ++kb) {
+            const int dz = 2 * kb + kg - zo;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                float val = 0.f;
+                if (co < COUT && j < CIN && dz >= 0 && dz <= 2) val = w[(co * CIN + j) * kTaps + dz * 9 + dy * 3 + dx];
+                bw[sh * 3 + kb][j] = (__bf16)val;
+            }
+        }
+    }
+    __syncthreads();
+    f32x16_t acc;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = 0.f;
+    const int r = lane & 31;
+#pragma unroll
+    for (int sh = 0; sh < 9; ++sh) {
+        const int dy = sh / 3, dx = sh % 3;
+#pragma unroll
+        for (int kb = 0; kb < 3; ++kb) {
+            const int zin = 2 * kb + kg;
+            const bf16x8_t av =
+                *reinterpret_cast<const bf16x8_t*>(&tile[((zin * kMLY + (wave + dy)) * kMLX + (r + dx)) * 8]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bw[sh * 3 + kb], acc, 0, 0, 0);
+        }
+    }
+    const int gy = y0 + wave;
+    const int gz = z0 + zo;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int gx = x0 + 8 * (i / 4) + 4 * kg + (i % 4);
+        float partner = 0.f;
+        if constexpr (OUT_MODE == OUT_UV) partner = __shfl(acc[i], lane + 4, 64);
+        if (gx >= a.nx || gy >= a.ny || gz >= a.nz) continue;
+        const int64_t gi = gx + (int64_t)a.nx * (gy + (int64_t)a.ny * gz);
+        if constexpr (OUT_MODE == OUT_UV) {
+            if (co == 0) out[(int64_t)b * a.out_bs + gi] = acc[i] + a.v[gi] * partner;
+        } else {
+            if (co < COUT) {
+                float val = acc[i];
+                if constexpr (OUT_MODE == OUT_GATE) val *= 1.f + tanhf(a.ga[co] * a.v[gi] + a.gb[co]);
+                out[(int64_t)b * a.out_bs + co * a.ng + gi] = val;
             }
         }
     }
@@ -313,6 +418,7 @@ struct qg_cnn {
     qg::DevBuf<float> pbf;              // BF16-rounded forward weights (as FP32)
     qg::DevBuf<float> vnorm;            // (V - mean) / std
     bool dirty = true;                  // derived weight copies out of date
+    bool mfma = false;                  // preconditioner network on the matrix cores
     int64_t adam_t = 0;
     // activation workspace (chunk of bands)
     qg::DevBuf<float> ws;
@@ -398,6 +504,42 @@ static void net_forward(qg_cnn* c, const Weights& W, int nb, const float* u, flo
     a.out_bs = ng;
     hipLaunchKernelGGL((conv3_kernel<kC, 1, IN_PLAIN, OUT_PLAIN, float>), grid, dim3(kConvThreads), 0, s, a, W.w3, h2, w,
                        nullptr);
+    QG_HIP(hipGetLastError());
+}
+
+static dim3 mfma_grid(const qg_grid& g, int nb) {
+    return dim3((unsigned)(((g.n[0] + kMX - 1) / kMX) * ((g.n[1] + kMY - 1) / kMY)), (unsigned)((g.n[2] + kMZ - 1) / kMZ),
+                (unsigned)nb);
+}
+
+// w = N u and q = N^T y on the matrix cores (BF16 weights and activations), scratch s8a/s8b
+static void net_forward_mfma(qg_cnn* c, const Weights& W, int nb, const float* u, float* w, float* s8a, float* s8b,
+                             hipStream_t s) {
+    const qg_grid& g = c->h->grid;
+    const int64_t ng = g.ng();
+    ConvArgs a = conv_args(c);
+    const dim3 grid = mfma_grid(g, nb);
+    a.in_bs = ng; a.out_bs = kC * ng; a.ga = W.a1; a.gb = W.b1;
+    hipLaunchKernelGGL((conv3_mfma_kernel<2, kC, IN_UV, OUT_GATE>), grid, dim3(kMThreads), 0, s, a, W.w1, u, s8a);
+    a.in_bs = kC * ng; a.ga = W.a2; a.gb = W.b2;
+    hipLaunchKernelGGL((conv3_mfma_kernel<kC, kC, IN_PLAIN, OUT_GATE>), grid, dim3(kMThreads), 0, s, a, W.w2, s8a, s8b);
+    a.out_bs = ng;
+    hipLaunchKernelGGL((conv3_mfma_kernel<kC, 1, IN_PLAIN, OUT_PLAIN>), grid, dim3(kMThreads), 0, s, a, W.w3, s8b, w);
+    QG_HIP(hipGetLastError());
+}
+
+static void net_adjoint_mfma(qg_cnn* c, const Weights& W, int nb, const float* y, float* q, float* s8a, float* s8b,
+                             hipStream_t s) {
+    const qg_grid& g = c->h->grid;
+    const int64_t ng = g.ng();
+    ConvArgs a = conv_args(c);
+    const dim3 grid = mfma_grid(g, nb);
+    a.in_bs = ng; a.out_bs = kC * ng; a.ga = W.a2; a.gb = W.b2;
+    hipLaunchKernelGGL((conv3_mfma_kernel<1, kC, IN_PLAIN, OUT_GATE>), grid, dim3(kMThreads), 0, s, a, W.t3, y, s8a);
+    a.in_bs = kC * ng; a.ga = W.a1; a.gb = W.b1;
+    hipLaunchKernelGGL((conv3_mfma_kernel<kC, kC, IN_PLAIN, OUT_GATE>), grid, dim3(kMThreads), 0, s, a, W.t2, s8a, s8b);
+    a.out_bs = ng;
+    hipLaunchKernelGGL((conv3_mfma_kernel<kC, 2, IN_PLAIN, OUT_UV>), grid, dim3(kMThreads), 0, s, a, W.t1, s8b, q);
     QG_HIP(hipGetLastError());
 }
 
@@ -550,8 +692,13 @@ void cnn_precond_apply(qg_cnn* c, int dtype, int nbands, void* r, int64_t ld, hi
             hipLaunchKernelGGL((cnn_convert_kernel<double, float>), dim3(grid1(n)), dim3(256), 0, s,
                                (const double*)r + (int64_t)b0 * ng, u, n, 1.0);
         }
-        net_forward(c, W, nb, u, w, nullptr, s8a, s8b, s);
-        net_adjoint(c, W, nb, w, y, s8a, s8b, nullptr, nullptr, s);
+        if (c->mfma) {
+            net_forward_mfma(c, W, nb, u, w, s8a, s8b, s);
+            net_adjoint_mfma(c, W, nb, w, y, s8a, s8b, s);
+        } else {
+            net_forward(c, W, nb, u, w, nullptr, s8a, s8b, s);
+            net_adjoint(c, W, nb, w, y, s8a, s8b, nullptr, nullptr, s);
+        }
         precond_apply(c->h, QG_PRECOND_KINETIC, c->shift, QG_F32, nb, y, ng, s);
         if (dtype == QG_F32)
             hipLaunchKernelGGL((add_kernel<float>), dim3(grid1(n)), dim3(256), 0, s, (float*)r + (int64_t)b0 * ng, y, n);
@@ -772,6 +919,8 @@ int qg_cnn_create(qg_ham* h, double shift, int max_pairs, uint64_t seed, qg_cnn*
         auto c = std::make_unique<qg_cnn>();
         c->h = h;
         c->shift = shift;
+        const char* mf = std::getenv("QG_CNN_MFMA");
+        c->mfma = mf && *mf && std::atoi(mf) != 0;
         const int64_t ng = h->grid.ng();
         // normalised potential (from the FP64 copy)
         std::vector<double> V(ng);
@@ -843,10 +992,14 @@ int qg_cnn_network(qg_cnn* c, int adjoint, int bf16, int nbands, const float* in
         s8.alloc((size_t)2 * nbands * kC * ng);
         float* a = s8.p;
         float* b = s8.p + (int64_t)nbands * kC * ng;
-        if (adjoint)
+        if (bf16 == 2) {
+            if (adjoint) net_adjoint_mfma(c, W, nbands, in, out, a, b, s);
+            else net_forward_mfma(c, W, nbands, in, out, a, b, s);
+        } else if (adjoint) {
             net_adjoint(c, W, nbands, in, out, a, b, nullptr, nullptr, s);
-        else
+        } else {
             net_forward(c, W, nbands, in, out, nullptr, a, b, s);
+        }
         QG_HIP(hipStreamSynchronize(s));
     });
 }

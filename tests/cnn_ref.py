@@ -40,16 +40,22 @@ def gates(P, v):
     return g1, g2
 
 
-def net(P, v, u):
-    """N u for u of shape (B, 1, Nz, Ny, Nx); v of shape (1, 1, Nz, Ny, Nx)."""
+def bf16(x):
+    """Round to bfloat16 (nearest even) and back: the matrix-core path's activation rounding."""
+    return x.to(torch.bfloat16).to(x.dtype)
+
+
+def net(P, v, u, rnd=lambda x: x):
+    """N u for u of shape (B, 1, Nz, Ny, Nx); v of shape (1, 1, Nz, Ny, Nx).  rnd is applied to
+    every convolution input (bf16 emulates the matrix-core inference kernels)."""
     g1, g2 = gates(P, v)
     x = torch.cat([u, v * u], 1)
-    return conv(conv(conv(x, P["W1"]) * g1, P["W2"]) * g2, P["W3"])
+    return conv(rnd(conv(rnd(conv(rnd(x), P["W1"]) * g1), P["W2"]) * g2), P["W3"])
 
 
-def net_adj(P, v, y):
+def net_adj(P, v, y, rnd=lambda x: x):
     g1, g2 = gates(P, v)
-    x = conv_adj(conv_adj(conv_adj(y, P["W3"]) * g2, P["W2"]) * g1, P["W1"])
+    x = conv_adj(rnd(conv_adj(rnd(conv_adj(rnd(y), P["W3"]) * g2), P["W2"]) * g1), P["W1"])
     return x[:, :1] + v * x[:, 1:]
 
 

@@ -89,6 +89,31 @@ def test_bf16_weights(gpu):
     assert np.abs(out.cpu().numpy() - ref).max() / np.abs(ref).max() < 2e-5
 
 
+def test_mfma_network_matches_bf16_emulation(gpu):
+    """The matrix-core inference kernels (BF16 weights and activations, FP32 accumulation)
+    equal the checker run with the same BF16 roundings."""
+    import torch
+    print("cnn mfma network", flush=True)
+    for dims in ((20, 16, 12), (40, 36, 8), (13, 9, 7)):
+        g, H, V, h = _ham(*dims)
+        net = Q.LearnedPreconditioner(H, 0.8, max_pairs=0, seed=15)
+        rng = np.random.default_rng(6)
+        net.set_params((rng.standard_normal(R.NPARAM) * 0.3).astype(np.float32))
+        pb = torch.from_numpy(net.get_params()).to(torch.bfloat16).to(torch.float64)
+        v, _ = _cpu_views(g, V, h)
+        nb = 2
+        x = rng.standard_normal((nb, g.size)).astype(np.float32)
+        xc = torch.from_numpy(x.astype(np.float64)).reshape(nb, 1, g.n[2], g.n[1], g.n[0])
+        for adjoint in (False, True):
+            xd = torch.from_numpy(x).to(gpu)
+            out = torch.empty_like(xd)
+            net.network(xd, out, adjoint=adjoint, bf16=2)
+            f = R.net_adj if adjoint else R.net
+            ref = f(R.split(pb), v, xc, rnd=R.bf16).reshape(nb, -1).numpy()
+            err = np.linalg.norm(out.cpu().numpy() - ref) / np.linalg.norm(ref)
+            assert err < 2e-3, (dims, adjoint, err)
+
+
 def test_loss_gradient_matches_autograd(gpu):
     """The device gradient (two forward and two backward passes of the project's kernels)
     equals torch autograd of the CPU checker's loss."""
