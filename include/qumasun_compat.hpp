@@ -199,5 +199,37 @@ inline qg_lobpcg_stats Eigen_d_multi(HamiltonianGPU& H, int num_solution, int it
     return st;
 }
 
+// The reference's form with a user operator: OpeA(double* Ax_dev, double* x_dev) applies A to
+// one band (device pointers, FP64), as the per-band lambda of QUMASUN_MPI::mSolveLOBPCG
+// (qumasun_mpi.lobpcg.h:33-35); H supplies the grid.  The solve runs in FP64.
+template <class OperationA>
+inline qg_lobpcg_stats Eigen_d_multi(HamiltonianGPU& H, int num_solution, int iter_max, double* eigen_values,
+                                     double* eigen_vectors_dev, double* keep_P_dev, int SCF_step, OperationA OpeA,
+                                     int64_t grid_size, const qg_lobpcg_opts* opts = nullptr) {
+    struct Ctx {
+        OperationA* op;
+        int64_t ng;
+    } ctx{&OpeA, grid_size};
+    qg_lobpcg_opts o;
+    qg_lobpcg_default_opts(&o);
+    if (opts) o = *opts;
+    o.max_iter = iter_max;
+    o.precision = QG_PREC_F64;
+    o.op_user = &ctx;
+    o.op = [](void* user, int dtype, int nbands, const void* x, void* ax, void* stream) {
+        Ctx* c = static_cast<Ctx*>(user);
+        (void)dtype;
+        (void)hipStreamSynchronize((hipStream_t)stream);
+        for (int k = 0; k < nbands; ++k)
+            (*c->op)(static_cast<double*>(ax) + (int64_t)k * c->ng,
+                     const_cast<double*>(static_cast<const double*>(x)) + (int64_t)k * c->ng);
+        (void)hipDeviceSynchronize();
+    };
+    qg_lobpcg_stats st{};
+    check(qg_lobpcg_d_multi(H.get(), &o, num_solution, eigen_values, eigen_vectors_dev, keep_P_dev, SCF_step, &st),
+          "qg_lobpcg_d_multi");
+    return st;
+}
+
 }  // namespace LOBPCG
 }  // namespace qumasun_gpu

@@ -152,6 +152,12 @@ typedef struct qg_lobpcg_opts {
     double switch_tol;       /* FP32 modes: residual at which FP64 refinement sweeps take over */
     int stall_window;        /* QG_PRECOND_LEARNED: iterations without a new minimum of the
                                 max residual before the kinetic preconditioner takes over */
+    /* Optional foreign operator (the reference's OpeA, lobpcg_d_multi_mpi.h:966-969): when
+     * set, ax = A x for nbands band-major device vectors (ld == Ng, dtype QG_F64 or QG_F32)
+     * is computed by this callback on the given stream instead of by the Hamiltonian; the
+     * Hamiltonian still supplies the grid (and the preconditioner's operator). */
+    void (*op)(void* user, int dtype, int nbands, const void* x_dev, void* ax_dev, void* stream);
+    void* op_user;
 } qg_lobpcg_opts;
 
 typedef struct qg_lobpcg_stats {

@@ -250,7 +250,12 @@ def Eigen_d_multi(ham, X, P, scf_step=0, **opts):
     pp, nb2, _ = _block(P, ham.grid.size)
     if nb != nb2 or X.dtype != torch.float64 or P.dtype != torch.float64:
         raise QGError("X and P must be float64 blocks of the same size")
+    op = opts.pop("op", None)
     o = lobpcg_opts(**opts)
+    if op is not None:
+        # op(dtype, nbands, x_ptr, ax_ptr, stream_ptr): the reference's OpeA on device blocks
+        cb = N.OP_CALLBACK(lambda user, dt, n, x, ax, s: op(dt, n, x, ax, s))
+        o.op = cb
     st = N.LobpcgStats()
     eig = np.zeros(nb)
     check(lib().qg_lobpcg_d_multi(ham._h, ctypes.byref(o), nb, dptr(eig), xp, pp, int(scf_step),

@@ -34,12 +34,16 @@ class ProjSpec(ctypes.Structure):
                 ("table", P(c_double))]
 
 
+# void (*op)(void* user, int dtype, int nbands, const void* x, void* ax, void* stream)
+OP_CALLBACK = ctypes.CFUNCTYPE(None, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p)
+
+
 class LobpcgOpts(ctypes.Structure):
     """qg_lobpcg_opts"""
     _fields_ = [("precision", c_int), ("max_iter", c_int), ("tol", c_double),
                 ("refresh_period", c_int), ("rollback_limit", c_double), ("precond", c_int),
                 ("precond_shift", c_double), ("verbose", c_int), ("switch_tol", c_double),
-                ("stall_window", c_int)]
+                ("stall_window", c_int), ("op", OP_CALLBACK), ("op_user", c_void_p)]
 
 
 class LobpcgStats(ctypes.Structure):

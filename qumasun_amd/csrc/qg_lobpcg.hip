@@ -400,7 +400,9 @@ static void run_lobpcg(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_ho
     auto sync = [&] { QG_HIP(hipStreamSynchronize(s)); };
     auto hpsi = [&](const T* in, T* outp) {
         auto t0 = now();
-        if (h->batch)
+        if (o.op)
+            o.op(o.op_user, dtype, N, in, outp, (void*)s);
+        else if (h->batch)
             batch_service_apply(h->batch, h, dtype, N, in, outp, s);
         else
             hamiltonian_apply(h, dtype, N, in, K, outp, K, s);
@@ -650,6 +652,8 @@ void qg_lobpcg_default_opts(qg_lobpcg_opts* o) {
     o->verbose = 0;
     o->switch_tol = 1.0e-3;
     o->stall_window = 5;
+    o->op = nullptr;
+    o->op_user = nullptr;
 }
 
 int qg_lobpcg_d_multi(qg_ham* h, const qg_lobpcg_opts* opts, int nbands, double* eig, double* X, double* P,

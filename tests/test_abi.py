@@ -108,6 +108,7 @@ def test_compat_header_compiles_and_links(tmp_path):
         "  HamiltonianGPU H(pp, nullptr); H.Apply<float>(1, nullptr, nullptr);\n"
         "  H.Apply<double>(1, nullptr, nullptr);\n"
         "  double e[1]; LOBPCG::Eigen_d_multi(H, 1, 1, e, nullptr, nullptr, 0);\n"
+        "  LOBPCG::Eigen_d_multi(H, 1, 1, e, nullptr, nullptr, 0, [](double* ax, double* x) { (void)ax; (void)x; }, 8);\n"
         "  return 0;\n}\n")
     libdir = os.path.dirname(N.LIB_PATH)
     exe = tmp_path / "compat"
