@@ -176,10 +176,22 @@ def _cell40():
 
 
 def _solve(H, g, nb, **kw):
+    """Solve to kw['tol'] within kw['max_iter'] iterations, continuing after a rollback with
+    the kept P block (as the next SCF step's call would)."""
     import torch
     X = torch.from_numpy(Q.random_states(g.size, nb, g.dvol)).cuda()
     P = torch.zeros_like(X)
-    e, st = Q.Eigen_d_multi(H, X, P, 0, **kw)
+    budget = kw.pop("max_iter")
+    total, step, fb = 0, 0, -1
+    while True:
+        e, st = Q.Eigen_d_multi(H, X, P, step, max_iter=budget - total, **kw)
+        if fb < 0 and st["fallback_iter"] >= 0:
+            fb = total + st["fallback_iter"]
+        total += st["iterations"]
+        step += 1
+        if st["max_residual"] <= kw.get("tol", 0.0) or total >= budget or st["iterations"] == 0:
+            break
+    st["iterations"], st["fallback_iter"] = total, fb
     return e, st
 
 
