@@ -438,11 +438,14 @@ static void run_lobpcg(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_ho
     stats.max_residual = maxres;
 
     const bool precond = o.precond != QG_PRECOND_NONE;
-    // FP32 working precision: columns of R, P (and AP) are normalised before every Gram
-    // product and X is renormalised after every update, all blocks are recomputed each
-    // iteration, and a norm drift is absorbed instead of rolling back (the reference's
-    // recurrences and rollback assume FP64 arithmetic)
-    const bool robust = sizeof(T) != 8;
+    // Robust form (FP32 working precision, or any preconditioner): columns of R, P (and AP)
+    // are normalised before every Gram product and X is renormalised after every update, all
+    // blocks are recomputed each iteration, and a norm drift is absorbed instead of rolling
+    // back.  The reference's recurrences and rollback assume FP64 arithmetic and the raw
+    // residual: with K R the recurrence identities no longer hold, and a rollback (P = R)
+    // followed by another call on the same operator puts K R twice into the basis.  The
+    // unpreconditioned FP64 run keeps the reference's iteration exactly.
+    const bool robust = sizeof(T) != 8 || precond;
     const bool direct_xr = !precond && !robust;   // DIRECT_S_MATRIX_XR_RP needs R = AX - XE
     auto blkS = [&](DevBuf<double>& S, int bi, int bj) { return S.p + (int64_t)N * bi + (int64_t)N * n3 * bj; };
     bool large_diff = false;
