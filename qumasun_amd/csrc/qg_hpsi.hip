@@ -61,6 +61,7 @@ struct FusedArgs {
     const int* atom_coef_off;
     int ncoef;
     int max_atom;
+    int nlp;                   // z-planes per projector phase (accumulator ring depth, 1..4)
 };
 
 // Buffer-resource access: SGPR base + 32-bit byte offsets (VGPR per point, SGPR per plane).
@@ -115,10 +116,11 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const StencilCoef
     unsigned char* lp = smem_raw;
     T* planes = reinterpret_cast<T*>(lp);
     if (MODE != MODE_NL) lp += 2 * (size_t)NB * lds_band * S;
-    T* nl_l = reinterpret_cast<T*>(lp);       // per-point projector term, NB x tile_pts
+    T* nl_l = reinterpret_cast<T*>(lp);       // per-point projector term, nlp x NB x tile_pts
     T* ec_l = nullptr;                        // E*coef per (local atom, slot, band)
+    const int nlp = a.nlp;
     if (a.has_nl) {
-        lp += (size_t)NB * tile_pts * S;
+        lp += (size_t)nlp * NB * tile_pts * S;
         ec_l = reinterpret_cast<T*>(lp);
     }
 
@@ -195,7 +197,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const StencilCoef
                 v = (T)(a.coef_energy[cb + j] * a.coef[(int64_t)(b0 + b) * a.ncoef + cb + j]);
             ec_l[i] = v;
         }
-        for (int i = tid; i < NB * tile_pts; i += nthr) nl_l[i] = T(0);
+        for (int i = tid; i < nlp * NB * tile_pts; i += nthr) nl_l[i] = T(0);
     }
 
     T q[NB][PPT][9];
@@ -258,11 +260,17 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const StencilCoef
         if (z + 1 < z1) prefetch(z + 1);
         __syncthreads();
 
-        // projector term: every (image, sphere point) reaching this plane is one work item
-        if (MODE != MODE_LAP && a.has_nl) {
+        // projector term: every (image, sphere point) reaching the next nlp planes is one work
+        // item, accumulated into the ring slot of its plane
+        if (MODE != MODE_LAP && a.has_nl && (z - z0) % nlp == 0) {
             const size_t tp = (size_t)ytile * nz + z;
+            const int zend = min(z + nlp, z1);
             const int64_t io0 = a.item_off[tp];
-            const int nitem = (int)(a.item_off[tp + 1] - io0);
+            const int64_t io_end = a.item_off[(size_t)ytile * nz + zend];
+            const int nitem = (int)(io_end - io0);
+            int rb[3];   // item index (relative to io0) where planes z+1, z+2, z+3 start
+#pragma unroll
+            for (int k = 0; k < 3; ++k) rb[k] = (z + k + 1 < zend) ? (int)(a.item_off[tp + k + 1] - io0) : nitem;
             for (int base = 0; base < nitem; base += nthr * kItemPT) {
                 uint32_t meta[kItemPT];
                 T bj[kItemPT][NJS];
@@ -282,12 +290,15 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const StencilCoef
                     if (meta[u] == 0xffffffffu) continue;
                     const int pt = (int)(meta[u] & 0xffffu);
                     const T* ec = ec_l + (int)(meta[u] >> 16) * NJS * NB;
+                    const int it = base + tid + nthr * u;
+                    const int pr = (it >= rb[0]) + (it >= rb[1]) + (it >= rb[2]);
+                    T* acc_base = nl_l + (size_t)pr * NB * tile_pts;
 #pragma unroll
                     for (int b = 0; b < NB; ++b) {
                         T acc = T(0);
 #pragma unroll
                         for (int j = 0; j < NJS; ++j) acc += ec[j * NB + b] * bj[u][j];
-                        atomicAdd(&nl_l[b * tile_pts + pt], acc);
+                        atomicAdd(&acc_base[b * tile_pts + pt], acc);
                     }
                 }
             }
@@ -305,9 +316,9 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const StencilCoef
                     res[b][k] = (a.accumulate && lc[k] >= 0) ? bload<T>(rsrc_out[b], vo[k], sz) : T(0);
                 }
                 if (MODE != MODE_LAP && a.has_nl && lc[k] >= 0) {
-                    T* nlp = &nl_l[b * tile_pts + tid + nthr * k];
-                    res[b][k] += *nlp;
-                    *nlp = T(0);
+                    T* slot = &nl_l[((size_t)((z - z0) % nlp) * NB + b) * tile_pts + tid + nthr * k];
+                    res[b][k] += *slot;
+                    *slot = T(0);
                 }
             }
         if constexpr (MODE != MODE_NL) {
@@ -516,6 +527,7 @@ static StencilCoef<T> make_coef(const qg_grid& g, double coef) {
 
 struct Plan {
     int ty, tz, ppt, nb, nthr;
+    int nlp = 1;
 };
 
 static int round64(int v) { return (v + 63) / 64 * 64; }
@@ -601,6 +613,7 @@ static Plan make_plan(const qg_grid& g, int dtype, int nbands, int mode) {
     int tz = nz;
     while (tz > 32 && (int64_t)ntile * nchunk * ((nz + tz - 1) / tz) < 4096) tz = (tz + 1) / 2;
     p.tz = std::max(1, std::min(nz, env_int("QG_PLAN_TZ", tz)));
+    p.nlp = std::max(1, std::min(4, env_int("QG_PLAN_NLP", 1)));
     return p;
 }
 
@@ -669,6 +682,7 @@ static FusedSetup<T> setup_fused(const qg_grid& g, int nbands, int64_t ld_in, in
     a.ld_in = ld_in; a.ld_out = ld_out;
     a.accumulate = accumulate;
     a.has_nl = 0;
+    a.nlp = p.nlp;
     size_t lds = (MODE == MODE_NL) ? 0 : 2 * (size_t)p.nb * (p.ty + 8) * (g.n[0] + 8) * sizeof(T);
     int njs = 4;
     if (nl && nl->geo.ncoef > 0 && MODE != MODE_LAP) {
@@ -686,7 +700,7 @@ static FusedSetup<T> setup_fused(const qg_grid& g, int nbands, int64_t ld_in, in
         a.atom_coef_off = nl->d_atom_coef_off.p;
         a.ncoef = nl->geo.ncoef;
         a.max_atom = std::max(1, t.max_atom);
-        lds += (size_t)p.nb * p.ty * g.n[0] * sizeof(T) + (size_t)a.max_atom * njs * p.nb * sizeof(T);
+        lds += (size_t)p.nlp * p.nb * p.ty * g.n[0] * sizeof(T) + (size_t)a.max_atom * njs * p.nb * sizeof(T);
     }
     QG_REQUIRE(lds <= 65536, "tile staging does not fit in 64 KiB of LDS");
     const StencilCoef<T> sc = make_coef<T>(g, coef);

@@ -438,14 +438,18 @@ static void run_lobpcg(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_ho
     stats.max_residual = maxres;
 
     const bool precond = o.precond != QG_PRECOND_NONE;
-    // Robust form (FP32 working precision, or any preconditioner): columns of R, P (and AP)
-    // are normalised before every Gram product and X is renormalised after every update, all
-    // blocks are recomputed each iteration, and a norm drift is absorbed instead of rolling
-    // back.  The reference's recurrences and rollback assume FP64 arithmetic and the raw
-    // residual: with K R the recurrence identities no longer hold, and a rollback (P = R)
-    // followed by another call on the same operator puts K R twice into the basis.  The
-    // unpreconditioned FP64 run keeps the reference's iteration exactly.
-    const bool robust = sizeof(T) != 8 || precond;
+    // FP32 working precision: columns of R, P (and AP) are normalised before every Gram
+    // product and X is renormalised after every update, all blocks are recomputed each
+    // iteration, and a norm drift is absorbed instead of rolling back (the reference's
+    // recurrences and rollback assume FP64 arithmetic).
+    const bool robust = sizeof(T) != 8;
+    // With a preconditioner (an extension of the reference) a norm drift or a failed
+    // generalized eigensolve restarts the iteration in place instead of leaving it: the step
+    // is discarded, P is dropped (the next basis is [X R]) and the residual is recomputed.
+    // The reference's exit with P = R would put K R twice into the basis of a following call
+    // on the same operator.  The unpreconditioned run keeps the reference's rollback exactly.
+    const bool soft_restart = precond || robust;
+    bool p_valid = !first_time;
     const bool direct_xr = !precond && !robust;   // DIRECT_S_MATRIX_XR_RP needs R = AX - XE
     auto blkS = [&](DevBuf<double>& S, int bi, int bj) { return S.p + (int64_t)N * bi + (int64_t)N * n3 * bj; };
     bool large_diff = false;
@@ -466,7 +470,7 @@ static void run_lobpcg(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_ho
             sync();
             stats.t_precond += secs(t0, now());
         }
-        const bool use_p = !(first_time && iter == 0);
+        const bool use_p = p_valid;
         if (robust) {
             hipLaunchKernelGGL((coldot_kernel<T>), dim3(N), dim3(256), 0, s, R, R, K, K, st.norm_a.p);
             hipLaunchKernelGGL((colscale_kernel<T>), dim3(grid_for(K) / 4 + 1, N), dim3(256), 0, s, R, (T*)nullptr,
@@ -514,7 +518,22 @@ static void run_lobpcg(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_ho
         sync();
         stats.t_eig += secs(t0, now());
         if (info != 0 && o.verbose)
-            std::fprintf(stderr, "lobpcg: dsygvd info = %d at iteration %d (ignored, as the reference)\n", info, iter);
+            std::fprintf(stderr, "lobpcg: dsygvd info = %d at iteration %d%s\n", info, iter,
+                         soft_restart ? " (restart without P)" : " (ignored, as the reference)");
+        auto restart = [&] {
+            // discard this step: X and AX stay, P and AP are dropped, R = AX - X E again
+            QG_HIP(hipMemsetAsync(P, 0, blk, s));
+            QG_HIP(hipMemsetAsync(AP, 0, blk, s));
+            hipLaunchKernelGGL((residual_kernel<T>), dim3(N), dim3(256), 0, s, AX, X, st.eig.p, R, K, K, st.res2.p);
+            p_valid = false;
+            large_diff = true;   // full Gram on the next iteration
+            ++stats.rollbacks;
+            sync();
+        };
+        if (info != 0 && soft_restart) {
+            restart();
+            continue;
+        }
 
         const double* Cx = st.A3.p;
         const double* Cr = st.A3.p + N;
@@ -541,6 +560,11 @@ static void run_lobpcg(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_ho
         double maxd = 0.0;
         for (int k = 0; k < N; ++k) maxd = std::max(maxd, std::fabs(1.0 - nx[k]));
         large_diff = !robust && maxd > o.rollback_limit;
+        if (large_diff && soft_restart) {
+            if (o.verbose) std::fprintf(stderr, "lobpcg: |1-|x|^2| = %g at iteration %d, restart without P\n", maxd, iter);
+            restart();
+            continue;
+        }
         if (large_diff) {
             // rollback (:1322-1346): P = R, X keeps the previous vectors, leave the loop
             QG_HIP(hipMemcpyAsync(P, R, blk, hipMemcpyDeviceToDevice, s));
@@ -556,6 +580,7 @@ static void run_lobpcg(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_ho
             break;
         }
         QG_HIP(hipMemcpyAsync(X, Tm, blk, hipMemcpyDeviceToDevice, s));
+        p_valid = true;
         for (int i = 0; i < N; ++i) eig_host[i] = ev[i];
         QG_HIP(hipMemcpyAsync(st.eig.p, st.D.p, N * 8, hipMemcpyDeviceToDevice, s));
         bool stop = final_step;

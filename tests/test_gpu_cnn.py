@@ -220,9 +220,11 @@ def test_learned_lobpcg_cfg0(gpu):
     for e, st in ((e_l, st_l), (e_k, st_k), (e_k1, st_k1)):
         assert st["max_residual"] <= 1e-8
         assert np.abs(e - ref).max() < 1e-8
-    # the stall fallback: a window of one iteration hands over to the kinetic preconditioner
-    e_f, st_f = _solve(H, g, nb, max_iter=300, tol=1e-8, precond=N.QG_PRECOND_LEARNED, precond_shift=shift,
-                       stall_window=1)
+    # the stall fallback: an FP32 run asked for a residual below FP32 resolution stalls, and
+    # the kinetic preconditioner takes over after stall_window iterations without progress
+    e_f, st_f = _solve(H, g, nb, max_iter=150, tol=1e-10, precision=N.QG_PREC_F32, switch_tol=1e-12,
+                       precond=N.QG_PRECOND_LEARNED, precond_shift=shift, stall_window=4)
+    print("fallback at iteration", st_f["fallback_iter"], "max residual", st_f["max_residual"], flush=True)
     assert st_f["fallback_iter"] >= 1
-    assert np.abs(e_f - ref).max() < 1e-8
+    assert np.abs(e_f - ref).max() < 1e-5
     net.detach()
