@@ -140,6 +140,14 @@ int qg_hamiltonian_last_chunks(qg_ham* h, int* nchunks);
                                   Hamiltonian (qg_hamiltonian_set_cnn); falls back to
                                   QG_PRECOND_KINETIC when the residual stalls */
 
+/* Rayleigh-Ritz basis handling.  QG_BASIS_REFERENCE: the reference's [X R P] generalized
+ * problem with Gram recurrences and the rollback exit (FP64 without a preconditioner only).
+ * QG_BASIS_ORTHO: [X W P] kept orthonormal (W = K R orthogonalised against X, P against X
+ * and W, each orthonormalised by SVQB with dependent directions dropped), standard symmetric
+ * Rayleigh-Ritz problem; always used with a preconditioner or in the FP32 modes. */
+#define QG_BASIS_REFERENCE 0
+#define QG_BASIS_ORTHO 1
+
 typedef struct qg_lobpcg_opts {
     int precision;           /* QG_PREC_* */
     int max_iter;            /* iter_max of Eigen_d_multi_mpi */
@@ -158,6 +166,7 @@ typedef struct qg_lobpcg_opts {
      * Hamiltonian still supplies the grid (and the preconditioner's operator). */
     void (*op)(void* user, int dtype, int nbands, const void* x_dev, void* ax_dev, void* stream);
     void* op_user;
+    int basis;               /* QG_BASIS_*: used for FP64 runs without a preconditioner */
 } qg_lobpcg_opts;
 
 typedef struct qg_lobpcg_stats {

@@ -20,6 +20,8 @@ QG_PREC_F32 = 2
 QG_PRECOND_NONE = 0
 QG_PRECOND_KINETIC = 1
 QG_PRECOND_LEARNED = 2
+QG_BASIS_REFERENCE = 0
+QG_BASIS_ORTHO = 1
 
 c_int, c_double, c_void_p, c_int64, c_uint32, c_uint64 = (ctypes.c_int, ctypes.c_double, ctypes.c_void_p,
                                                            ctypes.c_int64, ctypes.c_uint32, ctypes.c_uint64)
@@ -43,7 +45,7 @@ class LobpcgOpts(ctypes.Structure):
     _fields_ = [("precision", c_int), ("max_iter", c_int), ("tol", c_double),
                 ("refresh_period", c_int), ("rollback_limit", c_double), ("precond", c_int),
                 ("precond_shift", c_double), ("verbose", c_int), ("switch_tol", c_double),
-                ("stall_window", c_int), ("op", OP_CALLBACK), ("op_user", c_void_p)]
+                ("stall_window", c_int), ("op", OP_CALLBACK), ("op_user", c_void_p), ("basis", c_int)]
 
 
 class LobpcgStats(ctypes.Structure):

@@ -662,6 +662,339 @@ static void run_lobpcg(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_ho
     stats.iterations = iter;
 }
 
+
+// ------------------------------------------------------- orthonormal-basis LOBPCG ----
+// The iteration used with a preconditioner or in FP32: the Rayleigh-Ritz basis [X W P] is kept
+// orthonormal (Hetmaniuk and Lehoucq's basis selection), so the small problem is a standard
+// symmetric eigenproblem and near-dependence of W or P with X cannot make it indefinite:
+//   W = K R;  W -= X (X^T W);  W = SVQB(W)
+//   P -= X (X^T P) + W (W^T P);  P = SVQB(P)          (AP follows with the same coefficients)
+//   [X W P]^T H [X W P] c = theta c  (lowest N)  ->  X', P' = W Cw + P Cp, AX', AP'
+// SVQB(V): G = V^T V, D = diag(G)^-1/2, D G D = U diag(theta) U^T, V <- V D U theta^-1/2, with
+// the directions of theta <= tau theta_max dropped (zero columns, excluded from the small
+// problem by a large diagonal).  The Ritz values and the residual definition are the
+// reference's (lobpcg_d_multi_mpi.h:1280-1320).
+
+// d_i = G_ii^-1/2 (0 for an empty column); G <- D G D
+__global__ void svqb_scale_kernel(double* G, int N, int ld, double* d) {
+    const int j = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double gi = G[i + (int64_t)ld * i], gj = G[j + (int64_t)ld * j];
+    const double di = gi > 0.0 ? 1.0 / sqrt(gi) : 0.0, dj = gj > 0.0 ? 1.0 / sqrt(gj) : 0.0;
+    G[i + (int64_t)ld * j] *= di * dj;
+    if (j == 0) d[i] = di;
+}
+
+// T_ij = d_i U_ij theta_j^-1/2 for kept directions, 0 otherwise; keep_j = 1 / 0
+__global__ void svqb_transform_kernel(const double* U, int N, int ld, const double* theta, const double* d,
+                                      double tau, double* Tm, int ldt, int* keep) {
+    const int j = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double tmax = theta[N - 1];
+    const bool k = theta[j] > tau * tmax && theta[j] > 0.0;
+    Tm[i + (int64_t)ldt * j] = k ? d[i] * U[i + (int64_t)ld * j] / sqrt(theta[j]) : 0.0;
+    if (i == 0) keep[j] = k ? 1 : 0;
+}
+
+// dropped basis directions: large diagonal in the Rayleigh-Ritz matrix, so that none of the
+// lowest N eigenpairs uses them
+__global__ void mask_diag_kernel(double* A, int ld, const int* keep, int off, int N, double big) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= N || keep[j]) return;
+    A[(off + j) + (int64_t)ld * (off + j)] = big;
+}
+
+// copy an m x n FP64 matrix (ld) into a contiguous FP32 one with an optional sign
+__global__ void narrow_kernel(const double* A, int m, int n, int ld, float* B, double sgn) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= (int64_t)m * n) return;
+    B[i] = (float)(sgn * A[(i % m) + (int64_t)ld * (i / m)]);
+}
+
+__global__ void diag_copy_kernel(const double* D, double* e, int N) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) e[i] = D[i];
+}
+
+template <class T>
+static void run_lobpcg_ortho(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_host, double* X64, double* P64,
+                             int scf_step, qg_lobpcg_stats& stats, int iter_offset = 0) {
+    const qg_grid& g = h->grid;
+    const int64_t K = g.ng();
+    const double dvol = g.dvol();
+    const int dtype = sizeof(T) == 8 ? QG_F64 : QG_F32;
+    if (!h->lobpcg) h->lobpcg = new LobpcgState;
+    LobpcgState& st = *h->lobpcg;
+    if (!st.blas) {
+        QG_BLAS(rocblas_create_handle(&st.blas));
+        QG_HIP(hipStreamCreateWithFlags(&st.stream, hipStreamNonBlocking));
+        QG_BLAS(rocblas_set_stream(st.blas, st.stream));
+        for (auto& e : st.ev) QG_HIP(hipEventCreate(&e));
+    }
+    hipStream_t s = st.stream;
+    const size_t blk = (size_t)K * N * sizeof(T);
+    st.R.alloc(blk); st.AX.alloc(blk); st.AR.alloc(blk); st.AP.alloc(blk); st.T.alloc(blk);
+    if (sizeof(T) != 8) { st.Xw.alloc(blk); st.Pw.alloc(blk); }
+    const int n3 = 3 * N;
+    const int64_t nn = (int64_t)N * N, nn9 = (int64_t)n3 * n3;
+    st.A3.alloc(nn9); st.D.alloc(n3); st.E.alloc(n3); st.info.alloc(1);
+    st.eig.alloc(N); st.norm_a.alloc(N); st.norm_b.alloc(N); st.res2.alloc(N);
+    st.C32.alloc(nn9);
+    if (o.precision == QG_PREC_MIXED) st.gram_ws.alloc(gram_bf16_workspace(K, N, N));
+    st.split_ws.alloc(Ops<T>::split_workspace(K, N));
+    DevBuf<double> Gm, Tr, dsc;
+    Gm.alloc(nn); Tr.alloc(nn); dsc.alloc(N);
+    DevBuf<int> keep;
+    keep.alloc(n3);
+
+    Ops<T> op{st, s, N, K, o.precision};
+    // X and P live in the caller's FP64 arrays for T = double; blocks are swapped by pointer
+    T* X = (sizeof(T) == 8) ? reinterpret_cast<T*>(X64) : op.blk(st.Xw);
+    T* P = (sizeof(T) == 8) ? reinterpret_cast<T*>(P64) : op.blk(st.Pw);
+    T* W = op.blk(st.R);
+    T* AX = op.blk(st.AX);
+    T* AW = op.blk(st.AR);
+    T* AP = op.blk(st.AP);
+    T* Tm = op.blk(st.T);
+    const int64_t KN = K * N;
+    const int gsz = grid_for(KN);
+    auto now = [] { return std::chrono::steady_clock::now(); };
+    auto secs = [](auto a, auto b) { return std::chrono::duration<double>(b - a).count(); };
+    auto sync = [&] { QG_HIP(hipStreamSynchronize(s)); };
+    auto hpsi = [&](const T* in, T* outp) {
+        auto t0 = now();
+        if (o.op)
+            o.op(o.op_user, dtype, N, in, outp, (void*)s);
+        else if (h->batch)
+            batch_service_apply(h->batch, h, dtype, N, in, outp, s);
+        else
+            hamiltonian_apply(h, dtype, N, in, K, outp, K, s);
+        sync();
+        stats.t_hpsi += secs(t0, now());
+        stats.hpsi_calls += N;
+    };
+    // Out = alpha A C + beta Out, C: N x N FP64 (ld ldc), A: K x N
+    auto upd = [&](const T* A, const double* C, int ldc, double alpha, double beta, T* Out) {
+        if constexpr (sizeof(T) == 8) {
+            QG_BLAS(rocblas_dgemm(st.blas, rocblas_operation_none, rocblas_operation_none, (int)K, N, N, &alpha, A,
+                                  (int)K, C, ldc, &beta, Out, (int)K));
+        } else {
+            hipLaunchKernelGGL(narrow_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, C, N, N, ldc,
+                               st.C32.p, 1.0);
+            const float a = (float)alpha, b = (float)beta;
+            QG_BLAS(rocblas_sgemm(st.blas, rocblas_operation_none, rocblas_operation_none, (int)K, N, N, &a, A,
+                                  (int)K, st.C32.p, N, &b, Out, (int)K));
+        }
+    };
+    auto eigh = [&](double* A, int n, int lda, double* w) -> int {
+        auto t0 = now();
+        QG_BLAS(rocsolver_dsyevd(st.blas, rocblas_evect_original, rocblas_fill_upper, n, A, lda, w, st.E.p, st.info.p));
+        int info = 0;
+        QG_HIP(hipMemcpyAsync(&info, st.info.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        sync();
+        stats.t_eig += secs(t0, now());
+        return info;
+    };
+    const double tau = sizeof(T) == 8 ? 1e-12 : 1e-6;
+    // V <- SVQB(V); keep flags into keep_out.  Returns the smallest kept theta over the
+    // largest (1: already orthonormal; small: the transform amplified rounding, so callers
+    // orthogonalise once more).
+    std::vector<double> theta(N);
+    auto svqb = [&](T*& V, int* keep_out, bool drop) -> double {
+        auto t0 = now();
+        op.gram(V, V, Gm.p, N, true);
+        hipLaunchKernelGGL(svqb_scale_kernel, dim3((N + 255) / 256, N), dim3(256), 0, s, Gm.p, N, N, dsc.p);
+        sync();
+        stats.t_gram += secs(t0, now());
+        const int info = eigh(Gm.p, N, N, st.D.p);
+        QG_REQUIRE(info == 0, "SVQB eigensolve failed");
+        hipLaunchKernelGGL(svqb_transform_kernel, dim3((N + 255) / 256, N), dim3(256), 0, s, Gm.p, N, N, st.D.p,
+                           dsc.p, drop ? tau : 0.0, Tr.p, N, keep_out);
+        QG_HIP(hipMemcpyAsync(theta.data(), st.D.p, N * 8, hipMemcpyDeviceToHost, s));
+        t0 = now();
+        upd(V, Tr.p, N, 1.0, 0.0, Tm);
+        std::swap(V, Tm);
+        sync();
+        stats.t_update += secs(t0, now());
+        const double tmax = theta[N - 1];
+        double tmin = tmax;
+        for (int j = 0; j < N; ++j)
+            if (theta[j] > (drop ? tau : 0.0) * tmax && theta[j] > 0.0) tmin = std::min(tmin, theta[j]);
+        return tmax > 0.0 ? tmin / tmax : 1.0;
+    };
+    const double reortho = 1e-3;   // second pass when the first SVQB amplified by more than ~30
+    // Rayleigh-Ritz on the (orthonormal) columns of V with AV: V, AV rotated to Ritz vectors
+    auto rr_single = [&](T*& V, T*& AV) {
+        op.gram(V, AV, st.A3.p, N);
+        const int info = eigh(st.A3.p, N, N, st.D.p);
+        QG_REQUIRE(info == 0, "Rayleigh-Ritz eigensolve failed");
+        upd(V, st.A3.p, N, 1.0, 0.0, Tm);
+        std::swap(V, Tm);
+        upd(AV, st.A3.p, N, 1.0, 0.0, Tm);
+        std::swap(AV, Tm);
+        hipLaunchKernelGGL(diag_copy_kernel, dim3((N + 255) / 256), dim3(256), 0, s, st.D.p, st.eig.p, N);
+    };
+    auto residual = [&](double& maxres) {
+        hipLaunchKernelGGL((residual_kernel<T>), dim3(N), dim3(256), 0, s, AX, X, st.eig.p, W, K, K, st.res2.p);
+        std::vector<double> r2(N);
+        QG_HIP(hipMemcpyAsync(r2.data(), st.res2.p, N * 8, hipMemcpyDeviceToHost, s));
+        QG_HIP(hipMemcpyAsync(eig_host, st.eig.p, N * 8, hipMemcpyDeviceToHost, s));
+        sync();
+        maxres = 0.0;
+        for (int k = 0; k < N; ++k) maxres = std::max(maxres, std::sqrt(r2[k]));
+    };
+
+    // entry: X scaled by sqrt(dVol) (lobpcg_d_multi_mpi.h:954-956), made orthonormal and
+    // rotated to Ritz vectors
+    const double sq = std::sqrt(dvol);
+    if (sizeof(T) == 8) {
+        hipLaunchKernelGGL((scale_kernel<double>), dim3(gsz), dim3(256), 0, s, X64, KN, sq);
+    } else {
+        hipLaunchKernelGGL((convert_kernel<double, float>), dim3(gsz), dim3(256), 0, s, X64, (float*)X, KN, sq);
+        hipLaunchKernelGGL((convert_kernel<double, float>), dim3(gsz), dim3(256), 0, s, P64, (float*)P, KN, sq);
+    }
+    svqb(X, keep.p, false);
+    hpsi(X, AX);
+    rr_single(X, AX);
+    bool p_valid = scf_step != 0;   // the kept P block (AP is formed after its orthogonalisation)
+    if (p_valid && sizeof(T) == 8)
+        hipLaunchKernelGGL((scale_kernel<double>), dim3(gsz), dim3(256), 0, s, P64, KN, sq);
+    double maxres = 0.0;
+    residual(maxres);
+    stats.max_residual = maxres;
+
+    bool learned = o.precond == QG_PRECOND_LEARNED && h->cnn && stats.fallback_iter < 0;
+    double best_res = maxres;
+    int best_iter = 0;
+    int iter = 0;
+    const double big = 1e30;
+    for (iter = 0; iter < o.max_iter; ++iter) {
+        if (o.tol > 0.0 && maxres <= o.tol) break;
+        // W = K R, orthogonal to X, orthonormal
+        auto t0 = now();
+        if (o.precond != QG_PRECOND_NONE) {
+            if (learned) cnn_precond_apply(h->cnn, dtype, N, W, K, s);
+            else precond_apply(h, QG_PRECOND_KINETIC, o.precond_shift, dtype, N, W, K, s);
+            sync();
+            stats.t_precond += secs(t0, now());
+        }
+        for (int pass = 0; pass < 2; ++pass) {
+            t0 = now();
+            op.gram(X, W, Gm.p, N, true);
+            upd(X, Gm.p, N, -1.0, 1.0, W);
+            sync();
+            stats.t_gram += secs(t0, now());
+            if (svqb(W, keep.p + N, true) > reortho) break;
+        }
+        hpsi(W, AW);
+        if (p_valid) {
+            // P against X and W, orthonormal; AP recomputed (a transformed AP would carry the
+            // SVQB amplification of its rounding into the Rayleigh-Ritz matrix)
+            for (int pass = 0; pass < 2; ++pass) {
+                t0 = now();
+                op.gram(X, P, Gm.p, N, true);
+                upd(X, Gm.p, N, -1.0, 1.0, P);
+                op.gram(W, P, Gm.p, N, true);
+                upd(W, Gm.p, N, -1.0, 1.0, P);
+                sync();
+                stats.t_gram += secs(t0, now());
+                if (svqb(P, keep.p + 2 * N, true) > reortho) break;
+            }
+            hpsi(P, AP);
+        }
+        // Rayleigh-Ritz: upper blocks of [X W P]^T [AX AW AP]
+        t0 = now();
+        const int nb = p_valid ? n3 : 2 * N;
+        double* A = st.A3.p;
+        auto blkA = [&](int bi, int bj) { return A + (int64_t)N * bi + (int64_t)nb * N * bj; };
+        op.gram(X, AX, blkA(0, 0), nb);
+        op.gram(X, AW, blkA(0, 1), nb);
+        op.gram(W, AW, blkA(1, 1), nb);
+        if (p_valid) {
+            op.gram(X, AP, blkA(0, 2), nb);
+            op.gram(W, AP, blkA(1, 2), nb);
+            op.gram(P, AP, blkA(2, 2), nb);
+        }
+        hipLaunchKernelGGL(mask_diag_kernel, dim3((2 * N + 255) / 256), dim3(256), 0, s, A, nb, keep.p + N, N, N, big);
+        if (p_valid)
+            hipLaunchKernelGGL(mask_diag_kernel, dim3((N + 255) / 256), dim3(256), 0, s, A, nb, keep.p + 2 * N, 2 * N,
+                               N, big);
+        sync();
+        stats.t_gram += secs(t0, now());
+        const int info = eigh(A, nb, nb, st.D.p);
+        if (info != 0) {
+            // drop P and retry the iteration from X
+            if (o.verbose) std::fprintf(stderr, "lobpcg: Rayleigh-Ritz info %d at iteration %d, restart without P\n", info, iter);
+            ++stats.rollbacks;
+            p_valid = false;
+            residual(maxres);
+            continue;
+        }
+        // X' = X Cx + W Cw + P Cp,  P' = W Cw + P Cp  (and the same for AX, AP)
+        t0 = now();
+        const double* Cx = A;
+        const double* Cw = A + N;
+        const double* Cp = A + 2 * N;
+        upd(W, Cw, nb, 1.0, 0.0, Tm);                          // Tm = P'
+        if (p_valid) upd(P, Cp, nb, 1.0, 1.0, Tm);
+        QG_HIP(hipMemcpyAsync(P, Tm, blk, hipMemcpyDeviceToDevice, s));
+        upd(X, Cx, nb, 1.0, 1.0, P);                           // old P block = X'
+        std::swap(X, P);                                       // X = X', P = old X block
+        std::swap(P, Tm);                                      // P = P', Tm = free
+        upd(AW, Cw, nb, 1.0, 0.0, Tm);                         // Tm = AP'
+        if (p_valid) upd(AP, Cp, nb, 1.0, 1.0, Tm);
+        QG_HIP(hipMemcpyAsync(AP, Tm, blk, hipMemcpyDeviceToDevice, s));
+        upd(AX, Cx, nb, 1.0, 1.0, AP);                         // old AP block = AX'
+        std::swap(AX, AP);
+        std::swap(AP, Tm);
+        hipLaunchKernelGGL(diag_copy_kernel, dim3((N + 255) / 256), dim3(256), 0, s, st.D.p, st.eig.p, N);
+        // keep X normalised against rounding (AX with it)
+        hipLaunchKernelGGL((coldot_kernel<T>), dim3(N), dim3(256), 0, s, X, X, K, K, st.norm_a.p);
+        hipLaunchKernelGGL((colscale_kernel<T>), dim3(grid_for(K) / 4 + 1, N), dim3(256), 0, s, X, AX, st.norm_a.p, K,
+                           K);
+        sync();
+        stats.t_update += secs(t0, now());
+        p_valid = true;
+        residual(maxres);
+        stats.max_residual = maxres;
+        if (o.verbose >= 2) std::fprintf(stderr, "lobpcg(ortho) it %d: e0 %.10f max|r| %.3e\n", iter, eig_host[0], maxres);
+        if (maxres < best_res) {
+            best_res = maxres;
+            best_iter = iter;
+        } else if (learned && o.stall_window > 0 && iter - best_iter >= o.stall_window) {
+            learned = false;
+            stats.fallback_iter = iter_offset + iter + 1;
+            if (o.verbose) std::fprintf(stderr, "lobpcg: learned preconditioner stalled, kinetic from iteration %d\n",
+                                        stats.fallback_iter);
+        }
+    }
+    stats.iterations = iter;
+    stats.max_residual = maxres;
+    // hand back: X (Ritz vectors) normalised with dVol, P in the grid normalisation
+    const double isq = 1.0 / sq;
+    if constexpr (sizeof(T) == 8) {
+        // the blocks were swapped by pointer: move P out of the caller's X array first
+        if (P == reinterpret_cast<T*>(X64) && X != reinterpret_cast<T*>(X64)) {
+            QG_HIP(hipMemcpyAsync(Tm, P, blk, hipMemcpyDeviceToDevice, s));
+            std::swap(P, Tm);
+        }
+        if (X != reinterpret_cast<T*>(X64)) QG_HIP(hipMemcpyAsync(X64, X, blk, hipMemcpyDeviceToDevice, s));
+        if (P != reinterpret_cast<T*>(P64)) QG_HIP(hipMemcpyAsync(P64, P, blk, hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL((scale_kernel<double>), dim3(gsz), dim3(256), 0, s, X64, KN, isq);
+        if (p_valid) hipLaunchKernelGGL((scale_kernel<double>), dim3(gsz), dim3(256), 0, s, P64, KN, isq);
+        else QG_HIP(hipMemsetAsync(P64, 0, blk, s));
+    } else {
+        hipLaunchKernelGGL((convert_kernel<float, double>), dim3(gsz), dim3(256), 0, s, (const float*)X, X64, KN, isq);
+        if (p_valid)
+            hipLaunchKernelGGL((convert_kernel<float, double>), dim3(gsz), dim3(256), 0, s, (const float*)P, P64, KN,
+                               isq);
+        else QG_HIP(hipMemsetAsync(P64, 0, blk, s));
+    }
+    sync();
+}
+
 }  // namespace qg
 
 using namespace qg;
@@ -682,6 +1015,7 @@ void qg_lobpcg_default_opts(qg_lobpcg_opts* o) {
     o->stall_window = 5;
     o->op = nullptr;
     o->op_user = nullptr;
+    o->basis = QG_BASIS_REFERENCE;
 }
 
 int qg_lobpcg_d_multi(qg_ham* h, const qg_lobpcg_opts* opts, int nbands, double* eig, double* X, double* P,
@@ -702,21 +1036,24 @@ int qg_lobpcg_d_multi(qg_ham* h, const qg_lobpcg_opts* opts, int nbands, double*
         QG_REQUIRE(h->grid.ng() < (1LL << 31), "grid too large for 32-bit BLAS dimensions");
         qg_lobpcg_stats st{};
         st.fallback_iter = -1;
-        if (o.precision == QG_PREC_F64) {
+        const bool reference = o.precision == QG_PREC_F64 && o.precond == QG_PRECOND_NONE && o.basis == QG_BASIS_REFERENCE;
+        if (reference) {
             run_lobpcg<double>(h, o, nbands, eig, X, P, scf_step, st);
+        } else if (o.precision == QG_PREC_F64) {
+            run_lobpcg_ortho<double>(h, o, nbands, eig, X, P, scf_step, st);
         } else {
             // FP32 phase down to switch_tol, then FP64 refinement sweeps down to tol
             const bool refine = o.tol > 0.0 && o.tol < o.switch_tol;
             qg_lobpcg_opts o1 = o;
             if (refine) o1.tol = o.switch_tol;
-            run_lobpcg<float>(h, o1, nbands, eig, X, P, scf_step, st);
+            run_lobpcg_ortho<float>(h, o1, nbands, eig, X, P, scf_step, st);
             if (refine && st.max_residual > o.tol && st.iterations < o.max_iter) {
                 qg_lobpcg_opts o2 = o;
                 o2.precision = QG_PREC_F64;
                 o2.max_iter = o.max_iter - st.iterations;
                 qg_lobpcg_stats s2{};
                 s2.fallback_iter = st.fallback_iter;
-                run_lobpcg<double>(h, o2, nbands, eig, X, P, 0, s2, st.iterations);
+                run_lobpcg_ortho<double>(h, o2, nbands, eig, X, P, 0, s2, st.iterations);
                 st.fallback_iter = s2.fallback_iter;
                 st.iterations += s2.iterations;
                 st.rollbacks += s2.rollbacks;
