@@ -590,9 +590,10 @@ static Plan make_plan(const qg_grid& g, int dtype, int nbands, int mode) {
     }
     p.nb = env_int("QG_PLAN_NB", 1);
     p.nb = (p.nb == 2 && nbands >= 2) ? 2 : 1;
-    p.ty = std::max(1, std::min(ny, env_int("QG_PLAN_TY", 1280 / nx)));
-    p.ppt = env_int("QG_PLAN_PPT", dtype == QG_F64 ? 2 : 4);
-    if (dtype == QG_F64 && !std::getenv("QG_PLAN_TY")) p.ty = std::max(1, std::min(ny, 960 / nx));
+    // measured on MI355X at 80^3: narrow tiles (480 points FP32, 960 FP64) with 2 points per
+    // thread keep more workgroups per CU in flight through the projector phases
+    p.ty = std::max(1, std::min(ny, env_int("QG_PLAN_TY", (dtype == QG_F64 ? 960 : 480) / nx)));
+    p.ppt = env_int("QG_PLAN_PPT", 2);
     TunedShape ts;
     if (lookup_tuned(g, dtype, ts)) {
         p.ty = std::max(1, std::min(ny, ts.ty));
@@ -613,7 +614,7 @@ static Plan make_plan(const qg_grid& g, int dtype, int nbands, int mode) {
     int tz = nz;
     while (tz > 32 && (int64_t)ntile * nchunk * ((nz + tz - 1) / tz) < 4096) tz = (tz + 1) / 2;
     p.tz = std::max(1, std::min(nz, env_int("QG_PLAN_TZ", tz)));
-    p.nlp = std::max(1, std::min(4, env_int("QG_PLAN_NLP", 1)));
+    p.nlp = std::max(1, std::min(4, env_int("QG_PLAN_NLP", 4)));
     return p;
 }
 
@@ -700,7 +701,12 @@ static FusedSetup<T> setup_fused(const qg_grid& g, int nbands, int64_t ld_in, in
         a.atom_coef_off = nl->d_atom_coef_off.p;
         a.ncoef = nl->geo.ncoef;
         a.max_atom = std::max(1, t.max_atom);
-        lds += (size_t)p.nlp * p.nb * p.ty * g.n[0] * sizeof(T) + (size_t)a.max_atom * njs * p.nb * sizeof(T);
+        const size_t ring = (size_t)p.nb * p.ty * g.n[0] * sizeof(T);
+        const size_t ec = (size_t)a.max_atom * njs * p.nb * sizeof(T);
+        // fewer planes per projector phase when the accumulator ring would not fit
+        while (p.nlp > 1 && lds + p.nlp * ring + ec > 65536) p.nlp /= 2;
+        a.nlp = p.nlp;
+        lds += p.nlp * ring + ec;
     }
     QG_REQUIRE(lds <= 65536, "tile staging does not fit in 64 KiB of LDS");
     const StencilCoef<T> sc = make_coef<T>(g, coef);
