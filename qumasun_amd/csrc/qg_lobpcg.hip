@@ -675,15 +675,20 @@ static void run_lobpcg(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_ho
 // problem by a large diagonal).  The Ritz values and the residual definition are the
 // reference's (lobpcg_d_multi_mpi.h:1280-1320).
 
-// d_i = G_ii^-1/2 (0 for an empty column); G <- D G D
-__global__ void svqb_scale_kernel(double* G, int N, int ld, double* d) {
+// d_i = G_ii^-1/2 (0 for an empty column)
+__global__ void svqb_diag_kernel(const double* G, int N, int ld, double* d) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double gi = G[i + (int64_t)ld * i];
+    d[i] = gi > 0.0 ? 1.0 / sqrt(gi) : 0.0;
+}
+
+// G <- D G D (d computed beforehand: the diagonal is rewritten here)
+__global__ void svqb_scale_kernel(double* G, int N, int ld, const double* d) {
     const int j = blockIdx.y;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
-    const double gi = G[i + (int64_t)ld * i], gj = G[j + (int64_t)ld * j];
-    const double di = gi > 0.0 ? 1.0 / sqrt(gi) : 0.0, dj = gj > 0.0 ? 1.0 / sqrt(gj) : 0.0;
-    G[i + (int64_t)ld * j] *= di * dj;
-    if (j == 0) d[i] = di;
+    G[i + (int64_t)ld * j] *= d[i] * d[j];
 }
 
 // T_ij = d_i U_ij theta_j^-1/2 for kept directions, 0 otherwise; keep_j = 1 / 0
@@ -797,7 +802,9 @@ static void run_lobpcg_ortho(qg_ham* h, const qg_lobpcg_opts& o, int N, double* 
         stats.t_eig += secs(t0, now());
         return info;
     };
-    const double tau = sizeof(T) == 8 ? 1e-12 : 1e-6;
+    // drop threshold of SVQB: well above the relative rounding of the Gram matrix (FP32 Grams
+    // carry ~1e-6), so the kept directions are amplified at most ~1/sqrt(tau)
+    const double tau = sizeof(T) == 8 ? 1e-12 : 1e-4;
     // V <- SVQB(V); keep flags into keep_out.  Returns the smallest kept theta over the
     // largest (1: already orthonormal; small: the transform amplified rounding, so callers
     // orthogonalise once more).
@@ -805,7 +812,9 @@ static void run_lobpcg_ortho(qg_ham* h, const qg_lobpcg_opts& o, int N, double* 
     auto svqb = [&](T*& V, int* keep_out, bool drop) -> double {
         auto t0 = now();
         op.gram(V, V, Gm.p, N, true);
-        hipLaunchKernelGGL(svqb_scale_kernel, dim3((N + 255) / 256, N), dim3(256), 0, s, Gm.p, N, N, dsc.p);
+        hipLaunchKernelGGL(svqb_diag_kernel, dim3((N + 255) / 256), dim3(256), 0, s, Gm.p, N, N, dsc.p);
+        hipLaunchKernelGGL(svqb_scale_kernel, dim3((N + 255) / 256, N), dim3(256), 0, s, Gm.p, N, N,
+                           (const double*)dsc.p);
         sync();
         stats.t_gram += secs(t0, now());
         const int info = eigh(Gm.p, N, N, st.D.p);
@@ -824,7 +833,8 @@ static void run_lobpcg_ortho(qg_ham* h, const qg_lobpcg_opts& o, int N, double* 
             if (theta[j] > (drop ? tau : 0.0) * tmax && theta[j] > 0.0) tmin = std::min(tmin, theta[j]);
         return tmax > 0.0 ? tmin / tmax : 1.0;
     };
-    const double reortho = 1e-3;   // second pass when the first SVQB amplified by more than ~30
+    // second pass when the first SVQB amplified by more than ~30 (FP64), always in FP32
+    const double reortho = sizeof(T) == 8 ? 1e-3 : 2.0;
     // Rayleigh-Ritz on the (orthonormal) columns of V with AV: V, AV rotated to Ritz vectors
     auto rr_single = [&](T*& V, T*& AV) {
         op.gram(V, AV, st.A3.p, N);
@@ -956,6 +966,8 @@ static void run_lobpcg_ortho(qg_ham* h, const qg_lobpcg_opts& o, int N, double* 
                            K);
         sync();
         stats.t_update += secs(t0, now());
+        // AX follows X by recurrence; refresh it from H every 10 iterations
+        if ((iter + 1) % 10 == 0) hpsi(X, AX);
         p_valid = true;
         residual(maxres);
         stats.max_residual = maxres;
