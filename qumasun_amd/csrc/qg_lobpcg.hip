@@ -879,8 +879,13 @@ static void run_lobpcg_ortho(qg_ham* h, const qg_lobpcg_opts& o, int N, double* 
     double best_res = maxres;
     int best_iter = 0;
     int iter = 0;
-    const double big = 1e30;
     for (iter = 0; iter < o.max_iter; ++iter) {
+        // diagonal given to dropped basis directions: above every Ritz value of the kept
+        // basis, but small enough that the eigensolver's absolute error (~eps |A|) stays below
+        // the spectrum's resolution
+        double emax = 0.0;
+        for (int k = 0; k < N; ++k) emax = std::max(emax, std::fabs(eig_host[k]));
+        const double big = 10.0 * (emax + 1.0e3);
         if (o.tol > 0.0 && maxres <= o.tol) break;
         // W = K R, orthogonal to X, orthonormal
         auto t0 = now();
