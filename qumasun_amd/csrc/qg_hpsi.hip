@@ -19,7 +19,7 @@
 //   * workgroup ids are remapped so each XCD gets a contiguous range (y tile fastest), and
 //     neighbouring tiles share that XCD's L2 for halo rows, V and projector tables.
 // Projector coefficients come from a table-driven, atom-centric kernel.  A block of bands is
-// processed in chunks of about 64 MB (projection, then fused apply), so the fused kernel
+// processed in chunks of about 128 MB (projection, then fused apply), so the fused kernel
 // re-reads each chunk from the 256 MB Infinity Cache rather than from HBM.
 #include <algorithm>
 #include <cmath>
@@ -592,7 +592,7 @@ static Plan make_plan(const qg_grid& g, int dtype, int nbands, int mode) {
     p.nb = (p.nb == 2 && nbands >= 2) ? 2 : 1;
     // measured on MI355X at 80^3: narrow tiles (480 points FP32, 960 FP64) with 2 points per
     // thread keep more workgroups per CU in flight through the projector phases
-    p.ty = std::max(1, std::min(ny, env_int("QG_PLAN_TY", (dtype == QG_F64 ? 960 : 480) / nx)));
+    p.ty = std::max(1, std::min(ny, env_int("QG_PLAN_TY", 480 / nx)));
     p.ppt = env_int("QG_PLAN_PPT", 2);
     TunedShape ts;
     if (lookup_tuned(g, dtype, ts)) {
@@ -612,7 +612,9 @@ static Plan make_plan(const qg_grid& g, int dtype, int nbands, int mode) {
     const int ntile = (ny + p.ty - 1) / p.ty;
     const int nchunk = (nbands + p.nb - 1) / p.nb;
     int tz = nz;
-    while (tz > 32 && (int64_t)ntile * nchunk * ((nz + tz - 1) / tz) < 4096) tz = (tz + 1) / 2;
+    // slabs of 10-16 planes: short enough for many workgroups in flight, long enough for the
+    // 8-plane z warm-up
+    while (tz > 16 && (int64_t)ntile * nchunk * ((nz + tz - 1) / tz) < 65536) tz = (tz + 1) / 2;
     p.tz = std::max(1, std::min(nz, env_int("QG_PLAN_TZ", tz)));
     p.nlp = std::max(1, std::min(4, env_int("QG_PLAN_NLP", 4)));
     return p;
@@ -788,10 +790,10 @@ static void launch_project(qg_nonlocal* nl, int nbands, const T* psi, int64_t ld
 }
 
 // Bands per chunk so that one chunk of psi stays resident in the Infinity Cache between
-// the projection and the fused apply (QG_CHUNK_MB, default 64 MB).
+// the projection and the fused apply (QG_CHUNK_MB, default 128 MB).
 static int chunk_bands(const qg_grid& g, int dtype, int nbands) {
     const int64_t bytes = g.ng() * (dtype == QG_F32 ? 4 : 8);
-    const int64_t budget = (int64_t)env_int("QG_CHUNK_MB", 64) << 20;
+    const int64_t budget = (int64_t)env_int("QG_CHUNK_MB", 128) << 20;
     int64_t c = std::max<int64_t>(1, budget / std::max<int64_t>(1, bytes));
     return (int)std::min<int64_t>(nbands, c);
 }
