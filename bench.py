@@ -49,7 +49,21 @@ def parse():
                     help="bands of the CPU baseline sample (FP64 oracle, 1 thread)")
     ap.add_argument("--eigen", choices=["none", "cfg0", "cfg2"], default="none",
                     help="also time LOBPCG to 1e-6 Ha: plain FP64 vs FP32 + learned preconditioner")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 5 device copies of 256 MiB (FP32), the known-size kernel that scales "
+                         "the FETCH_SIZE / WRITE_SIZE counters to bytes")
     return ap.parse_args()
+
+
+def copy_calibration():
+    import torch
+    x = torch.ones(64 * 1024 * 1024, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read_per_launch": x.numel() * 4, "copy_bytes_written_per_launch": y.numel() * 4}))
 
 
 def cpu_model():
@@ -161,6 +175,9 @@ def main():
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(this build runs the whole grid on a single MI355X)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
     import torch
     import qumasun_amd as Q
 

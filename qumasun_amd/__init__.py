@@ -154,6 +154,12 @@ class PseudoPotIntegrator:
     def num_coef(self):
         return lib().qg_nonlocal_num_coef(self._h)
 
+    @property
+    def projector_form(self):
+        """'kernel' when H psi evaluates the Gaussian projectors in the kernel, 'tables'
+        when it reads per-point tables."""
+        return "kernel" if lib().qg_nonlocal_projector_form(self._h) == 1 else "tables"
+
     def InnerNonlocalPsi(self, psi):
         """coef[b, j] = dV sum beta_j psi_b (mInnerNonlocalPsi_v2), FP64 device tensor."""
         import torch

@@ -62,6 +62,18 @@ struct FusedArgs {
     int ncoef;
     int max_atom;
     int nlp;                   // z-planes per projector phase (accumulator ring depth, 1..4)
+    // in-kernel projector values (every projector Gaussian): sphere segments per tile row and
+    // the images / projectors they refer to
+    int analytic;
+    int max_img;
+    const int* seg_off;
+    const uint32_t* seg;
+    const int* wg_img_off;
+    const int* wg_img;
+    const int* wg_img_atom;
+    const DevImage* images;
+    const DevProj* projs;
+    double hx, hy, hz;
 };
 
 // Buffer-resource access: SGPR base + 32-bit byte offsets (VGPR per point, SGPR per plane).
@@ -119,9 +131,19 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const StencilCoef
     T* nl_l = reinterpret_cast<T*>(lp);       // per-point projector term, nlp x NB x tile_pts
     T* ec_l = nullptr;                        // E*coef per (local atom, slot, band)
     const int nlp = a.nlp;
+    const bool analytic = MODE != MODE_LAP && a.has_nl && a.analytic;
+    double* im_r = nullptr;                   // image centres (analytic projector values)
+    int* im_i = nullptr;                      // image shift[3], atom slot, first / end projector
     if (a.has_nl) {
-        lp += (size_t)nlp * NB * tile_pts * S;
+        if (!analytic) lp += (size_t)nlp * NB * tile_pts * S;
         ec_l = reinterpret_cast<T*>(lp);
+        lp += (size_t)a.max_atom * NJS * NB * S;
+        if (analytic) {
+            lp = smem_raw + (((size_t)(lp - smem_raw) + 7) & ~size_t(7));
+            im_r = reinterpret_cast<double*>(lp);
+            lp += (size_t)a.max_img * 3 * sizeof(double);
+            im_i = reinterpret_cast<int*>(lp);
+        }
     }
 
     // logical order: band chunk fastest, so the workgroups of one (tile, slab) run back to
@@ -197,7 +219,24 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const StencilCoef
                 v = (T)(a.coef_energy[cb + j] * a.coef[(int64_t)(b0 + b) * a.ncoef + cb + j]);
             ec_l[i] = v;
         }
-        for (int i = tid; i < nlp * NB * tile_pts; i += nthr) nl_l[i] = T(0);
+        if (analytic) {
+            const int i0 = a.wg_img_off[wgi];
+            const int nimg = a.wg_img_off[wgi + 1] - i0;
+            for (int li = tid; li < nimg; li += nthr) {
+                const DevImage& im = a.images[a.wg_img[i0 + li]];
+                im_r[3 * li] = im.R[0];
+                im_r[3 * li + 1] = im.R[1];
+                im_r[3 * li + 2] = im.R[2];
+                im_i[6 * li] = im.shift[0];
+                im_i[6 * li + 1] = im.shift[1];
+                im_i[6 * li + 2] = im.shift[2];
+                im_i[6 * li + 3] = a.wg_img_atom[i0 + li];
+                im_i[6 * li + 4] = im.proj_begin;
+                im_i[6 * li + 5] = im.proj_end;
+            }
+        } else {
+            for (int i = tid; i < nlp * NB * tile_pts; i += nthr) nl_l[i] = T(0);
+        }
     }
 
     T q[NB][PPT][9];
@@ -262,7 +301,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const StencilCoef
 
         // projector term: every (image, sphere point) reaching the next nlp planes is one work
         // item, accumulated into the ring slot of its plane
-        if (MODE != MODE_LAP && a.has_nl && (z - z0) % nlp == 0) {
+        if (MODE != MODE_LAP && a.has_nl && !analytic && (z - z0) % nlp == 0) {
             const size_t tp = (size_t)ytile * nz + z;
             const int zend = min(z + nlp, z1);
             const int64_t io0 = a.item_off[tp];
@@ -315,12 +354,64 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const StencilCoef
                 } else {
                     res[b][k] = (a.accumulate && lc[k] >= 0) ? bload<T>(rsrc_out[b], vo[k], sz) : T(0);
                 }
-                if (MODE != MODE_LAP && a.has_nl && lc[k] >= 0) {
+                if (MODE != MODE_LAP && a.has_nl && !analytic && lc[k] >= 0) {
                     T* slot = &nl_l[((size_t)((z - z0) % nlp) * NB + b) * tile_pts + tid + nthr * k];
                     res[b][k] += *slot;
                     *slot = T(0);
                 }
             }
+        if (analytic) {
+            // beta_j = p(r) Y_lm of every sphere this point lies in, from the image centre:
+            // the same arithmetic as the host tables (displacement and r^2 in FP64), and
+            // acc_b = sum_j E_j c_jb beta_j
+#pragma unroll
+            for (int k = 0; k < PPT; ++k) {
+                if (lc[k] < 0) continue;
+                const int p = tid + nthr * k;
+                const int ly = p / nx, ix = p - ly * nx;
+                const int iy = y0 + ly;
+                const size_t ks = ((size_t)ytile * nz + z) * a.ty + ly;
+                const int s0 = a.seg_off[ks], s1 = a.seg_off[ks + 1];
+                T acc[NB];
+#pragma unroll
+                for (int b = 0; b < NB; ++b) acc[b] = T(0);
+                for (int si = s0; si < s1; ++si) {
+                    const uint32_t sg = a.seg[si];
+                    const int x0 = (int)(sg & 0x7ffu), x1 = (int)((sg >> 11) & 0x7ffu);
+                    if (ix < x0 || ix >= x1) continue;
+                    const int li = (int)(sg >> 22);
+                    const int* ii = im_i + 6 * li;
+                    const double zc = a.hz * (double)(z - ii[2]) - im_r[3 * li + 2];
+                    const double yc = a.hy * (double)(iy - ii[1]) - im_r[3 * li + 1];
+                    const double xc = a.hx * (double)(ix - ii[0]) - im_r[3 * li];
+                    const double rr = xc * xc + (yc * yc + zc * zc);
+                    const T* ec = ec_l + ii[3] * NJS * NB;
+                    T yv[3] = {T(0), T(0), T(0)};
+                    if (!(1.0e-14 > rr)) {
+                        const double ir = 1.0 / sqrt(rr);
+                        yv[0] = (T)(0.48860251190291992 * (yc * ir));   // sqrt(3/4pi) y/r
+                        yv[1] = (T)(0.48860251190291992 * (zc * ir));   // sqrt(3/4pi) z/r
+                        yv[2] = (T)(0.48860251190291992 * (xc * ir));   // sqrt(3/4pi) x/r
+                    }
+                    for (int pj = ii[4]; pj < ii[5]; ++pj) {
+                        const DevProj& d = a.projs[pj];
+                        const T rad = (T)d.amp * (T)exp(-(T)(rr * d.inv_2s2));
+                        if (d.l == 0) {
+                            const T bv = rad * (T)0.28209479177387814;   // 1/sqrt(4 pi)
+#pragma unroll
+                            for (int b = 0; b < NB; ++b) acc[b] += ec[d.coef_rel * NB + b] * bv;
+                        } else {
+#pragma unroll
+                            for (int b = 0; b < NB; ++b)
+                                acc[b] += rad * (ec[d.coef_rel * NB + b] * yv[0] + ec[(d.coef_rel + 1) * NB + b] * yv[1] +
+                                                 ec[(d.coef_rel + 2) * NB + b] * yv[2]);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int b = 0; b < NB; ++b) res[b][k] += acc[b];
+            }
+        }
         if constexpr (MODE != MODE_NL) {
             const T* L = planes + buf * NB * lds_band;
 #pragma unroll
@@ -624,13 +715,23 @@ static Plan make_plan(const qg_grid& g, int dtype, int nbands, int mode) {
 static void prepare_tile_nl(qg_nonlocal* nl, const Plan& p) {
     if (nl->plan_ty == p.ty && nl->plan_tz == p.tz) return;
     TileNL& t = nl->tile;
-    build_tile_nl(nl->gcopy, nl->geo, p.ty, p.tz, t);
+    build_tile_nl(nl->gcopy, nl->geo, p.ty, p.tz, nl->analytic, t);
+    if (t.max_img >= 1024) {   // more images per workgroup than the segment encoding holds
+        nl->analytic = false;
+        build_tile_nl(nl->gcopy, nl->geo, p.ty, p.tz, false, t);
+    }
     auto up = [](auto& buf, auto& vec) {
         if (vec.empty()) vec.push_back({});
         buf.upload(vec.data(), vec.size());
     };
     up(nl->d_wg_atom_off, t.wg_atom_off);
     up(nl->d_wg_atom, t.wg_atom);
+    up(nl->d_wg_img_off, t.wg_img_off);
+    up(nl->d_wg_img, t.wg_img);
+    up(nl->d_wg_img_atom, t.wg_img_atom);
+    up(nl->d_seg_off, t.seg_off);
+    up(nl->d_seg, t.seg);
+    std::vector<uint32_t>().swap(t.seg);
     up(nl->d_item_off, t.item_off);
     up(nl->d_item_meta, t.item_meta);
     up(nl->d_item_beta64, t.item_beta);
@@ -703,12 +804,26 @@ static FusedSetup<T> setup_fused(const qg_grid& g, int nbands, int64_t ld_in, in
         a.atom_coef_off = nl->d_atom_coef_off.p;
         a.ncoef = nl->geo.ncoef;
         a.max_atom = std::max(1, t.max_atom);
-        const size_t ring = (size_t)p.nb * p.ty * g.n[0] * sizeof(T);
         const size_t ec = (size_t)a.max_atom * njs * p.nb * sizeof(T);
-        // fewer planes per projector phase when the accumulator ring would not fit
-        while (p.nlp > 1 && lds + p.nlp * ring + ec > 65536) p.nlp /= 2;
-        a.nlp = p.nlp;
-        lds += p.nlp * ring + ec;
+        if (nl->analytic) {
+            a.analytic = 1;
+            a.max_img = std::max(1, t.max_img);
+            a.seg_off = nl->d_seg_off.p;
+            a.seg = nl->d_seg.p;
+            a.wg_img_off = nl->d_wg_img_off.p;
+            a.wg_img = nl->d_wg_img.p;
+            a.wg_img_atom = nl->d_wg_img_atom.p;
+            a.images = nl->d_images.p;
+            a.projs = nl->d_projs.p;
+            a.hx = g.h[0]; a.hy = g.h[1]; a.hz = g.h[2];
+            lds = ((lds + ec + 7) & ~size_t(7)) + (size_t)a.max_img * (3 * sizeof(double) + 6 * sizeof(int));
+        } else {
+            const size_t ring = (size_t)p.nb * p.ty * g.n[0] * sizeof(T);
+            // fewer planes per projector phase when the accumulator ring would not fit
+            while (p.nlp > 1 && lds + p.nlp * ring + ec > 65536) p.nlp /= 2;
+            a.nlp = p.nlp;
+            lds += p.nlp * ring + ec;
+        }
     }
     QG_REQUIRE(lds <= 65536, "tile staging does not fit in 64 KiB of LDS");
     const StencilCoef<T> sc = make_coef<T>(g, coef);
@@ -1136,6 +1251,11 @@ int qg_nonlocal_create(const qg_grid* g, int natoms, const double* xyz, const in
             if (!geo.coef_energy.empty()) nl->d_coef_energy.upload(geo.coef_energy.data(), geo.coef_energy.size());
             nl->d_atom_img_off.upload(geo.atom_img_off.data(), geo.atom_img_off.size());
             nl->d_atom_coef_off.upload(geo.atom_coef_off.data(), geo.atom_coef_off.size());
+            // Gaussian projectors are evaluated inside the H psi kernel instead of read from
+            // per-point tables (QG_NL_ANALYTIC=0 keeps the tables)
+            bool gauss = !geo.projs.empty();
+            for (const DevProj& d : geo.projs) gauss = gauss && d.kind == QG_RADIAL_GAUSSIAN && d.l <= 1;
+            nl->analytic = gauss && g->n[0] <= 1024 && env_int("QG_NL_ANALYTIC", 1) != 0;
         } catch (...) {
             delete nl;
             throw;
@@ -1150,6 +1270,8 @@ int qg_nonlocal_destroy(qg_nonlocal* nl) {
 }
 
 int qg_nonlocal_num_coef(const qg_nonlocal* nl) { return nl ? nl->geo.ncoef : -1; }
+
+int qg_nonlocal_projector_form(const qg_nonlocal* nl) { return nl ? (nl->analytic ? 1 : 0) : -1; }
 
 int qg_nonlocal_project(qg_nonlocal* nl, int dtype, int nbands, const void* psi, int64_t ld,
                         double* coef, void* stream) {

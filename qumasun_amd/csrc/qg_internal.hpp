@@ -137,8 +137,8 @@ void build_nl_geometry(const qg_grid& g, int natoms, const double* xyz, const in
                        int nspecies, const int* nproj, const qg_proj_spec* specs, NlGeometry& out);
 
 // Projector data the fused kernel stages per workgroup: for every (y tile, z slab) the
-// images and atoms that reach it, and for every (y tile, z plane) the sphere segments of
-// each row, packed as x0 | x1 << 10 | local image slot << 21 and sorted by tile row.
+// images and atoms that reach it, and for every (y tile, z plane) either the table items
+// (point, atom slot, projector values) or the sphere segments of each tile row.
 struct TileNL {
     int ty = 0, tz = 0, ntile = 0, nslab = 0;
     std::vector<int> wg_img_off, wg_img;     // per (tile, slab): global image ids
@@ -149,9 +149,13 @@ struct TileNL {
     std::vector<int64_t> item_off;           // per (tile, plane): [item_off[tp], item_off[tp+1])
     std::vector<uint32_t> item_meta;         // tile point | atom slot << 16
     std::vector<double> item_beta;           // njs table values per item
+    // sphere segments of every tile row: per (tile, plane, row) [seg_off[k], seg_off[k+1]),
+    // each x0 | x1 << 11 | local image << 22 (the in-cutoff x range of one image on the row)
+    std::vector<int> seg_off;
+    std::vector<uint32_t> seg;
     int max_img = 0, max_atom = 0, max_items = 0;
 };
-void build_tile_nl(const qg_grid& g, const NlGeometry& geo, int ty, int tz, TileNL& out);
+void build_tile_nl(const qg_grid& g, const NlGeometry& geo, int ty, int tz, bool segments, TileNL& out);
 
 }  // namespace qg
 
@@ -172,6 +176,9 @@ struct qg_nonlocal {
     qg::DevBuf<int> d_wg_img_off, d_wg_img, d_wg_img_atom, d_wg_atom_off, d_wg_atom;
     qg::DevBuf<int64_t> d_item_off;
     qg::DevBuf<uint32_t> d_item_meta;
+    qg::DevBuf<int> d_seg_off;
+    qg::DevBuf<uint32_t> d_seg;
+    bool analytic = false;   // every projector Gaussian: values evaluated in the kernel
     qg::DevBuf<double> d_item_beta64;
     qg::DevBuf<float> d_item_beta32;
     qg::DevBuf<double> d_coef_energy;   // KB energy of every coefficient slot

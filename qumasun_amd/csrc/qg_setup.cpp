@@ -290,7 +290,7 @@ void build_nl_geometry(const qg_grid& g, int natoms, const double* xyz, const in
     build_tables(g, specs, out);
 }
 
-void build_tile_nl(const qg_grid& g, const NlGeometry& geo, int ty, int tz, TileNL& out) {
+void build_tile_nl(const qg_grid& g, const NlGeometry& geo, int ty, int tz, bool segments, TileNL& out) {
     out = TileNL{};
     const int ny = g.n[1], nz = g.n[2];
     out.ty = ty;
@@ -326,10 +326,12 @@ void build_tile_nl(const qg_grid& g, const NlGeometry& geo, int ty, int tz, Tile
         out.max_img = std::max(out.max_img, (int)wimg[w].size());
         out.max_atom = std::max(out.max_atom, (int)at.size());
     }
+    QG_REQUIRE(out.max_atom < 65536, "too many atoms per workgroup");
     // work items per (tile, plane): every sphere point of every image reaching the plane
+    // (the table form of the projector term; none when the kernel evaluates the projectors)
     const int njs = geo.njs;
     out.item_off.assign((size_t)out.ntile * nz + 1, 0);
-    for (int t = 0; t < out.ntile; ++t) {
+    for (int t = 0; t < out.ntile && !segments; ++t) {
         const int y0 = t * ty, rows = std::min(ty, ny - y0);
         QG_REQUIRE((int64_t)ty * g.n[0] <= 65535, "tile too large for 16-bit point indices");
         for (int z = 0; z < nz; ++z) {
@@ -357,7 +359,33 @@ void build_tile_nl(const qg_grid& g, const NlGeometry& geo, int ty, int tz, Tile
             out.max_items = std::max<int>(out.max_items, (int)items);
         }
     }
-    QG_REQUIRE(out.max_atom < 65536, "too many atoms per workgroup");
+    // sphere segments per (tile, plane, row), for the in-kernel projector evaluation
+    // (11-bit x bounds, 10-bit image slot: the caller falls back to the tables beyond that)
+    if (!segments || g.n[0] > 1024 || out.max_img >= 1024) return;
+    out.seg_off.assign((size_t)out.ntile * nz * ty + 1, 0);
+    for (int t = 0; t < out.ntile; ++t) {
+        const int y0 = t * ty, rows = std::min(ty, ny - y0);
+        for (int z = 0; z < nz; ++z) {
+            const int w = t * out.nslab + z / tz;
+            for (int r = 0; r < ty; ++r) {
+                const size_t k = ((size_t)t * nz + z) * ty + r;
+                int cnt = 0;
+                if (r < rows) {
+                    const int y = y0 + r;
+                    for (int li = 0; li < (int)wimg[w].size(); ++li) {
+                        const DevImage& im = geo.images[wimg[w][li]];
+                        if (z < im.b[2] || z >= im.e[2] || y < im.b[1] || y >= im.e[1]) continue;
+                        const int row = im.row_base + (y - im.b[1]) + (im.e[1] - im.b[1]) * (z - im.b[2]);
+                        const int2 iv = geo.intervals[row];
+                        if (iv.y <= iv.x) continue;
+                        out.seg.push_back((uint32_t)iv.x | ((uint32_t)iv.y << 11) | ((uint32_t)li << 22));
+                        ++cnt;
+                    }
+                }
+                out.seg_off[k + 1] = out.seg_off[k] + cnt;
+            }
+        }
+    }
 }
 
 }  // namespace qg

@@ -218,4 +218,6 @@ def test_lobpcg_foreign_operator(gpu):
         e, st = Q.Eigen_d_multi(H, X, P, 0, **kw)
         res.append(e)
     assert len(calls) == 16   # X once, then R every iteration
-    assert np.abs(res[0] - res[1]).max() == 0.0
+    # same iteration; H psi sums the projector term with LDS atomics in its table form, so the
+    # two runs agree to rounding, not bit for bit
+    assert np.abs(res[0] - res[1]).max() <= 1e-11
