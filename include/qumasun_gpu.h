@@ -94,9 +94,9 @@ int qg_nonlocal_destroy(qg_nonlocal* nl);
 /* Number of (atom, projector, m) coefficients per band (sum of TotalProjectorLM). */
 int qg_nonlocal_num_coef(const qg_nonlocal* nl);
 /* How the fused H psi kernel forms the projector values: 1 = evaluated in the kernel from
- * the image centres (every projector Gaussian, l <= 1; QG_NL_ANALYTIC=0 in the environment
- * at creation turns it off), 0 = read from per-point tables. Diagnostic; no counterpart in
- * the reference, which always tabulates (SetBlockYlm.h). */
+ * the image centres (every projector Gaussian, l <= 1, and QG_NL_ANALYTIC=1 in the
+ * environment at creation), 0 = read from per-point tables (the default). Diagnostic; no
+ * counterpart in the reference, which always tabulates (SetBlockYlm.h). */
 int qg_nonlocal_projector_form(const qg_nonlocal* nl);
 
 /* coef[b*ncoef + j] = dV * sum_i beta_j(i) psi_b(i)  (FP64 device array, ncoef per band).

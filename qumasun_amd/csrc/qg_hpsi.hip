@@ -66,6 +66,8 @@ struct FusedArgs {
     // the images / projectors they refer to
     int analytic;
     int max_img;
+    int max_seg;               // most sphere segments of one (tile, slab)
+    int nproj;
     const int* seg_off;
     const uint32_t* seg;
     const int* wg_img_off;
@@ -132,17 +134,32 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const StencilCoef
     T* ec_l = nullptr;                        // E*coef per (local atom, slot, band)
     const int nlp = a.nlp;
     const bool analytic = MODE != MODE_LAP && a.has_nl && a.analytic;
-    double* im_r = nullptr;                   // image centres (analytic projector values)
-    int* im_i = nullptr;                      // image shift[3], atom slot, first / end projector
+    // in-kernel projector values: per image the displacement at its box corner (3 x T) and
+    // box corner, atom slot, projector range (6 ints); per projector amplitude and 1/(2 sigma^2)
+    // (2 x T), l and coefficient slot (2 ints); the slab's segment offsets and segments
+    T* im_o = nullptr;
+    T* pr_f = nullptr;
+    int* im_i = nullptr;
+    int* pr_i = nullptr;
+    int* sg_off = nullptr;
+    uint32_t* sg = nullptr;
     if (a.has_nl) {
         if (!analytic) lp += (size_t)nlp * NB * tile_pts * S;
         ec_l = reinterpret_cast<T*>(lp);
         lp += (size_t)a.max_atom * NJS * NB * S;
         if (analytic) {
             lp = smem_raw + (((size_t)(lp - smem_raw) + 7) & ~size_t(7));
-            im_r = reinterpret_cast<double*>(lp);
-            lp += (size_t)a.max_img * 3 * sizeof(double);
+            im_o = reinterpret_cast<T*>(lp);
+            lp += (size_t)a.max_img * 3 * S;
+            pr_f = reinterpret_cast<T*>(lp);
+            lp += (size_t)a.nproj * 2 * S;
             im_i = reinterpret_cast<int*>(lp);
+            lp += (size_t)a.max_img * 6 * sizeof(int);
+            pr_i = reinterpret_cast<int*>(lp);
+            lp += (size_t)a.nproj * 2 * sizeof(int);
+            sg_off = reinterpret_cast<int*>(lp);
+            lp += (size_t)(a.tz * a.ty + 1) * sizeof(int);
+            sg = reinterpret_cast<uint32_t*>(lp);
         }
     }
 
@@ -222,18 +239,32 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const StencilCoef
         if (analytic) {
             const int i0 = a.wg_img_off[wgi];
             const int nimg = a.wg_img_off[wgi + 1] - i0;
+            const double hh[3] = {a.hx, a.hy, a.hz};
             for (int li = tid; li < nimg; li += nthr) {
                 const DevImage& im = a.images[a.wg_img[i0 + li]];
-                im_r[3 * li] = im.R[0];
-                im_r[3 * li + 1] = im.R[1];
-                im_r[3 * li + 2] = im.R[2];
-                im_i[6 * li] = im.shift[0];
-                im_i[6 * li + 1] = im.shift[1];
-                im_i[6 * li + 2] = im.shift[2];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    im_o[3 * li + d] = (T)(hh[d] * (double)(im.b[d] - im.shift[d]) - im.R[d]);
+                    im_i[6 * li + d] = im.b[d];
+                }
                 im_i[6 * li + 3] = a.wg_img_atom[i0 + li];
                 im_i[6 * li + 4] = im.proj_begin;
                 im_i[6 * li + 5] = im.proj_end;
             }
+            for (int pj = tid; pj < a.nproj; pj += nthr) {
+                const DevProj& d = a.projs[pj];
+                pr_f[2 * pj] = (T)d.amp;
+                pr_f[2 * pj + 1] = (T)d.inv_2s2;
+                pr_i[2 * pj] = d.l;
+                pr_i[2 * pj + 1] = d.coef_rel;
+            }
+            // segments of the slab's rows: [sg_off[(z - z0) * ty + row], ... + 1)
+            const size_t k0 = ((size_t)ytile * nz + z0) * a.ty;
+            const int nrow = (z1 - z0) * a.ty;
+            const int sbase = a.seg_off[k0];
+            for (int i = tid; i <= nrow; i += nthr) sg_off[i] = a.seg_off[k0 + i] - sbase;
+            const int nseg = a.seg_off[k0 + nrow] - sbase;
+            for (int i = tid; i < nseg; i += nthr) sg[i] = a.seg[sbase + i];
         } else {
             for (int i = tid; i < nlp * NB * tile_pts; i += nthr) nl_l[i] = T(0);
         }
@@ -361,50 +392,52 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const StencilCoef
                 }
             }
         if (analytic) {
-            // beta_j = p(r) Y_lm of every sphere this point lies in, from the image centre:
-            // the same arithmetic as the host tables (displacement and r^2 in FP64), and
-            // acc_b = sum_j E_j c_jb beta_j
+            // beta_j = p(r) Y_lm of every sphere this point lies in, evaluated here in the
+            // block's precision (displacement from the image's box corner, so every term is a
+            // few bohr at most), and acc_b = sum_j E_j c_jb beta_j
+            const T hx = (T)a.hx, hy = (T)a.hy, hz = (T)a.hz;
 #pragma unroll
             for (int k = 0; k < PPT; ++k) {
                 if (lc[k] < 0) continue;
                 const int p = tid + nthr * k;
                 const int ly = p / nx, ix = p - ly * nx;
                 const int iy = y0 + ly;
-                const size_t ks = ((size_t)ytile * nz + z) * a.ty + ly;
-                const int s0 = a.seg_off[ks], s1 = a.seg_off[ks + 1];
+                const int rk = (z - z0) * a.ty + ly;
+                const int s0 = sg_off[rk], s1 = sg_off[rk + 1];
                 T acc[NB];
 #pragma unroll
                 for (int b = 0; b < NB; ++b) acc[b] = T(0);
                 for (int si = s0; si < s1; ++si) {
-                    const uint32_t sg = a.seg[si];
-                    const int x0 = (int)(sg & 0x7ffu), x1 = (int)((sg >> 11) & 0x7ffu);
+                    const uint32_t sv = sg[si];
+                    const int x0 = (int)(sv & 0x7ffu), x1 = (int)((sv >> 11) & 0x7ffu);
                     if (ix < x0 || ix >= x1) continue;
-                    const int li = (int)(sg >> 22);
+                    const int li = (int)(sv >> 22);
                     const int* ii = im_i + 6 * li;
-                    const double zc = a.hz * (double)(z - ii[2]) - im_r[3 * li + 2];
-                    const double yc = a.hy * (double)(iy - ii[1]) - im_r[3 * li + 1];
-                    const double xc = a.hx * (double)(ix - ii[0]) - im_r[3 * li];
-                    const double rr = xc * xc + (yc * yc + zc * zc);
+                    const T* io = im_o + 3 * li;
+                    const T xc = io[0] + hx * (T)(ix - ii[0]);
+                    const T yc = io[1] + hy * (T)(iy - ii[1]);
+                    const T zc = io[2] + hz * (T)(z - ii[2]);
+                    const T rr = xc * xc + (yc * yc + zc * zc);
                     const T* ec = ec_l + ii[3] * NJS * NB;
                     T yv[3] = {T(0), T(0), T(0)};
-                    if (!(1.0e-14 > rr)) {
-                        const double ir = 1.0 / sqrt(rr);
-                        yv[0] = (T)(0.48860251190291992 * (yc * ir));   // sqrt(3/4pi) y/r
-                        yv[1] = (T)(0.48860251190291992 * (zc * ir));   // sqrt(3/4pi) z/r
-                        yv[2] = (T)(0.48860251190291992 * (xc * ir));   // sqrt(3/4pi) x/r
+                    if (!(T(1.0e-14) > rr)) {
+                        const T ir = T(0.48860251190291992) / sqrt(rr);   // sqrt(3/4pi) / r
+                        yv[0] = yc * ir;
+                        yv[1] = zc * ir;
+                        yv[2] = xc * ir;
                     }
                     for (int pj = ii[4]; pj < ii[5]; ++pj) {
-                        const DevProj& d = a.projs[pj];
-                        const T rad = (T)d.amp * (T)exp(-(T)(rr * d.inv_2s2));
-                        if (d.l == 0) {
-                            const T bv = rad * (T)0.28209479177387814;   // 1/sqrt(4 pi)
+                        const T rad = pr_f[2 * pj] * exp(-(rr * pr_f[2 * pj + 1]));
+                        const int cr = pr_i[2 * pj + 1];
+                        if (pr_i[2 * pj] == 0) {
+                            const T bv = rad * T(0.28209479177387814);   // 1/sqrt(4 pi)
 #pragma unroll
-                            for (int b = 0; b < NB; ++b) acc[b] += ec[d.coef_rel * NB + b] * bv;
+                            for (int b = 0; b < NB; ++b) acc[b] += ec[cr * NB + b] * bv;
                         } else {
 #pragma unroll
                             for (int b = 0; b < NB; ++b)
-                                acc[b] += rad * (ec[d.coef_rel * NB + b] * yv[0] + ec[(d.coef_rel + 1) * NB + b] * yv[1] +
-                                                 ec[(d.coef_rel + 2) * NB + b] * yv[2]);
+                                acc[b] += rad * (ec[cr * NB + b] * yv[0] + ec[(cr + 1) * NB + b] * yv[1] +
+                                                 ec[(cr + 2) * NB + b] * yv[2]);
                         }
                     }
                 }
@@ -791,6 +824,19 @@ static FusedSetup<T> setup_fused(const qg_grid& g, int nbands, int64_t ld_in, in
     int njs = 4;
     if (nl && nl->geo.ncoef > 0 && MODE != MODE_LAP) {
         prepare_tile_nl(nl, p);
+        if (nl->analytic) {
+            // the in-kernel form stages the slab's images and segments; tables when they do not fit
+            const TileNL& t = nl->tile;
+            const size_t need = ((lds + (size_t)std::max(1, t.max_atom) * nl->geo.njs * p.nb * sizeof(T) + 7) & ~size_t(7)) +
+                                (size_t)std::max(1, t.max_img) * (3 * sizeof(T) + 6 * sizeof(int)) +
+                                nl->geo.projs.size() * (2 * sizeof(T) + 2 * sizeof(int)) +
+                                (size_t)(p.tz * p.ty + 1) * sizeof(int) + (size_t)std::max(1, t.max_seg) * sizeof(uint32_t);
+            if (need > 65536) {
+                nl->analytic = false;
+                nl->plan_ty = nl->plan_tz = -1;
+                prepare_tile_nl(nl, p);
+            }
+        }
         const TileNL& t = nl->tile;
         njs = nl->geo.njs;
         a.has_nl = 1;
@@ -816,7 +862,11 @@ static FusedSetup<T> setup_fused(const qg_grid& g, int nbands, int64_t ld_in, in
             a.images = nl->d_images.p;
             a.projs = nl->d_projs.p;
             a.hx = g.h[0]; a.hy = g.h[1]; a.hz = g.h[2];
-            lds = ((lds + ec + 7) & ~size_t(7)) + (size_t)a.max_img * (3 * sizeof(double) + 6 * sizeof(int));
+            a.nproj = (int)nl->geo.projs.size();
+            a.max_seg = std::max(1, t.max_seg);
+            lds = ((lds + ec + 7) & ~size_t(7)) + (size_t)a.max_img * (3 * sizeof(T) + 6 * sizeof(int)) +
+                  (size_t)a.nproj * (2 * sizeof(T) + 2 * sizeof(int)) + (size_t)(p.tz * p.ty + 1) * sizeof(int) +
+                  (size_t)a.max_seg * sizeof(uint32_t);
         } else {
             const size_t ring = (size_t)p.nb * p.ty * g.n[0] * sizeof(T);
             // fewer planes per projector phase when the accumulator ring would not fit
@@ -1251,11 +1301,11 @@ int qg_nonlocal_create(const qg_grid* g, int natoms, const double* xyz, const in
             if (!geo.coef_energy.empty()) nl->d_coef_energy.upload(geo.coef_energy.data(), geo.coef_energy.size());
             nl->d_atom_img_off.upload(geo.atom_img_off.data(), geo.atom_img_off.size());
             nl->d_atom_coef_off.upload(geo.atom_coef_off.data(), geo.atom_coef_off.size());
-            // Gaussian projectors are evaluated inside the H psi kernel instead of read from
-            // per-point tables (QG_NL_ANALYTIC=0 keeps the tables)
+            // Gaussian projectors can be evaluated inside the H psi kernel instead of read from
+            // per-point tables (QG_NL_ANALYTIC=1)
             bool gauss = !geo.projs.empty();
             for (const DevProj& d : geo.projs) gauss = gauss && d.kind == QG_RADIAL_GAUSSIAN && d.l <= 1;
-            nl->analytic = gauss && g->n[0] <= 1024 && env_int("QG_NL_ANALYTIC", 1) != 0;
+            nl->analytic = gauss && g->n[0] <= 1024 && env_int("QG_NL_ANALYTIC", 0) != 0;
         } catch (...) {
             delete nl;
             throw;

@@ -153,7 +153,7 @@ struct TileNL {
     // each x0 | x1 << 11 | local image << 22 (the in-cutoff x range of one image on the row)
     std::vector<int> seg_off;
     std::vector<uint32_t> seg;
-    int max_img = 0, max_atom = 0, max_items = 0;
+    int max_img = 0, max_atom = 0, max_items = 0, max_seg = 0;   // max_seg: per (tile, slab)
 };
 void build_tile_nl(const qg_grid& g, const NlGeometry& geo, int ty, int tz, bool segments, TileNL& out);
 

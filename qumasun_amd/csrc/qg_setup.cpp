@@ -385,6 +385,11 @@ void build_tile_nl(const qg_grid& g, const NlGeometry& geo, int ty, int tz, bool
                 out.seg_off[k + 1] = out.seg_off[k] + cnt;
             }
         }
+        for (int sl = 0; sl < out.nslab; ++sl) {
+            const int zb = sl * tz, ze = std::min(zb + tz, nz);
+            const int n = out.seg_off[((size_t)t * nz + ze) * ty] - out.seg_off[((size_t)t * nz + zb) * ty];
+            out.max_seg = std::max(out.max_seg, n);
+        }
     }
 }
 
