@@ -162,7 +162,9 @@ typedef struct qg_lobpcg_opts {
     int precond;             /* QG_PRECOND_* */
     double precond_shift;    /* shift of the kinetic preconditioner (Ha) */
     int verbose;
-    double switch_tol;       /* FP32 modes: residual at which FP64 refinement sweeps take over */
+    double switch_tol;       /* FP32 modes: residual at which FP64 refinement sweeps take over
+                                (also when the FP32 residual stalls at its rounding floor: no
+                                30% decrease within 3 * max(stall_window, 3) iterations) */
     int stall_window;        /* QG_PRECOND_LEARNED: iterations without a new minimum of the
                                 max residual before the kinetic preconditioner takes over */
     /* Optional foreign operator (the reference's OpeA, lobpcg_d_multi_mpi.h:966-969): when

@@ -725,7 +725,7 @@ __global__ void diag_copy_kernel(const double* D, double* e, int N) {
 
 template <class T>
 static void run_lobpcg_ortho(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_host, double* X64, double* P64,
-                             int scf_step, qg_lobpcg_stats& stats, int iter_offset = 0) {
+                             int scf_step, qg_lobpcg_stats& stats, int iter_offset = 0, int switch_stall = 0) {
     const qg_grid& g = h->grid;
     const int64_t K = g.ng();
     const double dvol = g.dvol();
@@ -878,6 +878,9 @@ static void run_lobpcg_ortho(qg_ham* h, const qg_lobpcg_opts& o, int N, double* 
     bool learned = o.precond == QG_PRECOND_LEARNED && h->cnn && stats.fallback_iter < 0;
     double best_res = maxres;
     int best_iter = 0;
+    // precision floor: the residual no longer falls by 30% within switch_stall iterations
+    double floor_res = maxres;
+    int floor_iter = 0;
     int iter = 0;
     for (iter = 0; iter < o.max_iter; ++iter) {
         // diagonal given to dropped basis directions: above every Ritz value of the kept
@@ -977,6 +980,16 @@ static void run_lobpcg_ortho(qg_ham* h, const qg_lobpcg_opts& o, int N, double* 
         residual(maxres);
         stats.max_residual = maxres;
         if (o.verbose >= 2) std::fprintf(stderr, "lobpcg(ortho) it %d: e0 %.10f max|r| %.3e\n", iter, eig_host[0], maxres);
+        if (switch_stall > 0) {
+            if (maxres < 0.7 * floor_res) {
+                floor_res = maxres;
+                floor_iter = iter;
+            } else if (iter - floor_iter >= switch_stall) {
+                if (o.verbose) std::fprintf(stderr, "lobpcg: residual floor %.3e at iteration %d\n", maxres, iter);
+                ++iter;
+                break;
+            }
+        }
         if (maxres < best_res) {
             best_res = maxres;
             best_iter = iter;
@@ -1063,7 +1076,9 @@ int qg_lobpcg_d_multi(qg_ham* h, const qg_lobpcg_opts* opts, int nbands, double*
             const bool refine = o.tol > 0.0 && o.tol < o.switch_tol;
             qg_lobpcg_opts o1 = o;
             if (refine) o1.tol = o.switch_tol;
-            run_lobpcg_ortho<float>(h, o1, nbands, eig, X, P, scf_step, st);
+            // the FP32 phase also ends at its rounding floor (residual stalled)
+            run_lobpcg_ortho<float>(h, o1, nbands, eig, X, P, scf_step, st, 0,
+                                    refine ? 3 * std::max(o.stall_window, 3) : 0);
             if (refine && st.max_residual > o.tol && st.iterations < o.max_iter) {
                 qg_lobpcg_opts o2 = o;
                 o2.precision = QG_PREC_F64;
