@@ -172,13 +172,13 @@ __global__ __launch_bounds__(256) void residual_kernel(const T* ax, const T* x, 
 
 // C[m + ldc n] = sum_c W[c][m + N n] (split-K partial Gram products)
 template <class T>
-__global__ void split_reduce_kernel(const T* W, int nch, int N, double* C, int ldc) {
+__global__ void split_reduce_kernel(const T* W, int nch, int M, int N, double* C, int ldc) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const int64_t nn = (int64_t)N * N;
+    const int64_t nn = (int64_t)M * N;
     if (i >= nn) return;
     double s = 0.0;
     for (int c = 0; c < nch; ++c) s += (double)W[(int64_t)c * nn + i];
-    C[(i % N) + (int64_t)ldc * (i / N)] = s;
+    C[(i % M) + (int64_t)ldc * (i / M)] = s;
 }
 
 // x_k *= 1 / sqrt(n2[k]) for the columns with n2[k] > 0 (also applied to a partner block y)
@@ -254,41 +254,44 @@ struct Ops {
     // (enough parallel work even for small N), then summed in FP64.
     // In QG_PREC_MIXED the H blocks (s_matrix = false) use split-BF16 MFMA; the overlap blocks
     // stay FP32, since S must remain positive definite when R and P become nearly collinear.
-    void gram(const T* A, const T* B, double* C, int ldc, bool s_matrix = false) {
+    void gram(const T* A, const T* B, double* C, int ldc, bool s_matrix = false) { gram_wide(A, B, N, C, ldc, s_matrix); }
+    // C (N x nb) = A^T B for nb consecutive columns of B (ld K)
+    void gram_wide(const T* A, const T* B, int nb, double* C, int ldc, bool s_matrix = false) {
         if constexpr (sizeof(T) == 4) {
             if (prec == QG_PREC_MIXED && !s_matrix) {
-                gram_bf16(A, B, K, N, N, K, K, C, ldc, st.gram_ws.p, s, true);
+                gram_bf16(A, B, K, N, nb, K, K, C, ldc, st.gram_ws.p, s, true);
                 return;
             }
         }
         const int64_t nch = std::max<int64_t>(1, std::min<int64_t>(256, K / 2048));
         const int64_t kc = K / nch;
         const int64_t rem = K - nch * kc;
-        const int64_t nn = (int64_t)N * N;
+        const int64_t nn = (int64_t)N * nb;
         T* W = reinterpret_cast<T*>(st.split_ws.p);
+        QG_REQUIRE(st.split_ws.n >= (size_t)(nch + 1) * nn * sizeof(T), "split-K workspace too small");
         if constexpr (sizeof(T) == 8) {
             const double one = 1.0, zero = 0.0;
-            QG_BLAS(rocblas_dgemm_strided_batched(st.blas, rocblas_operation_transpose, rocblas_operation_none, N, N,
+            QG_BLAS(rocblas_dgemm_strided_batched(st.blas, rocblas_operation_transpose, rocblas_operation_none, N, nb,
                                                   (int)kc, &one, A, (int)K, kc, B, (int)K, kc, &zero, W, N, nn,
                                                   (int)nch));
             if (rem)
-                QG_BLAS(rocblas_dgemm(st.blas, rocblas_operation_transpose, rocblas_operation_none, N, N, (int)rem,
+                QG_BLAS(rocblas_dgemm(st.blas, rocblas_operation_transpose, rocblas_operation_none, N, nb, (int)rem,
                                       &one, A + nch * kc, (int)K, B + nch * kc, (int)K, &zero, W + nch * nn, N));
         } else {
             const float one = 1.f, zero = 0.f;
-            QG_BLAS(rocblas_sgemm_strided_batched(st.blas, rocblas_operation_transpose, rocblas_operation_none, N, N,
+            QG_BLAS(rocblas_sgemm_strided_batched(st.blas, rocblas_operation_transpose, rocblas_operation_none, N, nb,
                                                   (int)kc, &one, A, (int)K, kc, B, (int)K, kc, &zero, W, N, nn,
                                                   (int)nch));
             if (rem)
-                QG_BLAS(rocblas_sgemm(st.blas, rocblas_operation_transpose, rocblas_operation_none, N, N, (int)rem,
+                QG_BLAS(rocblas_sgemm(st.blas, rocblas_operation_transpose, rocblas_operation_none, N, nb, (int)rem,
                                       &one, A + nch * kc, (int)K, B + nch * kc, (int)K, &zero, W + nch * nn, N));
         }
         hipLaunchKernelGGL((split_reduce_kernel<T>), dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, W,
-                           (int)(nch + (rem ? 1 : 0)), N, C, ldc);
+                           (int)(nch + (rem ? 1 : 0)), N, nb, C, ldc);
     }
-    static size_t split_workspace(int64_t K, int N) {
+    static size_t split_workspace(int64_t K, int N, int nb_max = 0) {
         const int64_t nch = std::max<int64_t>(1, std::min<int64_t>(256, K / 2048));
-        return (size_t)(nch + 1) * N * N * sizeof(T);
+        return (size_t)(nch + 1) * N * (nb_max > 0 ? nb_max : N) * sizeof(T);
     }
     // Out = A * Cm + beta * Out, A: K x N (ld K), Cm: N x N FP64 (ld ldc) / FP32 copy for T = float
     void update(const T* A, const double* Cm, const float* Cm32, int ldc, double beta, T* Out) {
@@ -740,15 +743,17 @@ static void run_lobpcg_ortho(qg_ham* h, const qg_lobpcg_opts& o, int N, double* 
     }
     hipStream_t s = st.stream;
     const size_t blk = (size_t)K * N * sizeof(T);
-    st.R.alloc(blk); st.AX.alloc(blk); st.AR.alloc(blk); st.AP.alloc(blk); st.T.alloc(blk);
+    // AX, AW, AP are consecutive blocks of one allocation: the Rayleigh-Ritz Grams are three
+    // wide products X^T [AX AW AP], W^T [AW AP], P^T AP
+    st.R.alloc(blk); st.AX.alloc(3 * blk); st.T.alloc(blk);
     if (sizeof(T) != 8) { st.Xw.alloc(blk); st.Pw.alloc(blk); }
     const int n3 = 3 * N;
     const int64_t nn = (int64_t)N * N, nn9 = (int64_t)n3 * n3;
     st.A3.alloc(nn9); st.D.alloc(n3); st.E.alloc(n3); st.info.alloc(1);
     st.eig.alloc(N); st.norm_a.alloc(N); st.norm_b.alloc(N); st.res2.alloc(N);
     st.C32.alloc(nn9);
-    if (o.precision == QG_PREC_MIXED) st.gram_ws.alloc(gram_bf16_workspace(K, N, N));
-    st.split_ws.alloc(Ops<T>::split_workspace(K, N));
+    if (o.precision == QG_PREC_MIXED) st.gram_ws.alloc(gram_bf16_workspace(K, N, n3));
+    st.split_ws.alloc(Ops<T>::split_workspace(K, N, n3));
     DevBuf<double> Gm, Tr, dsc;
     Gm.alloc(nn); Tr.alloc(nn); dsc.alloc(N);
     DevBuf<int> keep;
@@ -759,9 +764,9 @@ static void run_lobpcg_ortho(qg_ham* h, const qg_lobpcg_opts& o, int N, double* 
     T* X = (sizeof(T) == 8) ? reinterpret_cast<T*>(X64) : op.blk(st.Xw);
     T* P = (sizeof(T) == 8) ? reinterpret_cast<T*>(P64) : op.blk(st.Pw);
     T* W = op.blk(st.R);
-    T* AX = op.blk(st.AX);
-    T* AW = op.blk(st.AR);
-    T* AP = op.blk(st.AP);
+    T* const AX = op.blk(st.AX);
+    T* const AW = AX + (int64_t)K * N;
+    T* const AP = AX + 2 * (int64_t)K * N;
     T* Tm = op.blk(st.T);
     const int64_t KN = K * N;
     const int gsz = grid_for(KN);
@@ -836,14 +841,14 @@ static void run_lobpcg_ortho(qg_ham* h, const qg_lobpcg_opts& o, int N, double* 
     // second pass when the first SVQB amplified by more than ~30 (FP64), always in FP32
     const double reortho = sizeof(T) == 8 ? 1e-3 : 2.0;
     // Rayleigh-Ritz on the (orthonormal) columns of V with AV: V, AV rotated to Ritz vectors
-    auto rr_single = [&](T*& V, T*& AV) {
+    auto rr_single = [&](T*& V, T* AV) {
         op.gram(V, AV, st.A3.p, N);
         const int info = eigh(st.A3.p, N, N, st.D.p);
         QG_REQUIRE(info == 0, "Rayleigh-Ritz eigensolve failed");
         upd(V, st.A3.p, N, 1.0, 0.0, Tm);
         std::swap(V, Tm);
         upd(AV, st.A3.p, N, 1.0, 0.0, Tm);
-        std::swap(AV, Tm);
+        QG_HIP(hipMemcpyAsync(AV, Tm, blk, hipMemcpyDeviceToDevice, s));
         hipLaunchKernelGGL(diag_copy_kernel, dim3((N + 255) / 256), dim3(256), 0, s, st.D.p, st.eig.p, N);
     };
     auto residual = [&](double& maxres) {
@@ -927,14 +932,9 @@ static void run_lobpcg_ortho(qg_ham* h, const qg_lobpcg_opts& o, int N, double* 
         const int nb = p_valid ? n3 : 2 * N;
         double* A = st.A3.p;
         auto blkA = [&](int bi, int bj) { return A + (int64_t)N * bi + (int64_t)nb * N * bj; };
-        op.gram(X, AX, blkA(0, 0), nb);
-        op.gram(X, AW, blkA(0, 1), nb);
-        op.gram(W, AW, blkA(1, 1), nb);
-        if (p_valid) {
-            op.gram(X, AP, blkA(0, 2), nb);
-            op.gram(W, AP, blkA(1, 2), nb);
-            op.gram(P, AP, blkA(2, 2), nb);
-        }
+        op.gram_wide(X, AX, nb, blkA(0, 0), nb);            // X^T [AX AW (AP)]
+        op.gram_wide(W, AW, nb - N, blkA(1, 1), nb);        // W^T [AW (AP)]
+        if (p_valid) op.gram(P, AP, blkA(2, 2), nb);        // P^T AP
         hipLaunchKernelGGL(mask_diag_kernel, dim3((2 * N + 255) / 256), dim3(256), 0, s, A, nb, keep.p + N, N, N, big);
         if (p_valid)
             hipLaunchKernelGGL(mask_diag_kernel, dim3((N + 255) / 256), dim3(256), 0, s, A, nb, keep.p + 2 * N, 2 * N,
@@ -950,7 +950,8 @@ static void run_lobpcg_ortho(qg_ham* h, const qg_lobpcg_opts& o, int N, double* 
             residual(maxres);
             continue;
         }
-        // X' = X Cx + W Cw + P Cp,  P' = W Cw + P Cp  (and the same for AX, AP)
+        // X' = X Cx + W Cw + P Cp,  P' = W Cw + P Cp; AX' = H X' (one H psi costs less than the
+        // three GEMMs of the recurrence, and carries no accumulated rounding)
         t0 = now();
         const double* Cx = A;
         const double* Cw = A + N;
@@ -961,21 +962,14 @@ static void run_lobpcg_ortho(qg_ham* h, const qg_lobpcg_opts& o, int N, double* 
         upd(X, Cx, nb, 1.0, 1.0, P);                           // old P block = X'
         std::swap(X, P);                                       // X = X', P = old X block
         std::swap(P, Tm);                                      // P = P', Tm = free
-        upd(AW, Cw, nb, 1.0, 0.0, Tm);                         // Tm = AP'
-        if (p_valid) upd(AP, Cp, nb, 1.0, 1.0, Tm);
-        QG_HIP(hipMemcpyAsync(AP, Tm, blk, hipMemcpyDeviceToDevice, s));
-        upd(AX, Cx, nb, 1.0, 1.0, AP);                         // old AP block = AX'
-        std::swap(AX, AP);
-        std::swap(AP, Tm);
         hipLaunchKernelGGL(diag_copy_kernel, dim3((N + 255) / 256), dim3(256), 0, s, st.D.p, st.eig.p, N);
-        // keep X normalised against rounding (AX with it)
+        // keep X normalised against rounding
         hipLaunchKernelGGL((coldot_kernel<T>), dim3(N), dim3(256), 0, s, X, X, K, K, st.norm_a.p);
-        hipLaunchKernelGGL((colscale_kernel<T>), dim3(grid_for(K) / 4 + 1, N), dim3(256), 0, s, X, AX, st.norm_a.p, K,
-                           K);
+        hipLaunchKernelGGL((colscale_kernel<T>), dim3(grid_for(K) / 4 + 1, N), dim3(256), 0, s, X, (T*)nullptr,
+                           st.norm_a.p, K, K);
         sync();
         stats.t_update += secs(t0, now());
-        // AX follows X by recurrence; refresh it from H every 10 iterations
-        if ((iter + 1) % 10 == 0) hpsi(X, AX);
+        hpsi(X, AX);
         p_valid = true;
         residual(maxres);
         stats.max_residual = maxres;
