@@ -93,10 +93,11 @@ int qg_nonlocal_create(const qg_grid* g, int natoms, const double* xyz_host,
 int qg_nonlocal_destroy(qg_nonlocal* nl);
 /* Number of (atom, projector, m) coefficients per band (sum of TotalProjectorLM). */
 int qg_nonlocal_num_coef(const qg_nonlocal* nl);
-/* How the fused H psi kernel forms the projector values: 1 = evaluated in the kernel from
- * the image centres (every projector Gaussian, l <= 1, and QG_NL_ANALYTIC=1 in the
- * environment at creation), 0 = read from per-point tables (the default). Diagnostic; no
- * counterpart in the reference, which always tabulates (SetBlockYlm.h). */
+/* How H psi forms the projector values: 1 = the FP32 projector pass evaluates them in the
+ * kernel from the image centres (every projector Gaussian with l <= 1; QG_NL_ANALYTIC=0 in
+ * the environment at creation turns it off), 0 = per-point tables everywhere. FP64 launches
+ * and the fused single-pass form read tables. Diagnostic; no counterpart in the reference,
+ * which always tabulates (SetBlockYlm.h). */
 int qg_nonlocal_projector_form(const qg_nonlocal* nl);
 
 /* coef[b*ncoef + j] = dV * sum_i beta_j(i) psi_b(i)  (FP64 device array, ncoef per band).

@@ -693,7 +693,7 @@ static Plan make_plan(const qg_grid& g, int dtype, int nbands, int mode) {
     Plan p{};
     if (mode == MODE_NL) {
         // projector-only pass: no stencil state, so many bands share each projector item
-        const int want = env_int("QG_PLAN_NBNL", dtype == QG_F64 ? 8 : 16);
+        const int want = env_int("QG_PLAN_NBNL", 8);
         p.nb = 1;
         for (int c : {16, 8, 4, 2})
             if (c <= want && c <= nbands) { p.nb = c; break; }
@@ -705,11 +705,9 @@ static Plan make_plan(const qg_grid& g, int dtype, int nbands, int mode) {
             p.nthr = round64((p.ty * nx + p.ppt - 1) / p.ppt);
             QG_REQUIRE(p.ppt <= 8 && (p.ppt < 8 || p.nthr <= kMaxThreads), "tile too large");
         }
-        const int ntile = (ny + p.ty - 1) / p.ty;
-        const int nchunk = (nbands + p.nb - 1) / p.nb;
-        int tz = nz;
-        while (tz > 16 && (int64_t)ntile * nchunk * ((nz + tz - 1) / tz) < 4096) tz = (tz + 1) / 2;
-        p.tz = std::max(1, std::min(nz, env_int("QG_PLAN_TZNL", tz)));
+        // no z warm-up in this pass: thin slabs give many workgroups (measured best at 80^3:
+        // 1-2 planes, 8 bands)
+        p.tz = std::max(1, std::min(nz, env_int("QG_PLAN_TZNL", 2)));
         return p;
     }
     p.nb = env_int("QG_PLAN_NB", 1);
@@ -745,36 +743,42 @@ static Plan make_plan(const qg_grid& g, int dtype, int nbands, int mode) {
 }
 
 // Upload the per-plan staging lists of the projector term (cached per (ty, tz)).
-static void prepare_tile_nl(qg_nonlocal* nl, const Plan& p) {
-    if (nl->plan_ty == p.ty && nl->plan_tz == p.tz) return;
-    TileNL& t = nl->tile;
-    build_tile_nl(nl->gcopy, nl->geo, p.ty, p.tz, nl->analytic, t);
-    if (t.max_img >= 1024) {   // more images per workgroup than the segment encoding holds
-        nl->analytic = false;
-        build_tile_nl(nl->gcopy, nl->geo, p.ty, p.tz, false, t);
-    }
+static qg_nonlocal::Tiles& prepare_tile_nl(qg_nonlocal* nl, const Plan& p, bool segments) {
+    qg_nonlocal::Tiles& c = nl->tiles[segments ? 1 : 0];
+    if (c.ty == p.ty && c.tz == p.tz) return c;
+    TileNL& t = c.tile;
+    build_tile_nl(nl->gcopy, nl->geo, p.ty, p.tz, segments, t);
     auto up = [](auto& buf, auto& vec) {
         if (vec.empty()) vec.push_back({});
         buf.upload(vec.data(), vec.size());
     };
-    up(nl->d_wg_atom_off, t.wg_atom_off);
-    up(nl->d_wg_atom, t.wg_atom);
-    up(nl->d_wg_img_off, t.wg_img_off);
-    up(nl->d_wg_img, t.wg_img);
-    up(nl->d_wg_img_atom, t.wg_img_atom);
-    up(nl->d_seg_off, t.seg_off);
-    up(nl->d_seg, t.seg);
-    std::vector<uint32_t>().swap(t.seg);
-    up(nl->d_item_off, t.item_off);
-    up(nl->d_item_meta, t.item_meta);
-    up(nl->d_item_beta64, t.item_beta);
+    up(c.d_wg_atom_off, t.wg_atom_off);
+    up(c.d_wg_atom, t.wg_atom);
+    up(c.d_wg_img_off, t.wg_img_off);
+    up(c.d_wg_img, t.wg_img);
+    up(c.d_wg_img_atom, t.wg_img_atom);
+    up(c.d_seg_off, t.seg_off);
+    up(c.d_seg, t.seg);
+    up(c.d_item_off, t.item_off);
+    up(c.d_item_meta, t.item_meta);
+    up(c.d_item_beta64, t.item_beta);
     std::vector<float> b32(t.item_beta.begin(), t.item_beta.end());
-    up(nl->d_item_beta32, b32);
+    up(c.d_item_beta32, b32);
     // the host copies are not needed once uploaded
+    std::vector<uint32_t>().swap(t.seg);
     std::vector<uint32_t>().swap(t.item_meta);
     std::vector<double>().swap(t.item_beta);
-    nl->plan_ty = p.ty;
-    nl->plan_tz = p.tz;
+    c.ty = p.ty;
+    c.tz = p.tz;
+    return c;
+}
+
+// Projector form of a launch: in-kernel values for the FP32 projector pass when the handle
+// allows them (QG_NL_ANALYTIC=2: for every launch, a diagnostic), per-point tables otherwise.
+static bool segments_form(const qg_nonlocal* nl, int dtype, int mode) {
+    if (!nl->analytic) return false;
+    if (nl->analytic_level >= 2) return true;
+    return mode == MODE_NL && dtype == QG_F32;
 }
 
 template <class T, int NB, int MODE, int NJS>
@@ -823,42 +827,40 @@ static FusedSetup<T> setup_fused(const qg_grid& g, int nbands, int64_t ld_in, in
     size_t lds = (MODE == MODE_NL) ? 0 : 2 * (size_t)p.nb * (p.ty + 8) * (g.n[0] + 8) * sizeof(T);
     int njs = 4;
     if (nl && nl->geo.ncoef > 0 && MODE != MODE_LAP) {
-        prepare_tile_nl(nl, p);
-        if (nl->analytic) {
-            // the in-kernel form stages the slab's images and segments; tables when they do not fit
-            const TileNL& t = nl->tile;
+        bool seg = segments_form(nl, dtype, MODE);
+        if (seg) {
+            // the in-kernel form stages the slab's images and segments; tables when they do not
+            // fit in LDS or the segment encoding
+            const TileNL& t = prepare_tile_nl(nl, p, true).tile;
             const size_t need = ((lds + (size_t)std::max(1, t.max_atom) * nl->geo.njs * p.nb * sizeof(T) + 7) & ~size_t(7)) +
                                 (size_t)std::max(1, t.max_img) * (3 * sizeof(T) + 6 * sizeof(int)) +
                                 nl->geo.projs.size() * (2 * sizeof(T) + 2 * sizeof(int)) +
                                 (size_t)(p.tz * p.ty + 1) * sizeof(int) + (size_t)std::max(1, t.max_seg) * sizeof(uint32_t);
-            if (need > 65536) {
-                nl->analytic = false;
-                nl->plan_ty = nl->plan_tz = -1;
-                prepare_tile_nl(nl, p);
-            }
+            if (need > 65536 || t.seg_off.empty()) seg = false;
         }
-        const TileNL& t = nl->tile;
+        qg_nonlocal::Tiles& tc = prepare_tile_nl(nl, p, seg);
+        const TileNL& t = tc.tile;
         njs = nl->geo.njs;
         a.has_nl = 1;
-        a.wg_atom_off = nl->d_wg_atom_off.p;
-        a.wg_atom = nl->d_wg_atom.p;
-        a.item_off = nl->d_item_off.p;
-        a.item_meta = nl->d_item_meta.p;
-        a.item_beta = (dtype == QG_F32) ? (const void*)nl->d_item_beta32.p : (const void*)nl->d_item_beta64.p;
+        a.wg_atom_off = tc.d_wg_atom_off.p;
+        a.wg_atom = tc.d_wg_atom.p;
+        a.item_off = tc.d_item_off.p;
+        a.item_meta = tc.d_item_meta.p;
+        a.item_beta = (dtype == QG_F32) ? (const void*)tc.d_item_beta32.p : (const void*)tc.d_item_beta64.p;
         a.coef = coefs;
         a.coef_energy = nl->d_coef_energy.p;
         a.atom_coef_off = nl->d_atom_coef_off.p;
         a.ncoef = nl->geo.ncoef;
         a.max_atom = std::max(1, t.max_atom);
         const size_t ec = (size_t)a.max_atom * njs * p.nb * sizeof(T);
-        if (nl->analytic) {
+        if (seg) {
             a.analytic = 1;
             a.max_img = std::max(1, t.max_img);
-            a.seg_off = nl->d_seg_off.p;
-            a.seg = nl->d_seg.p;
-            a.wg_img_off = nl->d_wg_img_off.p;
-            a.wg_img = nl->d_wg_img.p;
-            a.wg_img_atom = nl->d_wg_img_atom.p;
+            a.seg_off = tc.d_seg_off.p;
+            a.seg = tc.d_seg.p;
+            a.wg_img_off = tc.d_wg_img_off.p;
+            a.wg_img = tc.d_wg_img.p;
+            a.wg_img_atom = tc.d_wg_img_atom.p;
             a.images = nl->d_images.p;
             a.projs = nl->d_projs.p;
             a.hx = g.h[0]; a.hy = g.h[1]; a.hz = g.h[2];
@@ -956,9 +958,11 @@ static void launch_project(qg_nonlocal* nl, int nbands, const T* psi, int64_t ld
 
 // Bands per chunk so that one chunk of psi stays resident in the Infinity Cache between
 // the projection and the fused apply (QG_CHUNK_MB, default 128 MB).
-static int chunk_bands(const qg_grid& g, int dtype, int nbands) {
+static int chunk_bands(const qg_grid& g, int dtype, int nbands, bool split) {
     const int64_t bytes = g.ng() * (dtype == QG_F32 ? 4 : 8);
-    const int64_t budget = (int64_t)env_int("QG_CHUNK_MB", 128) << 20;
+    // the split form keeps nothing in the Infinity Cache between its passes, so its chunks
+    // only bound the coefficient scratch
+    const int64_t budget = (int64_t)env_int("QG_CHUNK_MB", split ? 512 : 128) << 20;
     int64_t c = std::max<int64_t>(1, budget / std::max<int64_t>(1, bytes));
     return (int)std::min<int64_t>(nbands, c);
 }
@@ -968,8 +972,11 @@ void hamiltonian_apply(qg_ham* h, int dtype, int nbands, const void* psi, int64_
                        int64_t ld_out, hipStream_t s) {
     const qg_grid& g = h->grid;
     qg_nonlocal* nl = (h->nl && h->nl->geo.ncoef > 0) ? h->nl : nullptr;
-    const int cb = nl ? chunk_bands(g, dtype, nbands) : nbands;
-    const bool split = env_int("QG_SPLIT_NL", 0) != 0;
+    // split form (stencil + V pass, then a projector pass over many bands at a time) for FP32
+    // with in-kernel projector values; one fused pass otherwise (QG_SPLIT_NL=0/1 forces)
+    const int split_env = env_int("QG_SPLIT_NL", -1);
+    const bool split = nl && (split_env >= 0 ? split_env != 0 : segments_form(nl, dtype, MODE_NL));
+    const int cb = nl ? chunk_bands(g, dtype, nbands, split) : nbands;
     if (nl) nl->d_coef.alloc((size_t)std::min(cb, nbands) * nl->geo.ncoef);
     const int nchunks = (nbands + cb - 1) / cb;
     if (h->timing) {
@@ -1057,7 +1064,7 @@ static void batch_apply_t(const std::vector<BatchReq>& reqs, hipStream_t s, DevB
     for (int c = 0; c < n; ++c) {
         const BatchReq& r = reqs[c];
         qg_nonlocal* nl = (r.h->nl && r.h->nl->geo.ncoef > 0) ? r.h->nl : nullptr;
-        cb[c] = nl ? chunk_bands(r.h->grid, dtype, r.nbands) : std::max(1, r.nbands);
+        cb[c] = nl ? chunk_bands(r.h->grid, dtype, r.nbands, false) : std::max(1, r.nbands);
         if (nl) {
             nl->d_coef.alloc((size_t)std::min(cb[c], r.nbands) * nl->geo.ncoef);
             QG_REQUIRE(njs == 0 || njs == nl->geo.njs, "cells of one batch must share the projector slot count");
@@ -1301,11 +1308,12 @@ int qg_nonlocal_create(const qg_grid* g, int natoms, const double* xyz, const in
             if (!geo.coef_energy.empty()) nl->d_coef_energy.upload(geo.coef_energy.data(), geo.coef_energy.size());
             nl->d_atom_img_off.upload(geo.atom_img_off.data(), geo.atom_img_off.size());
             nl->d_atom_coef_off.upload(geo.atom_coef_off.data(), geo.atom_coef_off.size());
-            // Gaussian projectors can be evaluated inside the H psi kernel instead of read from
-            // per-point tables (QG_NL_ANALYTIC=1)
+            // Gaussian projectors can be evaluated inside the projector pass instead of read
+            // from per-point tables (QG_NL_ANALYTIC=0 keeps the tables everywhere)
             bool gauss = !geo.projs.empty();
             for (const DevProj& d : geo.projs) gauss = gauss && d.kind == QG_RADIAL_GAUSSIAN && d.l <= 1;
-            nl->analytic = gauss && g->n[0] <= 1024 && env_int("QG_NL_ANALYTIC", 0) != 0;
+            nl->analytic_level = env_int("QG_NL_ANALYTIC", 1);
+            nl->analytic = gauss && g->n[0] <= 1024 && nl->analytic_level != 0;
         } catch (...) {
             delete nl;
             throw;

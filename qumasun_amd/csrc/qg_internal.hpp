@@ -170,17 +170,23 @@ struct qg_nonlocal {
     qg::DevBuf<int> d_atom_img_off;
     qg::DevBuf<int> d_atom_coef_off;
     qg::DevBuf<double> d_coef;   // scratch coefficients (nbands * ncoef)
-    // per-plan staging lists, cached for one (ty, tz)
-    int plan_ty = -1, plan_tz = -1;
-    qg::TileNL tile;
-    qg::DevBuf<int> d_wg_img_off, d_wg_img, d_wg_img_atom, d_wg_atom_off, d_wg_atom;
-    qg::DevBuf<int64_t> d_item_off;
-    qg::DevBuf<uint32_t> d_item_meta;
-    qg::DevBuf<int> d_seg_off;
-    qg::DevBuf<uint32_t> d_seg;
-    bool analytic = false;   // every projector Gaussian: values evaluated in the kernel
-    qg::DevBuf<double> d_item_beta64;
-    qg::DevBuf<float> d_item_beta32;
+    // per-plan staging lists, cached for one (ty, tz) per projector form: [0] per-point
+    // tables (item lists), [1] in-kernel values (row segments)
+    struct Tiles {
+        int ty = -1, tz = -1;
+        qg::TileNL tile;
+        qg::DevBuf<int> d_wg_img_off, d_wg_img, d_wg_img_atom, d_wg_atom_off, d_wg_atom;
+        qg::DevBuf<int64_t> d_item_off;
+        qg::DevBuf<uint32_t> d_item_meta;
+        qg::DevBuf<int> d_seg_off;
+        qg::DevBuf<uint32_t> d_seg;
+        qg::DevBuf<double> d_item_beta64;
+        qg::DevBuf<float> d_item_beta32;
+    } tiles[2];
+    // every projector Gaussian with l <= 1 (and QG_NL_ANALYTIC != 0 at creation): the FP32
+    // projector pass evaluates the values in the kernel
+    bool analytic = false;
+    int analytic_level = 0;   // QG_NL_ANALYTIC: 1 FP32 projector pass, 2 every launch
     qg::DevBuf<double> d_coef_energy;   // KB energy of every coefficient slot
     // projector tables
     qg::DevBuf<double> d_beta64;

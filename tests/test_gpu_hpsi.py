@@ -144,16 +144,20 @@ def test_hamiltonian_cfg1_subset(gpu, oracle):
         assert err <= tol, (dt, err)
 
 
-@pytest.mark.parametrize("analytic", ["0", "1"])
+@pytest.mark.parametrize("analytic,split", [("0", None), ("1", None), ("2", None), ("2", "0"), ("0", "1")])
 @pytest.mark.parametrize("prec", ["f64", "f32"])
-def test_hamiltonian_projector_paths(gpu, oracle, monkeypatch, prec, analytic):
-    """Both forms of the projector term of the fused kernel (per-point tables, and values
-    evaluated in the kernel from the image centres) against the oracle: an orthorhombic cell
-    with unequal spacings, two species with different cutoffs, atoms on the cell faces and a
-    cutoff sphere wider than the cell along z (two images of one atom on the same rows)."""
+def test_hamiltonian_projector_paths(gpu, oracle, monkeypatch, prec, analytic, split):
+    """Every form of the projector term against the oracle: per-point tables or values
+    evaluated in the kernel from the image centres (QG_NL_ANALYTIC 0 / 1 = FP32 projector pass /
+    2 = every launch), in the fused pass or in a separate projector pass (QG_SPLIT_NL). The cell
+    is orthorhombic with unequal spacings, two species with different cutoffs, atoms on the cell
+    faces and a cutoff sphere wider than the cell along z (two images of one atom on the same
+    rows)."""
     import torch
-    print(f"projector paths prec={prec} analytic={analytic}", flush=True)
+    print(f"projector paths prec={prec} analytic={analytic} split={split}", flush=True)
     monkeypatch.setenv("QG_NL_ANALYTIC", analytic)
+    if split is not None:
+        monkeypatch.setenv("QG_SPLIT_NL", split)
     n = (30, 26, 22)
     L = (6.0, 5.5, 5.0)
     h = tuple(L[d] / n[d] for d in range(3))
@@ -176,7 +180,7 @@ def test_hamiltonian_projector_paths(gpu, oracle, monkeypatch, prec, analytic):
     out = torch.empty_like(xt)
     H.apply(xt, out)
     torch.cuda.synchronize()
-    assert integ.projector_form == ("kernel" if analytic == "1" else "tables")
+    assert integ.projector_form == ("tables" if analytic == "0" else "kernel")
     err = rel(_np(out), ref)
     assert err <= (TOL64 if prec == "f64" else TOL32), err
     # the projector term alone (H psi minus the local part) to the same bar
