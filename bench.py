@@ -134,6 +134,9 @@ def eigen_leg(cfg):
             step += 1
             it += st["iterations"]
             el = time.perf_counter() - t0
+            # progress on stderr (stdout carries the one JSON line)
+            print(f"eigen {cfg}: {it} iterations, {el:.1f} s, max residual {st['max_residual']:.2e}",
+                  file=sys.stderr, flush=True)
             if ref is None and st["max_residual"] <= kw.get("tol", 0.0):
                 return e, el, it, True
             if ref is not None and float(np.abs(e - ref).max()) <= 1e-6:

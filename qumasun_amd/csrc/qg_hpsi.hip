@@ -134,8 +134,9 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const StencilCoef
     T* ec_l = nullptr;                        // E*coef per (local atom, slot, band)
     const int nlp = a.nlp;
     const bool analytic = MODE != MODE_LAP && a.has_nl && a.analytic;
-    // in-kernel projector values: per image the displacement at its box corner (3 x T) and
-    // box corner, atom slot, projector range (6 ints); per projector amplitude and 1/(2 sigma^2)
+    // in-kernel projector values: per image the fractional part of its centre in grid units
+    // (3 x T) and integer part + shift, atom slot, projector range (6 ints); per projector
+    // amplitude and 1/(2 sigma^2)
     // (2 x T), l and coefficient slot (2 ints); the slab's segment offsets and segments
     T* im_o = nullptr;
     T* pr_f = nullptr;
@@ -242,10 +243,15 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const StencilCoef
             const double hh[3] = {a.hx, a.hy, a.hz};
             for (int li = tid; li < nimg; li += nthr) {
                 const DevImage& im = a.images[a.wg_img[i0 + li]];
+                // centre in grid units u = R / h = n + f (n nearest integer): the displacement
+                // h ((i - shift - n) - f) is then exact to the last bit of f, so a nucleus on a
+                // grid point gives r^2 ~ 1e-30 there (Y_1m = 0, as in the host tables) in FP32 too
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
-                    im_o[3 * li + d] = (T)(hh[d] * (double)(im.b[d] - im.shift[d]) - im.R[d]);
-                    im_i[6 * li + d] = im.b[d];
+                    const double u = im.R[d] / hh[d];
+                    const double nn = rint(u);
+                    im_o[3 * li + d] = (T)(u - nn);
+                    im_i[6 * li + d] = im.shift[d] + (int)nn;
                 }
                 im_i[6 * li + 3] = a.wg_img_atom[i0 + li];
                 im_i[6 * li + 4] = im.proj_begin;
@@ -393,8 +399,8 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const StencilCoef
             }
         if (analytic) {
             // beta_j = p(r) Y_lm of every sphere this point lies in, evaluated here in the
-            // block's precision (displacement from the image's box corner, so every term is a
-            // few bohr at most), and acc_b = sum_j E_j c_jb beta_j
+            // block's precision (displacement from the integer part of the image centre, so
+            // every term is a few grid spacings at most), and acc_b = sum_j E_j c_jb beta_j
             const T hx = (T)a.hx, hy = (T)a.hy, hz = (T)a.hz;
 #pragma unroll
             for (int k = 0; k < PPT; ++k) {
@@ -414,9 +420,9 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const StencilCoef
                     const int li = (int)(sv >> 22);
                     const int* ii = im_i + 6 * li;
                     const T* io = im_o + 3 * li;
-                    const T xc = io[0] + hx * (T)(ix - ii[0]);
-                    const T yc = io[1] + hy * (T)(iy - ii[1]);
-                    const T zc = io[2] + hz * (T)(z - ii[2]);
+                    const T xc = hx * ((T)(ix - ii[0]) - io[0]);
+                    const T yc = hy * ((T)(iy - ii[1]) - io[1]);
+                    const T zc = hz * ((T)(z - ii[2]) - io[2]);
                     const T rr = xc * xc + (yc * yc + zc * zc);
                     const T* ec = ec_l + ii[3] * NJS * NB;
                     T yv[3] = {T(0), T(0), T(0)};
