@@ -133,7 +133,10 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const StencilCoef
     T* nl_l = reinterpret_cast<T*>(lp);       // per-point projector term, nlp x NB x tile_pts
     T* ec_l = nullptr;                        // E*coef per (local atom, slot, band)
     const int nlp = a.nlp;
-    const bool analytic = MODE != MODE_LAP && a.has_nl && a.analytic;
+    // in-kernel projector values exist in the projector pass only (the fused pass keeps its
+    // registers for the stencil queue)
+    constexpr bool kAnalyticMode = MODE == MODE_NL;
+    const bool analytic = kAnalyticMode && a.has_nl && a.analytic;
     // in-kernel projector values: per image the fractional part of its centre in grid units
     // (3 x T) and integer part + shift, atom slot, projector range (6 ints); per projector
     // amplitude and 1/(2 sigma^2)
@@ -397,7 +400,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const StencilCoef
                     *slot = T(0);
                 }
             }
-        if (analytic) {
+        if (kAnalyticMode && analytic) {
             // beta_j = p(r) Y_lm of every sphere this point lies in, evaluated here in the
             // block's precision (displacement from the integer part of the image centre, so
             // every term is a few grid spacings at most), and acc_b = sum_j E_j c_jb beta_j
@@ -710,7 +713,7 @@ static Plan make_plan(const qg_grid& g, int dtype, int nbands, int mode) {
     Plan p{};
     if (mode == MODE_NL) {
         // projector-only pass: no stencil state, so many bands share each projector item
-        const int want = env_int("QG_PLAN_NBNL", 8);
+        const int want = env_int("QG_PLAN_NBNL", dtype == QG_F64 ? 4 : 8);
         p.nb = 1;
         for (int c : {16, 8, 4, 2})
             if (c <= want && c <= nbands) { p.nb = c; break; }
@@ -791,11 +794,11 @@ static qg_nonlocal::Tiles& prepare_tile_nl(qg_nonlocal* nl, const Plan& p, bool 
 }
 
 // Projector form of a launch: in-kernel values for the FP32 projector pass when the handle
-// allows them (QG_NL_ANALYTIC=2: for every launch, a diagnostic), per-point tables otherwise.
+// allows them (QG_NL_ANALYTIC=2: for the FP64 projector pass too), per-point tables otherwise;
+// the fused single pass always reads tables.
 static bool segments_form(const qg_nonlocal* nl, int dtype, int mode) {
-    if (!nl->analytic) return false;
-    if (nl->analytic_level >= 2) return true;
-    return mode == MODE_NL && dtype == QG_F32;
+    if (!nl->analytic || mode != MODE_NL) return false;
+    return dtype == QG_F32 || nl->analytic_level >= 2;
 }
 
 template <class T, int NB, int MODE, int NJS>

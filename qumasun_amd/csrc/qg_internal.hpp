@@ -186,7 +186,7 @@ struct qg_nonlocal {
     // every projector Gaussian with l <= 1 (and QG_NL_ANALYTIC != 0 at creation): the FP32
     // projector pass evaluates the values in the kernel
     bool analytic = false;
-    int analytic_level = 0;   // QG_NL_ANALYTIC: 1 FP32 projector pass, 2 every launch
+    int analytic_level = 0;   // QG_NL_ANALYTIC: 1 FP32 projector pass, 2 FP64 projector pass too
     qg::DevBuf<double> d_coef_energy;   // KB energy of every coefficient slot
     // projector tables
     qg::DevBuf<double> d_beta64;

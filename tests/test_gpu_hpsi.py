@@ -149,7 +149,8 @@ def test_hamiltonian_cfg1_subset(gpu, oracle):
 def test_hamiltonian_projector_paths(gpu, oracle, monkeypatch, prec, analytic, split):
     """Every form of the projector term against the oracle: per-point tables or values
     evaluated in the kernel from the image centres (QG_NL_ANALYTIC 0 / 1 = FP32 projector pass /
-    2 = every launch), in the fused pass or in a separate projector pass (QG_SPLIT_NL). The cell
+    2 = FP64 projector pass too), in the fused pass (tables) or in a separate projector pass
+    (QG_SPLIT_NL). The cell
     is orthorhombic with unequal spacings, two species with different cutoffs, atoms on the cell
     faces and a cutoff sphere wider than the cell along z (two images of one atom on the same
     rows)."""
