@@ -64,6 +64,7 @@ struct FusedArgs {
     int nlp;                   // z-planes per projector phase (accumulator ring depth, 1..4)
     // in-kernel projector values (every projector Gaussian): sphere segments per tile row and
     // the images / projectors they refer to
+    int tile_major;            // workgroup order: tile fastest (no projector term) or band fastest
     int analytic;
     int max_img;
     int max_seg;               // most sphere segments of one (tile, slab)
@@ -167,14 +168,23 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const StencilCoef
         }
     }
 
-    // logical order: band chunk fastest, so the workgroups of one (tile, slab) run back to
-    // back on one XCD and find V and the projector items in its L2; halo rows and the
-    // z warm-up planes of neighbouring tiles come from the Infinity Cache
+    // logical order with a projector term: band chunk fastest, so the workgroups of one
+    // (tile, slab) run back to back on one XCD and find V and the projector data in its L2
     const int nchunk = (a.nbands + NB - 1) / NB;
-    const int chunk = wg % nchunk;
-    const int rest = wg / nchunk;
-    const int ytile = rest % a.ntile_y;
-    const int slab = rest / a.ntile_y;
+    int chunk, ytile, slab;
+    if (a.tile_major) {
+        // no projector term: neighbouring tiles and slabs of one band run back to back on one
+        // XCD, so their shared halo rows and warm-up planes come from its L2
+        ytile = wg % a.ntile_y;
+        const int rest = wg / a.ntile_y;
+        slab = rest % a.nslab;
+        chunk = rest / a.nslab;
+    } else {
+        chunk = wg % nchunk;
+        const int rest = wg / nchunk;
+        ytile = rest % a.ntile_y;
+        slab = rest / a.ntile_y;
+    }
     const int y0 = ytile * a.ty;
     const int rows = min(a.ty, ny - y0);
     const int z0 = slab * a.tz;
@@ -626,7 +636,9 @@ __device__ __forceinline__ void project_body(const ProjArgs& a, const T* __restr
 template <class T, int NBA, int NJS>
 __global__ __launch_bounds__(kProjBlock) void project_kernel(ProjArgs a, const T* __restrict__ psi) {
     __shared__ double red[kProjBlock / 64][NJS * NBA];
-    project_body<T, NBA, NJS>(a, psi, blockIdx.x, red);
+    // consecutive items (neighbouring atoms of one band chunk, whose spheres share grid rows)
+    // on one XCD, so the shared rows come from that XCD's L2
+    project_body<T, NBA, NJS>(a, psi, xcd_remap(blockIdx.x, gridDim.x), red);
 }
 
 template <class T>
@@ -898,6 +910,7 @@ static FusedSetup<T> setup_fused(const qg_grid& g, int nbands, int64_t ld_in, in
         }
     }
     QG_REQUIRE(lds <= 65536, "tile staging does not fit in 64 KiB of LDS");
+    a.tile_major = (!a.has_nl && MODE != MODE_NL && env_int("QG_TILE_MAJOR", 1) != 0) ? 1 : 0;
     const StencilCoef<T> sc = make_coef<T>(g, coef);
     const int nchunk = (nbands + p.nb - 1) / p.nb;
     const int64_t nwg = (int64_t)a.ntile_y * a.nslab * nchunk;
