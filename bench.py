@@ -237,6 +237,10 @@ def main():
     fused_launch_ms = t_fused / nchunks
     achieved = bytes_launch / (fused_launch_ms * 1e-3) / 1e9
     table_bytes = int(round(1.51 * ng)) * 4 * s   # beta tables, 4 slots per sphere point
+    split = (args.dtype == "f32" and integ.projector_form == "kernel" and os.environ.get("QG_SPLIT_NL", "") != "0") \
+        or os.environ.get("QG_SPLIT_NL", "") == "1"
+    kernel_desc = ("fused_hpsi_kernel: stencil+V pass + projector pass (in-kernel projector values) per band chunk"
+                   if split else "fused_hpsi_kernel: one fused stencil+V+projector pass per band chunk")
 
     res = {
         "metric": "Hpsi HBM GB/s (algorithmic bytes / step time)",
@@ -254,7 +258,7 @@ def main():
         "config": {"workload": "Hpsi 64-atom diamond 2x2x2, L=13.48 bohr, 80^3 grid, 128 bands",
                    "grid": [n, n, n], "atoms": len(sysd["positions"]), "bands": args.bands,
                    "projectors": "gaussian s+p, sigma 0.6, rc 2.4", "band_chunks": nchunks},
-        "roofline": {"bound": "hbm", "kernel": "fused_hpsi_kernel",
+        "roofline": {"bound": "hbm", "kernel": kernel_desc,
                      "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": traffic_per_launch(args.dtype, chunk_bands),
