@@ -117,6 +117,28 @@ constexpr int kItemPT = 4;  // projector work items per thread per round
 
 // Fused kernel.  T = float/double, NB bands per workgroup, PPT points per thread, NJS table
 // slots per projector item.  blockDim.x = threads (multiple of 64, <= 512).
+// NB consecutive LDS values (16-byte aligned rows of the E*coef table) as 16-byte reads
+template <class T, int NB>
+__device__ __forceinline__ void lds_row(const T* __restrict__ src, T (&dst)[NB]) {
+    constexpr int V = 16 / sizeof(T);
+    if constexpr (NB % V == 0) {
+        using Vec = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+        const Vec* s4 = reinterpret_cast<const Vec*>(src);
+#pragma unroll
+        for (int i = 0; i < NB / V; ++i) {
+            const Vec v = s4[i];
+            if constexpr (sizeof(T) == 4) {
+                dst[4 * i] = v.x; dst[4 * i + 1] = v.y; dst[4 * i + 2] = v.z; dst[4 * i + 3] = v.w;
+            } else {
+                dst[2 * i] = v.x; dst[2 * i + 1] = v.y;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int b = 0; b < NB; ++b) dst[b] = src[b];
+    }
+}
+
 template <class T, int NB, int PPT, int MODE, int NJS>
 __device__ __forceinline__ void fused_body(const FusedArgs& a, const StencilCoef<T>& sc, const T* __restrict__ V,
                                            const T* __restrict__ psi, T* __restrict__ out, const int wg,
@@ -450,7 +472,11 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const StencilCoef
                     const T* ec = ec_l + ii[3] * NJS * NB;
                     T yv[3] = {T(0), T(0), T(0)};
                     if (!(T(1.0e-14) > rr)) {
-                        const T ir = T(0.48860251190291992) / sqrt(rr);   // sqrt(3/4pi) / r
+                        // sqrt(3/4pi) / r; FP32 through the hardware reciprocal square root
+                        // (about 1 ulp), FP64 with a correctly rounded division
+                        T ir;
+                        if constexpr (sizeof(T) == 4) ir = T(0.48860251190291992) * __frsqrt_rn(rr);
+                        else ir = T(0.48860251190291992) / sqrt(rr);
                         yv[0] = yc * ir;
                         yv[1] = zc * ir;
                         yv[2] = xc * ir;
@@ -460,13 +486,18 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const StencilCoef
                         const int cr = pr_i[2 * pj + 1];
                         if (pr_i[2 * pj] == 0) {
                             const T bv = rad * T(0.28209479177387814);   // 1/sqrt(4 pi)
+                            T e0[NB];
+                            lds_row<T, NB>(ec + cr * NB, e0);
 #pragma unroll
-                            for (int b = 0; b < NB; ++b) acc[b] += ec[cr * NB + b] * bv;
+                            for (int b = 0; b < NB; ++b) acc[b] += e0[b] * bv;
                         } else {
+                            T e0[NB], e1[NB], e2[NB];
+                            lds_row<T, NB>(ec + cr * NB, e0);
+                            lds_row<T, NB>(ec + (cr + 1) * NB, e1);
+                            lds_row<T, NB>(ec + (cr + 2) * NB, e2);
 #pragma unroll
                             for (int b = 0; b < NB; ++b)
-                                acc[b] += rad * (ec[cr * NB + b] * yv[0] + ec[(cr + 1) * NB + b] * yv[1] +
-                                                 ec[(cr + 2) * NB + b] * yv[2]);
+                                acc[b] += rad * (e0[b] * yv[0] + e1[b] * yv[1] + e2[b] * yv[2]);
                         }
                     }
                   }
@@ -730,10 +761,10 @@ static Plan make_plan(const qg_grid& g, int dtype, int nbands, int mode) {
         for (int c : {16, 8, 4, 2})
             if (c <= want && c <= nbands) { p.nb = c; break; }
         p.ty = std::max(1, std::min(ny, env_int("QG_PLAN_TYNL", std::max(1, 640 / nx))));
-        p.ppt = 4;
+        p.ppt = env_int("QG_PLAN_PPTNL", 2) == 4 ? 4 : 2;
         p.nthr = round64((p.ty * nx + p.ppt - 1) / p.ppt);
         while (p.nthr > kMaxThreads) {
-            p.ppt = (p.ppt == 4) ? 5 : 8;
+            p.ppt = (p.ppt == 2) ? 4 : (p.ppt == 4) ? 5 : 8;
             p.nthr = round64((p.ty * nx + p.ppt - 1) / p.ppt);
             QG_REQUIRE(p.ppt <= 8 && (p.ppt < 8 || p.nthr <= kMaxThreads), "tile too large");
         }
