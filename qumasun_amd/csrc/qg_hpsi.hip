@@ -719,10 +719,6 @@ struct Plan {
 
 static int round64(int v) { return (v + 63) / 64 * 64; }
 
-static int env_int(const char* name, int dflt) {
-    const char* v = std::getenv(name);
-    return (v && *v) ? std::atoi(v) : dflt;
-}
 
 // Tile shapes chosen by the autotuner (qg_autotune_hamiltonian), per grid shape and dtype;
 // make_plan uses them for the stencil modes unless QG_PLAN_* overrides are set.
@@ -760,7 +756,8 @@ static Plan make_plan(const qg_grid& g, int dtype, int nbands, int mode) {
         p.nb = 1;
         for (int c : {16, 8, 4, 2})
             if (c <= want && c <= nbands) { p.nb = c; break; }
-        p.ty = std::max(1, std::min(ny, env_int("QG_PLAN_TYNL", std::max(1, 640 / nx))));
+        // measured at 80^3 FP32: 4-row tiles, 2 points per thread, 2-plane slabs, 8 bands
+        p.ty = std::max(1, std::min(ny, env_int("QG_PLAN_TYNL", std::max(1, 320 / nx))));
         p.ppt = env_int("QG_PLAN_PPTNL", 2) == 4 ? 4 : 2;
         p.nthr = round64((p.ty * nx + p.ppt - 1) / p.ppt);
         while (p.nthr > kMaxThreads) {
@@ -768,8 +765,7 @@ static Plan make_plan(const qg_grid& g, int dtype, int nbands, int mode) {
             p.nthr = round64((p.ty * nx + p.ppt - 1) / p.ppt);
             QG_REQUIRE(p.ppt <= 8 && (p.ppt < 8 || p.nthr <= kMaxThreads), "tile too large");
         }
-        // no z warm-up in this pass: thin slabs give many workgroups (measured best at 80^3:
-        // 1-2 planes, 8 bands)
+        // no z warm-up in this pass: thin slabs give many workgroups
         p.tz = std::max(1, std::min(nz, env_int("QG_PLAN_TZNL", 2)));
         return p;
     }

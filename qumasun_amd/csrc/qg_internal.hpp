@@ -5,6 +5,7 @@
 #include <cstdint>
 #include <string>
 #include <vector>
+#include <cstdlib>
 #include "../../include/qumasun_gpu.h"
 
 namespace qg {
@@ -46,6 +47,12 @@ hipStream_t default_stream();
 inline hipStream_t as_stream(void* s) { return (hipStream_t)s; }
 
 // ------------------------------------------------------------- device buffers ----
+// Integer tuning override from the environment (dflt when unset or empty).
+inline int env_int(const char* name, int dflt) {
+    const char* v = std::getenv(name);
+    return (v && *v) ? std::atoi(v) : dflt;
+}
+
 template <class T>
 struct DevBuf {
     T* p = nullptr;

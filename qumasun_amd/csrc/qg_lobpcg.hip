@@ -263,7 +263,7 @@ struct Ops {
                 return;
             }
         }
-        const int64_t nch = std::max<int64_t>(1, std::min<int64_t>(256, K / 2048));
+        const int64_t nch = split_chunks(K, N);
         const int64_t kc = K / nch;
         const int64_t rem = K - nch * kc;
         const int64_t nn = (int64_t)N * nb;
@@ -289,8 +289,14 @@ struct Ops {
         hipLaunchKernelGGL((split_reduce_kernel<T>), dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, W,
                            (int)(nch + (rem ? 1 : 0)), N, nb, C, ldc);
     }
+    // K chunks of the split-K Gram: enough strided-batched work for small N; QG_GRAM_NCH overrides
+    static int64_t split_chunks(int64_t K, int N) {
+        const int64_t want = env_int("QG_GRAM_NCH", 0);
+        if (want > 0) return std::max<int64_t>(1, std::min<int64_t>(want, K / 64));
+        return std::max<int64_t>(1, std::min<int64_t>(256, K / 2048));
+    }
     static size_t split_workspace(int64_t K, int N, int nb_max = 0) {
-        const int64_t nch = std::max<int64_t>(1, std::min<int64_t>(256, K / 2048));
+        const int64_t nch = split_chunks(K, N);
         return (size_t)(nch + 1) * N * (nb_max > 0 ? nb_max : N) * sizeof(T);
     }
     // Out = A * Cm + beta * Out, A: K x N (ld K), Cm: N x N FP64 (ld ldc) / FP32 copy for T = float
