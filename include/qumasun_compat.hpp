@@ -179,6 +179,12 @@ private:
     qg_ham* h_ = nullptr;
 };
 
+// GramSchmidt(N, num_solution, psi_set, dVol) (GramSchmidt.h:6-38) / GramSchmidt_ddm
+// (GramSchmidt_mpi.h:9-53) on a num_solution x N FP64 band-major device block, in place.
+inline void GramSchmidt(int64_t N, int num_solution, double* psi_dev, double dVol) {
+    check(qg_gram_schmidt(QG_F64, N, num_solution, psi_dev, N, dVol, nullptr), "qg_gram_schmidt");
+}
+
 namespace LOBPCG {
 
 // Eigen_d_multi_mpi(l_grid, dVol, num_solution, iter_max, eigen_values, eigen_vectors, keep_P,

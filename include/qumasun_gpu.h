@@ -202,6 +202,14 @@ int qg_lobpcg_d_multi(qg_ham* h, const qg_lobpcg_opts* opts, int nbands,
                       double* eigvals_host, double* X_dev, double* P_dev, int scf_step,
                       qg_lobpcg_stats* stats);
 
+/* Orthonormalise nbands band-major device vectors X (ld >= ng, FP64 or FP32) in place so
+ * that dVol * sum_i x_m(i) x_n(i) = delta_mn, with the result of Gram-Schmidt in band order.
+ * Replaces GramSchmidt (GramSchmidt.h:6-38) and GramSchmidt_ddm (GramSchmidt_mpi.h:9-53),
+ * used on the initial states (qumasun_mpi.initialize.h:183, :253): computed as two
+ * Cholesky-QR passes (Gram GEMM, Cholesky, triangular solve), which give the same basis as
+ * the band-by-band projections up to rounding.  QG_ERR_NUMERIC when the bands are dependent. */
+int qg_gram_schmidt(int dtype, int64_t ng, int nbands, void* X, int64_t ld, double dvol, void* stream);
+
 /* Apply a preconditioner in place to nbands residual vectors r (device, ld == Ng):
  * QG_PRECOND_KINETIC: r <- (T + shift)^-1 r with T = -1/2 Lap8 the periodic 8th-order kinetic
  * operator, exactly, through FFTs.  No counterpart in the reference (which uses the raw

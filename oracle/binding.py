@@ -15,7 +15,7 @@ c_int, c_double, c_void_p, c_int64, c_uint32 = (ctypes.c_int, ctypes.c_double, c
 P = ctypes.POINTER
 DP = P(c_double)
 
-__all__ = ["REF", "REF_SRC", "_d", "build", "olib", "ref_lib", "OracleNL", "laplacian8", "hamiltonian_block",
+__all__ = ["REF", "REF_SRC", "_d", "build", "olib", "ref_lib", "OracleNL", "laplacian8", "hamiltonian_block", "gram_schmidt",
            "lobpcg_hamiltonian", "lobpcg_dense", "dsygvx", "random_states", "radial_by_xi"]
 
 
@@ -57,6 +57,7 @@ def olib():
         l.or_lobpcg_dense.argtypes = [c_int64, DP, c_int, c_int, c_double, DP, DP, DP, c_int, P(c_int), P(c_int), DP]
         l.or_dsygvx.argtypes = [c_int, DP, DP, DP, DP]
         l.or_set_threads.argtypes = [c_int]
+        l.or_gram_schmidt.argtypes = [c_int64, c_int, DP, c_double]
         _o = l
     return _o
 
@@ -68,6 +69,8 @@ def ref_lib():
         l = ctypes.CDLL(REF)
         l.ref_laplacian8.argtypes = [c_int] * 3 + [c_double] * 4 + [DP, DP]
         l.ref_laplacian8_f32.argtypes = [c_int] * 3 + [c_double] * 4 + [P(ctypes.c_float), P(ctypes.c_float)]
+        if hasattr(l, "ref_gram_schmidt"):
+            l.ref_gram_schmidt.argtypes = [c_int, c_int, DP, c_double]
         _r = l
     return _r
 
@@ -178,3 +181,10 @@ def random_states(ng, nbands, dvol, seed=123456789):
     g = np.zeros(nbands)
     olib().or_random_states(ng, nbands, dvol, seed, _d(X), _d(g))
     return X.reshape(nbands, ng), g
+
+
+def gram_schmidt(X, dvol):
+    """Oracle GramSchmidt (GramSchmidt.h:6-38) of the band-major block X (nbands x ng), a copy."""
+    Y = np.ascontiguousarray(X, dtype=np.float64).copy()
+    olib().or_gram_schmidt(Y.shape[1], Y.shape[0], _d(Y), dvol)
+    return Y

@@ -470,6 +470,28 @@ OR_API void or_random_states(int64_t ng, int nbands, double dvol, uint32_t seed,
     }
 }
 
+// GramSchmidt (GramSchmidt.h:6-38): band n is orthogonalised against bands 0..n-1 one after
+// another (each projection uses the already-updated band, dVol-weighted inner products),
+// then normalised to dVol * sum psi^2 = 1.  Pinned bit for bit against the reference's own
+// function (_ref) and against numpy's QR (tests/test_oracle_gram_schmidt.py).
+OR_API void or_gram_schmidt(int64_t ng, int num, double* psi, double dvol) {
+    for (int n = 0; n < num; ++n) {
+        double* pn = psi + (int64_t)n * ng;
+        for (int k = 0; k < n; ++k) {
+            const double* pk = psi + (int64_t)k * ng;
+            double in = 0.0;
+            for (int64_t i = 0; i < ng; ++i) in += pn[i] * pk[i];
+            in *= dvol;
+            for (int64_t i = 0; i < ng; ++i) pn[i] -= in * pk[i];
+        }
+        double nrm = 0.0;
+        for (int64_t i = 0; i < ng; ++i) nrm += pn[i] * pn[i];
+        nrm *= dvol;
+        const double inv = 1.0 / std::sqrt(nrm);
+        for (int64_t i = 0; i < ng; ++i) pn[i] *= inv;
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // Block LOBPCG: LOBPCG::Eigen_d_multi_mpi (lobpcg_d_multi_mpi.h:884-1572) on one process
 // with the BLAS Gram path (MakeMatrix_d_blas :192-250, DIRECT_S_MATRIX_XR_RP, the

@@ -8,6 +8,8 @@
 #include <cstdio>
 #include <vector>
 #include "GridDifference8th.h"
+#include "GramSchmidt.h"
+#include "GramSchmidt_mpi.h"
 
 extern "C" {
 
@@ -23,6 +25,14 @@ void ref_laplacian8_f32(int nx, int ny, int nz, double dx, double dy, double dz,
                         const float* p, float* out) {
     GridRange g{0, 0, 0, nx, ny, nz};
     Laplasian8th<float>(g, out, p, coef, dx, dy, dz);
+}
+
+// GramSchmidt (GramSchmidt.h:6-38): num bands of ng points, band-major, in place, with the
+// reference's own wave-function wrappers referring to the caller's storage.
+void ref_gram_schmidt(int ng, int num, double* psi, double dvol) {
+    std::vector<RspaceFunc<double>> set(num);
+    for (int n = 0; n < num; ++n) set[n].Refer(psi + (size_t)n * ng);
+    GramSchmidt(ng, num, set.data(), dvol);
 }
 
 }

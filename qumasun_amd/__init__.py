@@ -493,6 +493,15 @@ def uneven_cells(ncells=32, seed=SEED_CELLS, n_per_cell=40):
     return out
 
 
+def GramSchmidt(psi, dvol, ng=None):
+    """Orthonormalise the band-major device block psi (nbands x Ng) in place with the dVol
+    inner product, giving the Gram-Schmidt basis (GramSchmidt.h:6-38, GramSchmidt_mpi.h:9-53)."""
+    ng = psi.shape[-1] if ng is None else ng
+    pp, nb, ld = _block(psi, ng)
+    check(lib().qg_gram_schmidt(_dtype_code(psi), ng, nb, pp, ld, float(dvol), _stream()))
+    return psi
+
+
 def random_states(ng, nbands, dvol, seed=SEED_STATES):
     X = np.zeros(ng * nbands)
     check(lib().qg_synth_random_states(ng, nbands, dvol, seed, dptr(X)))

@@ -70,6 +70,7 @@ SIGNATURES = {
     "qg_nonlocal_destroy": (c_int, [c_void_p]),
     "qg_nonlocal_num_coef": (c_int, [c_void_p]),
     "qg_nonlocal_projector_form": (c_int, [c_void_p]),
+    "qg_gram_schmidt": (c_int, [c_int, c_int64, c_int, c_void_p, c_int64, c_double, c_void_p]),
     "qg_nonlocal_project": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, c_void_p, c_void_p]),
     "qg_nonlocal_apply": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, c_void_p, c_int64,
                                   c_int, c_void_p]),

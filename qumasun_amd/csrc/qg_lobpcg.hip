@@ -1030,7 +1030,75 @@ static void run_lobpcg_ortho(qg_ham* h, const qg_lobpcg_opts& o, int N, double* 
 
 using namespace qg;
 
+// y[k*ld_y + i] = x[k*ld_x + i] (convert between precisions, any leading dimensions)
+template <class Ts, class Td>
+__global__ void strided_convert_kernel(const Ts* x, int64_t ld_x, Td* y, int64_t ld_y, int64_t ng) {
+    const int k = blockIdx.y;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < ng; i += (int64_t)gridDim.x * blockDim.x)
+        y[(int64_t)k * ld_y + i] = (Td)x[(int64_t)k * ld_x + i];
+}
+
 extern "C" {
+
+int qg_gram_schmidt(int dtype, int64_t ng, int nbands, void* X, int64_t ld, double dvol, void* stream) {
+    return guarded([&] {
+        QG_REQUIRE(dtype == QG_F64 || dtype == QG_F32, "bad dtype");
+        QG_REQUIRE(ng >= 1 && ng < (1LL << 31) && nbands >= 0 && ld >= ng && dvol > 0.0, "bad sizes");
+        QG_REQUIRE(nbands <= ng, "more bands than grid points");
+        if (nbands == 0) return;
+        QG_REQUIRE(X, "null block");
+        hipStream_t s = as_stream(stream);
+        const int N = nbands;
+        rocblas_handle hb = nullptr;
+        QG_BLAS(rocblas_create_handle(&hb));
+        struct HandleGuard {
+            rocblas_handle h;
+            ~HandleGuard() { rocblas_destroy_handle(h); }
+        } guard{hb};
+        QG_BLAS(rocblas_set_stream(hb, s));
+        // work in FP64 on a contiguous copy unless the block already is one
+        DevBuf<double> tmp;
+        double* Y = (double*)X;
+        const dim3 cg((unsigned)std::min<int64_t>((ng + 255) / 256, 1024), (unsigned)N);
+        if (dtype == QG_F32 || ld != ng) {
+            tmp.alloc((size_t)ng * N);
+            Y = tmp.p;
+            if (dtype == QG_F32)
+                hipLaunchKernelGGL((strided_convert_kernel<float, double>), cg, dim3(256), 0, s, (const float*)X, ld, Y, ng, ng);
+            else
+                hipLaunchKernelGGL((strided_convert_kernel<double, double>), cg, dim3(256), 0, s, (const double*)X, ld, Y, ng, ng);
+            QG_HIP(hipGetLastError());
+        }
+        DevBuf<double> G;
+        DevBuf<int> info;
+        G.alloc((size_t)N * N);
+        info.alloc(1);
+        const double one = 1.0, zero = 0.0;
+        // two Cholesky-QR passes: G = dVol Y^T Y = R^T R, Y <- Y R^-1 (the second pass restores
+        // orthogonality to rounding); R has a positive diagonal, so Y is the Gram-Schmidt basis
+        for (int pass = 0; pass < 2; ++pass) {
+            QG_BLAS(rocblas_dgemm(hb, rocblas_operation_transpose, rocblas_operation_none, N, N, (int)ng, &dvol, Y,
+                                  (int)ng, Y, (int)ng, &zero, G.p, N));
+            QG_BLAS(rocsolver_dpotrf(hb, rocblas_fill_upper, N, G.p, N, info.p));
+            int h_info = 0;
+            QG_HIP(hipMemcpyAsync(&h_info, info.p, sizeof(int), hipMemcpyDeviceToHost, s));
+            QG_HIP(hipStreamSynchronize(s));
+            if (h_info != 0)
+                throw Error{QG_ERR_NUMERIC, "Gram-Schmidt: the bands are linearly dependent (Cholesky info " +
+                                                std::to_string(h_info) + ")"};
+            QG_BLAS(rocblas_dtrsm(hb, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                                  rocblas_diagonal_non_unit, (int)ng, N, &one, G.p, N, Y, (int)ng));
+        }
+        if (Y != (double*)X) {
+            if (dtype == QG_F32)
+                hipLaunchKernelGGL((strided_convert_kernel<double, float>), cg, dim3(256), 0, s, Y, ng, (float*)X, ld, ng);
+            else
+                hipLaunchKernelGGL((strided_convert_kernel<double, double>), cg, dim3(256), 0, s, Y, ng, (double*)X, ld, ng);
+            QG_HIP(hipGetLastError());
+        }
+        QG_HIP(hipStreamSynchronize(s));
+    });
+}
 
 void qg_lobpcg_default_opts(qg_lobpcg_opts* o) {
     if (!o) return;

@@ -221,3 +221,31 @@ def test_lobpcg_foreign_operator(gpu):
     # same iteration; H psi sums the projector term with LDS atomics in its table form, so the
     # two runs agree to rounding, not bit for bit
     assert np.abs(res[0] - res[1]).max() <= 1e-11
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("ng,nb", [(40 ** 3, 16), (12 ** 3, 7), (5000, 1)])
+def test_gram_schmidt_matches_oracle(gpu, oracle, prec, ng, nb):
+    """qg_gram_schmidt (two Cholesky-QR passes) equals the oracle's band-by-band Gram-Schmidt
+    (itself equal bit for bit to the reference's GramSchmidt) on the reference's U[0,1) start."""
+    import torch
+    print(f"gram-schmidt {prec} ng={ng} nb={nb}", flush=True)
+    dvol = 0.1685 ** 3
+    rng = np.random.default_rng(ng + nb)
+    X = rng.uniform(0.0, 1.0, (nb, ng))
+    ref = oracle.gram_schmidt(X, dvol)
+    dt = torch.float64 if prec == "f64" else torch.float32
+    t = torch.from_numpy(X).to(gpu).to(dt).contiguous()
+    Q.GramSchmidt(t, dvol)
+    torch.cuda.synchronize()
+    got = t.double().cpu().numpy()
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    assert err <= (1e-12 if prec == "f64" else 2e-6), err
+    assert np.abs(dvol * got @ got.T - np.eye(nb)).max() <= (1e-12 if prec == "f64" else 1e-5)
+
+
+def test_gram_schmidt_rejects_dependent_bands(gpu):
+    import torch
+    X = torch.ones(3, 1000, dtype=torch.float64, device=gpu)
+    with pytest.raises(Q.QGError):
+        Q.GramSchmidt(X, 0.5)
