@@ -158,7 +158,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const StencilCoef
     const int nlp = a.nlp;
     // in-kernel projector values exist in the projector pass only (the fused pass keeps its
     // registers for the stencil queue)
-    constexpr bool kAnalyticMode = MODE == MODE_NL;
+    constexpr bool kAnalyticMode = MODE == MODE_NL || (MODE == MODE_HAM && NB >= 2);
     const bool analytic = kAnalyticMode && a.has_nl && a.analytic;
     // in-kernel projector values: per image the fractional part of its centre in grid units
     // (3 x T) and integer part + shift, atom slot, projector range (6 ints); per projector
@@ -886,7 +886,9 @@ static FusedSetup<T> setup_fused(const qg_grid& g, int nbands, int64_t ld_in, in
     size_t lds = (MODE == MODE_NL) ? 0 : 2 * (size_t)p.nb * (p.ty + 8) * (g.n[0] + 8) * sizeof(T);
     int njs = 4;
     if (nl && nl->geo.ncoef > 0 && MODE != MODE_LAP) {
-        bool seg = segments_form(nl, dtype, MODE);
+        // QG_NL_ANALYTIC=3: also the fused pass when it carries 2 bands per workgroup
+        bool seg = segments_form(nl, dtype, MODE) ||
+                   (MODE == MODE_HAM && p.nb >= 2 && nl->analytic && nl->analytic_level >= 3);
         if (seg) {
             // the in-kernel form stages the slab's images and segments; tables when they do not
             // fit in LDS or the segment encoding
