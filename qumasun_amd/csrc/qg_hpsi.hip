@@ -158,7 +158,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, const StencilCoef
     const int nlp = a.nlp;
     // in-kernel projector values exist in the projector pass only (the fused pass keeps its
     // registers for the stencil queue)
-    constexpr bool kAnalyticMode = MODE == MODE_NL || (MODE == MODE_HAM && NB >= 2);
+    constexpr bool kAnalyticMode = MODE == MODE_NL;
     const bool analytic = kAnalyticMode && a.has_nl && a.analytic;
     // in-kernel projector values: per image the fractional part of its centre in grid units
     // (3 x T) and integer part + shift, atom slot, projector range (6 ints); per projector
@@ -886,9 +886,7 @@ static FusedSetup<T> setup_fused(const qg_grid& g, int nbands, int64_t ld_in, in
     size_t lds = (MODE == MODE_NL) ? 0 : 2 * (size_t)p.nb * (p.ty + 8) * (g.n[0] + 8) * sizeof(T);
     int njs = 4;
     if (nl && nl->geo.ncoef > 0 && MODE != MODE_LAP) {
-        // QG_NL_ANALYTIC=3: also the fused pass when it carries 2 bands per workgroup
-        bool seg = segments_form(nl, dtype, MODE) ||
-                   (MODE == MODE_HAM && p.nb >= 2 && nl->analytic && nl->analytic_level >= 3);
+        bool seg = segments_form(nl, dtype, MODE);
         if (seg) {
             // the in-kernel form stages the slab's images and segments; tables when they do not
             // fit in LDS or the segment encoding
@@ -1116,6 +1114,34 @@ static void launch_fused_batch(int ppt, int nthr, size_t lds, int total, const B
     QG_HIP(hipGetLastError());
 }
 
+// Batched projector passes (split form): NB bands per workgroup and PPT points per thread are
+// shared by the cells of one launch; the host groups cells by them.
+template <class T, int NJS>
+static void launch_nl_batch(int nb, int ppt, int nthr, size_t lds, int total, const BatchCell<T>* cells,
+                            const int* prefix, int ncells, hipStream_t s) {
+    const int grid = std::max(1, std::min(total, 2048));
+#define QG_NLB(NBV, PP)                                                                                     \
+    hipLaunchKernelGGL((fused_batch_kernel<T, NBV, PP, MODE_NL, NJS>), dim3(grid), dim3(nthr), lds, s, cells, \
+                       prefix, ncells)
+    if (ppt == 2) {
+        switch (nb) {
+            case 8: QG_NLB(8, 2); break;
+            case 4: QG_NLB(4, 2); break;
+            case 2: QG_NLB(2, 2); break;
+            default: QG_NLB(1, 2); break;
+        }
+    } else {
+        switch (nb) {
+            case 8: QG_NLB(8, 4); break;
+            case 4: QG_NLB(4, 4); break;
+            case 2: QG_NLB(2, 4); break;
+            default: QG_NLB(1, 4); break;
+        }
+    }
+#undef QG_NLB
+    QG_HIP(hipGetLastError());
+}
+
 template <class T>
 static void batch_apply_t(const std::vector<BatchReq>& reqs, hipStream_t s, DevBuf<unsigned char>& scratch) {
     const int dtype = sizeof(T) == 4 ? QG_F32 : QG_F64;
@@ -1123,10 +1149,22 @@ static void batch_apply_t(const std::vector<BatchReq>& reqs, hipStream_t s, DevB
     std::vector<int> cb(n);
     int rounds = 0;
     int njs = 0;
+    // split form (stencil + V launch, then projector-pass launches) when every cell with
+    // projectors would use it on its own
+    const int split_env = env_int("QG_SPLIT_NL", -1);
+    bool split = false, all_split = true;
+    for (int c = 0; c < n; ++c) {
+        qg_nonlocal* nl = (reqs[c].h->nl && reqs[c].h->nl->geo.ncoef > 0) ? reqs[c].h->nl : nullptr;
+        if (!nl) continue;
+        const bool sc = split_env >= 0 ? split_env != 0 : segments_form(nl, dtype, MODE_NL);
+        split = split || sc;
+        all_split = all_split && sc;
+    }
+    split = split && all_split;
     for (int c = 0; c < n; ++c) {
         const BatchReq& r = reqs[c];
         qg_nonlocal* nl = (r.h->nl && r.h->nl->geo.ncoef > 0) ? r.h->nl : nullptr;
-        cb[c] = nl ? chunk_bands(r.h->grid, dtype, r.nbands, false) : std::max(1, r.nbands);
+        cb[c] = nl ? chunk_bands(r.h->grid, dtype, r.nbands, split) : std::max(1, r.nbands);
         if (nl) {
             nl->d_coef.alloc((size_t)std::min(cb[c], r.nbands) * nl->geo.ncoef);
             QG_REQUIRE(njs == 0 || njs == nl->geo.njs, "cells of one batch must share the projector slot count");
@@ -1143,6 +1181,12 @@ static void batch_apply_t(const std::vector<BatchReq>& reqs, hipStream_t s, DevB
         size_t lds;
     };
     std::vector<RoundDesc> rd(rounds);
+    struct NlGroupDesc {
+        size_t fc, fp;
+        int nb, ppt, ncells, total, nthr;
+        size_t lds;
+    };
+    std::vector<std::vector<NlGroupDesc>> nlg(rounds);
     std::vector<unsigned char> host;
     auto push = [&](const void* src, size_t bytes) {
         const size_t off = (host.size() + 255) & ~size_t(255);
@@ -1157,6 +1201,13 @@ static void batch_apply_t(const std::vector<BatchReq>& reqs, hipStream_t s, DevB
         std::vector<int> fpre{0};
         int ppt = -1, nthr = 0;
         size_t lds = 0;
+        struct NlGroup {
+            std::vector<BatchCell<T>> cells;
+            std::vector<int> pre{0};
+            int nthr = 0;
+            size_t lds = 0;
+        };
+        std::map<std::pair<int, int>, NlGroup> groups;
         for (int c = 0; c < n; ++c) {
             const BatchReq& q = reqs[c];
             const int b0 = r * cb[c];
@@ -1183,8 +1234,21 @@ static void batch_apply_t(const std::vector<BatchReq>& reqs, hipStream_t s, DevB
                 pcs.push_back(ProjCell<T>{a, pin});
                 ppre.push_back(ppre.back() + a.natoms * ((nb + NBA - 1) / NBA));
             }
-            const FusedSetup<T> fs = setup_fused<T, MODE_HAM>(g, nb, ng, ng, -0.5, 0, nl, nl ? nl->d_coef.p : nullptr);
+            qg_nonlocal* fnl = split ? nullptr : nl;
+            const FusedSetup<T> fs = setup_fused<T, MODE_HAM>(g, nb, ng, ng, -0.5, 0, fnl, fnl ? fnl->d_coef.p : nullptr);
             QG_REQUIRE(fs.p.nb == 1, "batched cells use one band per workgroup");
+            if (split && nl) {
+                const FusedSetup<T> fn = setup_fused<T, MODE_NL>(g, nb, ng, ng, 0.0, 1, nl, nl->d_coef.p);
+                QG_REQUIRE(fn.p.nb == 1 || fn.p.nb == 2 || fn.p.nb == 4 || fn.p.nb == 8,
+                           "batched projector pass: unsupported bands per workgroup");
+                QG_REQUIRE(fn.p.ppt == 2 || fn.p.ppt == 4, "batched projector pass: unsupported points per thread");
+                NlGroup& gr = groups[{fn.p.nb, fn.p.ppt}];
+                gr.cells.push_back(BatchCell<T>{fn.a, fn.sc, nullptr, pin, pout});
+                QG_REQUIRE((int64_t)gr.pre.back() + fn.nwg < (1LL << 31), "batch too large");
+                gr.pre.push_back(gr.pre.back() + (int)fn.nwg);
+                gr.nthr = std::max(gr.nthr, fn.p.nthr);
+                gr.lds = std::max(gr.lds, fn.lds);
+            }
             QG_REQUIRE(ppt < 0 || ppt == fs.p.ppt, "cells of one batch must share the points-per-thread plan");
             ppt = fs.p.ppt;
             nthr = std::max(nthr, fs.p.nthr);
@@ -1206,6 +1270,19 @@ static void batch_apply_t(const std::vector<BatchReq>& reqs, hipStream_t s, DevB
         d.pp = push(ppre.data(), ppre.size() * sizeof(int));
         d.fc = push(fcs.data(), fcs.size() * sizeof(BatchCell<T>));
         d.fp = push(fpre.data(), fpre.size() * sizeof(int));
+        for (auto& kv : groups) {
+            NlGroup& gr = kv.second;
+            NlGroupDesc gd{};
+            gd.nb = kv.first.first;
+            gd.ppt = kv.first.second;
+            gd.ncells = (int)gr.cells.size();
+            gd.total = gr.pre.back();
+            gd.nthr = gr.nthr;
+            gd.lds = gr.lds;
+            gd.fc = push(gr.cells.data(), gr.cells.size() * sizeof(BatchCell<T>));
+            gd.fp = push(gr.pre.data(), gr.pre.size() * sizeof(int));
+            nlg[r].push_back(gd);
+        }
     }
     scratch.alloc(host.size());
     QG_HIP(hipMemcpyAsync(scratch.p, host.data(), host.size(), hipMemcpyHostToDevice, s));
@@ -1229,6 +1306,15 @@ static void batch_apply_t(const std::vector<BatchReq>& reqs, hipStream_t s, DevB
             else
                 launch_fused_batch<T, 8>(d.ppt, d.nthr, d.lds, d.fused_total, (const BatchCell<T>*)(b + d.fc),
                                          (const int*)(b + d.fp), d.nfused, s);
+        }
+        for (const NlGroupDesc& gd : nlg[r]) {
+            if (gd.total <= 0) continue;
+            if (njs == 4)
+                launch_nl_batch<T, 4>(gd.nb, gd.ppt, gd.nthr, gd.lds, gd.total, (const BatchCell<T>*)(b + gd.fc),
+                                      (const int*)(b + gd.fp), gd.ncells, s);
+            else
+                launch_nl_batch<T, 8>(gd.nb, gd.ppt, gd.nthr, gd.lds, gd.total, (const BatchCell<T>*)(b + gd.fc),
+                                      (const int*)(b + gd.fp), gd.ncells, s);
         }
     }
     // the host descriptors must outlive the upload
