@@ -51,6 +51,9 @@ def parse():
                     help="also time LOBPCG to 1e-6 Ha: plain FP64 vs FP32 + learned preconditioner")
     ap.add_argument("--eigen-budget", type=float, default=600.0,
                     help="seconds after which the plain FP64 eigen runs stop (reported as unconverged)")
+    ap.add_argument("--lobpcg-iters", type=int, default=0,
+                    help="only run this many LOBPCG iterations per precision (FP64, FP32, mixed) on the --eigen "
+                         "workload and print the per-sector times: the command the solver's GEMM profile is taken on")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device copies of 256 MiB (FP32), the known-size kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -217,6 +220,47 @@ def eigen_leg(cfg, budget):
             "reference_fp64_kinetic": {"seconds": round(t_ref, 3), "iterations": it_ref}}
 
 
+def lobpcg_iterations(cfg, iters):
+    """A few orthonormal-basis LOBPCG iterations (kinetic preconditioner) per precision, from the
+    seeded start, with the solver's per-sector times; the GEMM work per iteration is stated so a
+    kernel trace of this run gives the Gram / update GEMM rate."""
+    import torch
+    import qumasun_amd as Q
+    from qumasun_amd import _native as N
+    ncell, nb = (1, 16) if cfg == "cfg0" else (3, 512)
+    s = Q.diamond_system(ncell, 40)
+    n, h = s["n"], s["h"]
+    g = Q.Grid(n, n, n, h, h, h)
+    integ = Q.PseudoPotIntegrator(g, s["projectors"])
+    integ.UpdatePosition(s["positions"])
+    H = Q.Hamiltonian(g, s["V"], integ)
+    X0 = Q.random_states(g.size, nb, g.dvol)
+    K = g.size
+    gemm = 2.0 * nb * nb * K   # flops of one N x N x Ng product
+    out = {"workload": f"{len(s['positions'])} atoms, {n}^3 grid, {nb} bands", "iterations": iters,
+           "flops_per_NxNxNg_product": gemm}
+    for name, prec in (("f64", N.QG_PREC_F64), ("f32", N.QG_PREC_F32), ("mixed", N.QG_PREC_MIXED)):
+        X = torch.from_numpy(X0.copy()).cuda()
+        P = torch.zeros_like(X)
+        # one warm-up iteration (library handles, workspaces), then the timed ones from the same start
+        Q.Eigen_d_multi(H, X, P, 0, max_iter=1, precision=prec, precond=N.QG_PRECOND_KINETIC, precond_shift=0.5,
+                        basis=N.QG_BASIS_ORTHO, switch_tol=0.0)
+        X = torch.from_numpy(X0.copy()).cuda()
+        P = torch.zeros_like(X)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e, st = Q.Eigen_d_multi(H, X, P, 0, max_iter=iters, precision=prec, precond=N.QG_PRECOND_KINETIC,
+                                precond_shift=0.5, basis=N.QG_BASIS_ORTHO, switch_tol=0.0)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        it = max(1, st["iterations"])
+        out[name] = {"seconds": round(el, 3), "iterations": st["iterations"],
+                     "per_iteration_ms": {k[2:]: round(st[k] / it * 1e3, 2)
+                                          for k in ("t_gram", "t_update", "t_eig", "t_hpsi", "t_precond")}}
+        print(f"lobpcg {name}: {st['iterations']} iterations in {el:.2f} s", file=sys.stderr, flush=True)
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -225,6 +269,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.lobpcg_iters > 0:
+        lobpcg_iterations(args.eigen if args.eigen != "none" else "cfg2", args.lobpcg_iters)
         return
     import torch
     import qumasun_amd as Q
