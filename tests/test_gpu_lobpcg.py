@@ -152,7 +152,8 @@ def _solve(H, X0, **kw):
 
 def test_lobpcg_dense_small(gpu, oracle):
     """12^3 cell with overlapping periodic images: converged eigenvalues equal the lowest
-    eigenvalues of the dense oracle Hamiltonian (numpy eigh)."""
+    eigenvalues of the dense oracle Hamiltonian (numpy eigh), and the converged block spans the
+    same subspace as the dense eigenvectors (equal after a rotation inside the block)."""
     print("lobpcg dense small", flush=True)
     s, g, integ, H = _cell(1, 12)
     n = (s["n"],) * 3
@@ -161,13 +162,21 @@ def test_lobpcg_dense_small(gpu, oracle):
     onl = oracle.OracleNL(n, h, s["positions"], np.zeros(len(s["positions"]), np.int32), s["projectors"])
     Hd = oracle.hamiltonian_block(n, h, s["V"], onl, np.eye(ng)).T
     assert np.abs(Hd - Hd.T).max() < 1e-10
-    w = np.linalg.eigvalsh(0.5 * (Hd + Hd.T))
+    w, V = np.linalg.eigh(0.5 * (Hd + Hd.T))
     # a block that does not cut through a (symmetry-)degenerate multiplet
     nb = next(k for k in range(4, 12) if w[k] - w[k - 1] > 1e-3)
-    e, st, _ = _solve(H, Q.random_states(ng, nb, g.dvol), precision=N.QG_PREC_F64, max_iter=300, tol=1e-8,
+    e, st, X = _solve(H, Q.random_states(ng, nb, g.dvol), precision=N.QG_PREC_F64, max_iter=300, tol=1e-8,
                       precond=N.QG_PRECOND_KINETIC, precond_shift=1.0)
     assert st["max_residual"] <= 1e-8, st
     assert np.abs(e - w[:nb]).max() < 1e-9, (e, w[:nb])
+    # subspace: with sqrt(dVol) X orthonormal, the projector onto its span equals V V^T of the
+    # lowest nb dense eigenvectors (error ~ residual / gap)
+    Xs = X.cpu().numpy() * np.sqrt(g.dvol)
+    assert np.abs(Xs @ Xs.T - np.eye(nb)).max() < 1e-10
+    Vb = V[:, :nb]
+    # sines of the principal angles between the two subspaces
+    sv = np.linalg.svd(Xs @ Vb, compute_uv=False)
+    assert np.sqrt(max(0.0, 1.0 - sv.min() ** 2)) < 1e-6, sv
 
 
 def test_lobpcg_cfg0_f64_fixture(gpu):
