@@ -188,3 +188,39 @@ def test_hamiltonian_projector_paths(gpu, oracle, monkeypatch, prec, analytic, s
     loc = oracle.hamiltonian_block(n, h, V, None, X)
     errnl = rel(_np(out) - loc, ref - loc)
     assert errnl <= (1e-11 if prec == "f64" else 2e-5), errnl
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("nb", [1, 3, 13])
+def test_hamiltonian_ragged_padded(gpu, oracle, prec, nb):
+    """Band counts that are not multiples of the per-workgroup band groups (1, 3, 13), on blocks
+    whose leading dimension is padded (ld = Ng + 37): H psi equals the oracle and the padding
+    of the output is left untouched."""
+    import ctypes
+    import torch
+    print(f"ragged/padded prec={prec} nb={nb}", flush=True)
+    s = _system(1, 40)
+    n = (s["n"],) * 3
+    h = (s["h"],) * 3
+    ng = n[0] ** 3
+    ld = ng + 37
+    nat = len(s["positions"])
+    onl = oracle.OracleNL(n, h, s["positions"], np.zeros(nat, np.int32), s["projectors"])
+    X = Q.random_states(ng, nb, h[0] ** 3, seed=99 + nb)
+    ref = oracle.hamiltonian_block(n, h, s["V"], onl, X)
+    g = Q.Grid(*n, *h)
+    integ = Q.PseudoPotIntegrator(g, s["projectors"])
+    integ.UpdatePosition(s["positions"])
+    H = Q.Hamiltonian(g, s["V"], integ)
+    dt = torch.float64 if prec == "f64" else torch.float32
+    xin = torch.zeros(nb, ld, dtype=dt, device=gpu)
+    xin[:, :ng] = _t(X, dt, gpu)
+    out = torch.full((nb, ld), 7.0, dtype=dt, device=gpu)
+    code = Q.QG_F64 if prec == "f64" else Q.QG_F32
+    Q.check(Q.lib().qg_hamiltonian_apply(H._h, code, nb, ctypes.c_void_p(xin.data_ptr()), ld,
+                                         ctypes.c_void_p(out.data_ptr()), ld, ctypes.c_void_p(0)))
+    torch.cuda.synchronize()
+    got = out[:, :ng].double().cpu().numpy()
+    err = rel(got, ref)
+    assert err <= (TOL64 if prec == "f64" else TOL32), err
+    assert bool((out[:, ng:] == 7.0).all())
