@@ -175,6 +175,8 @@ typedef struct qg_lobpcg_opts {
     void (*op)(void* user, int dtype, int nbands, const void* x_dev, void* ax_dev, void* stream);
     void* op_user;
     int basis;               /* QG_BASIS_*: used for FP64 runs without a preconditioner */
+    double fallback_shift;   /* QG_PRECOND_LEARNED: shift of the kinetic preconditioner that
+                                takes over after a stall (0: precond_shift) */
 } qg_lobpcg_opts;
 
 typedef struct qg_lobpcg_stats {

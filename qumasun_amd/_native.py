@@ -45,7 +45,8 @@ class LobpcgOpts(ctypes.Structure):
     _fields_ = [("precision", c_int), ("max_iter", c_int), ("tol", c_double),
                 ("refresh_period", c_int), ("rollback_limit", c_double), ("precond", c_int),
                 ("precond_shift", c_double), ("verbose", c_int), ("switch_tol", c_double),
-                ("stall_window", c_int), ("op", OP_CALLBACK), ("op_user", c_void_p), ("basis", c_int)]
+                ("stall_window", c_int), ("op", OP_CALLBACK), ("op_user", c_void_p), ("basis", c_int),
+                ("fallback_shift", c_double)]
 
 
 class LobpcgStats(ctypes.Structure):

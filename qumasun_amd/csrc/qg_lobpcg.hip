@@ -240,6 +240,11 @@ struct LobpcgState {
 
 void lobpcg_state_free(LobpcgState* s) { delete s; }
 
+// shift of the kinetic preconditioner: the run's own, or the fallback of a learned run
+static double kinetic_shift(const qg_lobpcg_opts& o) {
+    return (o.precond == QG_PRECOND_LEARNED && o.fallback_shift > 0.0) ? o.fallback_shift : o.precond_shift;
+}
+
 template <class T>
 struct Ops {
     LobpcgState& st;
@@ -476,7 +481,7 @@ static void run_lobpcg(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_ho
             if (learned)
                 cnn_precond_apply(h->cnn, dtype, N, R, K, s);
             else
-                precond_apply(h, QG_PRECOND_KINETIC, o.precond_shift, dtype, N, R, K, s);
+                precond_apply(h, QG_PRECOND_KINETIC, kinetic_shift(o), dtype, N, R, K, s);
             sync();
             stats.t_precond += secs(t0, now());
         }
@@ -906,7 +911,7 @@ static void run_lobpcg_ortho(qg_ham* h, const qg_lobpcg_opts& o, int N, double* 
         auto t0 = now();
         if (o.precond != QG_PRECOND_NONE) {
             if (learned) cnn_precond_apply(h->cnn, dtype, N, W, K, s);
-            else precond_apply(h, QG_PRECOND_KINETIC, o.precond_shift, dtype, N, W, K, s);
+            else precond_apply(h, QG_PRECOND_KINETIC, kinetic_shift(o), dtype, N, W, K, s);
             sync();
             stats.t_precond += secs(t0, now());
         }
@@ -1115,6 +1120,7 @@ void qg_lobpcg_default_opts(qg_lobpcg_opts* o) {
     o->op = nullptr;
     o->op_user = nullptr;
     o->basis = QG_BASIS_REFERENCE;
+    o->fallback_shift = 0.0;
 }
 
 int qg_lobpcg_d_multi(qg_ham* h, const qg_lobpcg_opts* opts, int nbands, double* eig, double* X, double* P,
