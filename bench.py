@@ -191,7 +191,7 @@ def eigen_leg(cfg, budget):
     net.attach()
     tols = (1e-5, 1e-6) if cfg == "cfg0" else (2e-3, 1e-3, 3e-4, 1e-4)
     t_l, it_l, ok_l = ladder("fp32-learned", ref, tols, precision=N.QG_PREC_F32, precond=N.QG_PRECOND_LEARNED,
-                             precond_shift=2.0, fallback_shift=0.5, switch_tol=1e-3)
+                             precond_shift=2.0, switch_tol=1e-3)
     t_k, it_k, ok_k = ladder("fp32-kinetic", ref, tols, precision=N.QG_PREC_F32, precond=N.QG_PRECOND_KINETIC,
                              precond_shift=0.5, switch_tol=1e-3)
     net.detach()
