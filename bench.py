@@ -5,20 +5,24 @@ H·psi workload (the configuration the H·psi bandwidth metric is quoted on): th
 2x2x2 diamond supercell, L = 13.48 bohr, periodic 80^3 grid, Vloc = sum -4 erf(r/0.8)/r +
 U(+-0.05), Gaussian s+p KB projectors (sigma 0.6, cutoff 2.4, E = +1.0 / -0.5), and a block of
 128 seeded random real bands in FP32.  A step is one application of H to the whole 128-band
-block with inputs resident in HBM: per band chunk of about 128 MB, the projector-dot kernel and
-the fused stencil + V + projector-apply kernel.
+block with inputs resident in HBM: per band chunk, the projector-dot kernel, then (FP32 default)
+a stencil + V launch and a projector-term launch with in-kernel projector values, or one fused
+stencil + V + projector launch reading per-point tables.
 
 value     = algorithmic bytes of a step / step time, algorithmic bytes = 2*B*Ng*s + Ng*s (psi
             read once, H psi written once, V read once; the projector tables are reported
             separately as table_bytes).
-roofline  = the fused kernel: algorithmic bytes of one launch (one band chunk) / its average
-            duration (hipEvents around each launch, on the launch stream); traffic = HBM bytes
-            per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this
+roofline  = the H psi launches of one band chunk: its algorithmic bytes / their average
+            duration (hipEvents around them, on the launch stream); traffic = HBM bytes per
+            chunk from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this
             workload, scaled by the copy-kernel calibration factor recorded there.
 cpu_baseline = the FP64 oracle (a C++ restatement of the reference's CPU path) on 1 host
             thread, one pass over a bounded sample of the same block.
---eigen cfg0|cfg2: also time the LOBPCG solve to eigenvalues within 1e-6 Ha of the FP64
-            reference: plain FP64 (the reference's algorithm) vs FP32 + learned preconditioner.
+--eigen cfg0|cfg2|none (default cfg0, 8 atoms / 40^3 / 16 bands, ~15 s): also time the
+            LOBPCG solve to eigenvalues within 1e-6 Ha of the FP64 reference: plain FP64 (the
+            reference's algorithm, and on the orthonormal basis) vs FP32 + learned
+            preconditioner.  cfg2 (216 atoms / 120^3 / 512 bands) takes ~15 minutes with
+            --eigen-budget 650.
 """
 import argparse
 import json
@@ -47,7 +51,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-bands", type=int, default=128,
                     help="bands of the CPU baseline sample (FP64 oracle, 1 thread)")
-    ap.add_argument("--eigen", choices=["none", "cfg0", "cfg2"], default="none",
+    ap.add_argument("--eigen", choices=["none", "cfg0", "cfg2"], default="cfg0",
                     help="also time LOBPCG to 1e-6 Ha: plain FP64 vs FP32 + learned preconditioner")
     ap.add_argument("--eigen-budget", type=float, default=600.0,
                     help="seconds after which the plain FP64 eigen runs stop (reported as unconverged)")
@@ -362,7 +366,10 @@ def main():
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(sysd, X, min(args.cpu_bands, args.bands))
     if args.eigen != "none":
-        res["eigen"] = eigen_leg(args.eigen, args.eigen_budget)
+        try:
+            res["eigen"] = eigen_leg(args.eigen, args.eigen_budget)
+        except Exception as exc:   # the H psi line stands on its own
+            res["eigen"] = {"error": f"{type(exc).__name__}: {exc}"}
     print(json.dumps(res))
 
 
