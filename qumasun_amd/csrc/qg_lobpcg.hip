@@ -851,7 +851,9 @@ static void run_lobpcg_ortho(qg_ham* h, const qg_lobpcg_opts& o, int N, double* 
         return tmax > 0.0 ? tmin / tmax : 1.0;
     };
     // second pass when the first SVQB amplified by more than ~30 (FP64), always in FP32
-    const double reortho = sizeof(T) == 8 ? 1e-3 : 2.0;
+    // (QG_REORTHO32_MILLI = 1000 x the FP32 threshold overrides, for measurements)
+    const int r32 = env_int("QG_REORTHO32_MILLI", 0);
+    const double reortho = sizeof(T) == 8 ? 1e-3 : (r32 > 0 ? r32 * 1e-3 : 2.0);
     // Rayleigh-Ritz on the (orthonormal) columns of V with AV: V, AV rotated to Ritz vectors
     auto rr_single = [&](T*& V, T* AV) {
         op.gram(V, AV, st.A3.p, N);
