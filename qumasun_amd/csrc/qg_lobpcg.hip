@@ -294,12 +294,15 @@ struct Ops {
         hipLaunchKernelGGL((split_reduce_kernel<T>), dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, W,
                            (int)(nch + (rem ? 1 : 0)), N, nb, C, ldc);
     }
-    // K chunks of the split-K Gram: enough strided-batched work for small N (measured at
-    // 120^3 / 512 bands: 64 chunks beat 16-32 and 256 in both precisions); QG_GRAM_NCH overrides
+    // K chunks of the split-K Gram (each chunk summed by the library in the block precision,
+    // the chunks in FP64). FP32: chunks of at most ~6,750 terms (256 at 120^3) keep the Gram
+    // rounding low enough for the FP32 phase to converge to ~2e-3 at 512 bands; 64 chunks of
+    // 27,000 terms were 9% faster but stalled it near 7e-2. FP64: 64 chunks (the fastest
+    // measured). QG_GRAM_NCH overrides.
     static int64_t split_chunks(int64_t K, int N) {
         const int64_t want = env_int("QG_GRAM_NCH", 0);
         if (want > 0) return std::max<int64_t>(1, std::min<int64_t>(want, K / 64));
-        return std::max<int64_t>(1, std::min<int64_t>(64, K / 2048));
+        return std::max<int64_t>(1, std::min<int64_t>(sizeof(T) == 4 ? 256 : 64, K / 2048));
     }
     static size_t split_workspace(int64_t K, int N, int nb_max = 0) {
         const int64_t nch = split_chunks(K, N);
