@@ -269,7 +269,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(this build runs the whole grid on a single MI355X)", file=sys.stderr)
+              "(the engine runs the whole grid on a single MI355X)", file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration()

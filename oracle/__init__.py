@@ -1,7 +1,7 @@
 """CPU FP64 oracle of QUMASUN's H·psi + LOBPCG path — TEST INFRASTRUCTURE ONLY.
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import this
-package, and only as the checker (or as the timed CPU baseline).  The product package
+package, and only as the oracle the tests compare with (or as the timed CPU baseline).  The product package
 qumasun_amd never imports it.
 """
 from .binding import *  # noqa: F401,F403

@@ -1,6 +1,6 @@
 // oracle.cpp — CPU FP64 restatement of QUMASUN's Kohn-Sham H·psi + block-LOBPCG path.
 //
-// TEST INFRASTRUCTURE ONLY.  This file is the checker for the MI355X engine: only tests/,
+// TEST INFRASTRUCTURE ONLY.  This file is the oracle for the MI355X engine: only tests/,
 // __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The product
 // library (qumasun_amd/) never links, calls or falls back to it.
 //

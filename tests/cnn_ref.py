@@ -1,5 +1,5 @@
 """CPU (torch float32/float64) restatement of the learned preconditioner's network, used by
-the tests as the checker of the HIP kernels (qumasun_amd/csrc/qg_cnn.hip).
+the tests as the CPU reference of the HIP kernels (qumasun_amd/csrc/qg_cnn.hip).
 
 Parameter vector layout (2408 floats): W1 (8,2,27) | W2 (8,8,27) | W3 (1,8,27) | A1 B1 A2 B2 (8
 each); tap t = 9 dz + 3 dy + dx is the offset (dz-1, dy-1, dx-1) in (z, y, x); grid arrays

@@ -1,6 +1,6 @@
 """Writes cfg0_eigenvalues.json: the 16 lowest eigenvalues of the oracle Hamiltonian of the
 8-atom diamond cell (L = 6.74 bohr, periodic 40^3 grid, Vloc = sum -4 erf(r/0.8)/r + U(+-0.05),
-Gaussian s + p KB projectors).  Solved independently of this build's LOBPCG: scipy's
+Gaussian s + p KB projectors).  Solved independently of the engine's LOBPCG: scipy's
 lobpcg on a LinearOperator wrapping the FP64 oracle H, with a numpy FFT preconditioner,
 20 vectors, converged to 1e-10, cross-checked by a second run from another start."""
 import json

@@ -1,4 +1,4 @@
-"""CPU pins of the learned preconditioner's checker (tests/cnn_ref.py): its convolution equals
+"""CPU pins of the learned preconditioner's CPU reference (tests/cnn_ref.py): its convolution equals
 a brute-force periodic sum on a tiny grid, its adjoint satisfies <N a, b> = <a, N^T b>, and
 its kinetic operator matches the 8th-order stencil's plane-wave symbol."""
 import numpy as np

@@ -1,7 +1,7 @@
 """GPU tests of the learned preconditioner (qumasun_amd/csrc/qg_cnn.hip).
 
-Checker: tests/cnn_ref.py (torch CPU restatement, pinned by test_cnn_ref.py).  Covered: the
-network and its adjoint against the checker; the BF16 inference weights; the device loss and
+CPU reference: tests/cnn_ref.py (torch CPU restatement, pinned by test_cnn_ref.py).  Covered: the
+network and its adjoint against that reference; the BF16 inference weights; the device loss and
 gradient against torch autograd on the CPU (the training gradient is the project's own
 forward/backward kernels, not autograd); M = K + K N^T N K symmetric positive definite; the
 device training loop lowering the loss; LOBPCG with the learned preconditioner converging to
@@ -91,7 +91,7 @@ def test_bf16_weights(gpu):
 
 def test_mfma_network_matches_bf16_emulation(gpu):
     """The matrix-core inference kernels (BF16 weights and activations, FP32 accumulation)
-    equal the checker run with the same BF16 roundings."""
+    equal the CPU reference run with the same BF16 roundings."""
     import torch
     print("cnn mfma network", flush=True)
     for dims in ((20, 16, 12), (40, 36, 8), (13, 9, 7)):
@@ -116,7 +116,7 @@ def test_mfma_network_matches_bf16_emulation(gpu):
 
 def test_loss_gradient_matches_autograd(gpu):
     """The device gradient (two forward and two backward passes of the project's kernels)
-    equals torch autograd of the CPU checker's loss."""
+    equals torch autograd of the CPU reference's loss."""
     import torch
     print("cnn gradient vs autograd", flush=True)
     g, H, V, h = _ham()
