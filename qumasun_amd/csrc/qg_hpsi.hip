@@ -1024,6 +1024,8 @@ static int chunk_bands(const qg_grid& g, int dtype, int nbands, bool split) {
     // only bound the coefficient scratch
     const int64_t budget = (int64_t)env_int("QG_CHUNK_MB", split ? 512 : 128) << 20;
     int64_t c = std::max<int64_t>(1, budget / std::max<int64_t>(1, bytes));
+    // whole groups of the projection's 8 bands per workgroup (9-band chunks left a 1-band group)
+    if (c > 8) c = c / 8 * 8;
     return (int)std::min<int64_t>(nbands, c);
 }
 
