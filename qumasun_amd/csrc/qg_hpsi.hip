@@ -1032,6 +1032,9 @@ static int chunk_bands(const qg_grid& g, int dtype, int nbands, bool split) {
 // H psi for a band block, used by qg_hamiltonian_apply and the LOBPCG driver.
 void hamiltonian_apply(qg_ham* h, int dtype, int nbands, const void* psi, int64_t ld, void* hpsi,
                        int64_t ld_out, hipStream_t s) {
+    // the row kernels (one projection pass, one stencil + V + projector pass) where they
+    // serve the grid and projector set; the table kernels below otherwise
+    if (rows_hamiltonian_apply(h, dtype, nbands, psi, ld, hpsi, ld_out, s, nullptr)) return;
     const qg_grid& g = h->grid;
     qg_nonlocal* nl = (h->nl && h->nl->geo.ncoef > 0) ? h->nl : nullptr;
     // split form (stencil + V pass, then a projector pass over many bands at a time) for FP32
@@ -1423,6 +1426,7 @@ int qg_laplacian8_apply(const qg_grid* g, int dtype, int nbands, const void* psi
         check_block(g, dtype, nbands, psi, ld_psi, out, ld_out);
         if (nbands == 0) return;
         hipStream_t s = as_stream(stream);
+        if (rows_laplacian_apply(*g, dtype, nbands, psi, ld_psi, out, ld_out, coef, accumulate, s)) return;
         if (dtype == QG_F64)
             launch_fused<double, MODE_LAP>(*g, nbands, (const double*)psi, ld_psi, (double*)out, ld_out, coef,
                                            accumulate, nullptr, nullptr, nullptr, s);
@@ -1490,6 +1494,7 @@ int qg_nonlocal_project(qg_nonlocal* nl, int dtype, int nbands, const void* psi,
         if (nbands == 0 || nl->geo.ncoef == 0) return;
         QG_REQUIRE(psi && coef, "null pointer");
         hipStream_t s = as_stream(stream);
+        if (rows_nonlocal_project(nl, dtype, nbands, psi, ld, coef, nl->rows_part, s)) return;
         if (dtype == QG_F64)
             launch_project<double>(nl, nbands, (const double*)psi, ld, coef, s);
         else

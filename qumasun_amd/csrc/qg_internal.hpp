@@ -6,6 +6,7 @@
 #include <string>
 #include <vector>
 #include <cstdlib>
+#include <memory>
 #include "../../include/qumasun_gpu.h"
 
 namespace qg {
@@ -164,6 +165,33 @@ struct TileNL {
 };
 void build_tile_nl(const qg_grid& g, const NlGeometry& geo, int ty, int tz, bool segments, TileNL& out);
 
+// Geometry of the row kernels (qg_rows.hip) for y-tiles of ty rows and projection slabs of
+// tzp planes: the periodic sphere images of every tile column, the sphere points of every
+// (tile, plane) as items (row | x start << 5 | (count - 1) << 15 | column image << 17) grouped
+// by image, and the projection's per-thread image ownership with its partial slots.
+struct RowsGeo {
+    int ty = 0, tzp = 0, ntile = 0, nslabp = 0;
+    int max_col = 0;     // most images of one tile column
+    int nslot = 0;       // projection partial slots (one per active (tile, slab, image))
+    int njs = 4;         // coefficient slots per atom (4 or 8)
+    bool sp4 = false;    // every image: one s channel (slot 0) and one p channel (slots 1..3)
+    bool ok = false;     // the geometry fits the encodings
+    std::vector<int> col_off, col_img, it_off, pl_base, pl_off, thr_slot, atom_slot_off, atom_slot;
+    std::vector<uint32_t> items, thr_map;
+    DevBuf<int> d_col_off, d_col_img, d_it_off, d_pl_base, d_pl_off, d_thr_slot, d_atom_slot_off, d_atom_slot;
+    DevBuf<uint32_t> d_items, d_thr_map;
+};
+
+// Threads of a projection workgroup of the row kernels (the thread map is built for it).
+constexpr int kRowsProjThreads = 256;
+bool build_rows_geometry(const qg_grid& g, const NlGeometry& geo, int ty, int tzp, RowsGeo& o);
+
+// Caller-provided FP64 scratch of one application (projector partials, then coefficients).
+struct RowsScratch {
+    double* base;
+    size_t capacity;   // doubles
+};
+
 }  // namespace qg
 
 struct qg_nonlocal {
@@ -200,6 +228,9 @@ struct qg_nonlocal {
     qg::DevBuf<float> d_beta32;
     qg::DevBuf<int> d_gidx;
     qg::DevBuf<int64_t> d_img_tbl_off;
+    // row-kernel geometry, one per (tile rows, projection slab) shape in use
+    std::vector<std::unique_ptr<qg::RowsGeo>> rows;
+    qg::DevBuf<double> rows_part;   // projector partials of qg_nonlocal_project
 };
 
 namespace qg {
@@ -208,6 +239,12 @@ struct PrecondPlan;
 struct BatchService;
 void lobpcg_state_free(LobpcgState*);
 void precond_state_free(PrecondPlan*);
+bool rows_hamiltonian_apply(qg_ham* h, int dtype, int nbands, const void* psi, int64_t ld, void* hpsi,
+                            int64_t ld_out, hipStream_t s, RowsScratch* scratch);
+bool rows_nonlocal_project(qg_nonlocal* nl, int dtype, int nbands, const void* psi, int64_t ld, double* coef,
+                           DevBuf<double>& part_buf, hipStream_t s);
+bool rows_laplacian_apply(const qg_grid& g, int dtype, int nbands, const void* psi, int64_t ld, void* out,
+                          int64_t ld_out, double coef, int accumulate, hipStream_t s);
 }  // namespace qg
 
 struct qg_ham {
@@ -222,6 +259,7 @@ struct qg_ham {
     bool timing = false;
     bool timed_valid = false;
     std::vector<hipEvent_t> tev;   // per band chunk: before projection, before fused, after fused
+    qg::DevBuf<double> rows_part;  // projector partials of the row path
     int tchunks = 0;
     ~qg_ham() {
         for (auto& e : tev)

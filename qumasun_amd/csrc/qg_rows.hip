@@ -1,0 +1,939 @@
+// qg_rows.hip — H·psi on MI355X (gfx950) with 16-byte row vectors: a tile-centric projector
+// dot-product kernel and a stencil + local potential + projector-apply kernel that writes each
+// H psi value exactly once.
+//
+// Reference semantics (QUMASUN source/):
+//   out += coef Lap8 psi, periodic            GridDifference8th.h:83-185
+//   H psi = V psi + V_NL psi - 1/2 Lap8 psi   qumasun_mpi.matrix.h:21-43
+//   c_aj = dV sum_i beta_j(i) psi(i)          PseudoPotOperator_mpi_v2.h:336-463
+//   V_NL psi = sum_aj E_j c_aj beta_j         PseudoPotOperator_mpi_v2.h:465-570
+//   beta_j = p(r) Y_lm(r), Y_1m = sqrt(3/4pi) (y, z, x)/r, Y_1m(r^2 < 1e-14) = 0
+//                                             RealSphericalHarmonics.h:149-161, SetBlockYlm.h:41
+//
+// Geometry.  A y-tile is TY full x-rows of every z-plane.  Its "column" images are the
+// periodic sphere images whose y-range meets the tile.  The sphere points of one (tile,
+// plane) are listed as items: up to 4 consecutive points of one image on one tile row
+// (row | x start << 5 | (count - 1) << 15 | column image << 17), sorted by image.
+//
+// Projection (rows_project_kernel): a workgroup owns a tile, a slab of planes and NBP bands.
+// Per plane it stages the tile's rows of its bands in LDS (band fastest, one HBM read of
+// psi); every thread owns one column image for the whole slab (threads per image in
+// proportion to its sphere points) and accumulates beta . psi in registers.  At the slab's
+// end the image's threads reduce through LDS and write one FP64 partial per (slot, band);
+// rows_coef_reduce_kernel sums an atom's partials (no global atomics, deterministic).
+//
+// Apply (rows_hpsi_kernel): a workgroup owns a tile, a slab of planes and NBW bands (one
+// group of threads per band); each thread owns VN = 16 / sizeof(T) consecutive x-points of
+// one row and streams through the slab in z with the 9 z-neighbours in registers, the
+// current plane (+ 4 wrapped columns each side and the 4-row y-halo) in LDS, double
+// buffered.  Per plane the workgroup first evaluates the projector term item-parallel
+// (beta computed once per sphere point and applied to all NBW bands, LDS float adds into a
+// per-plane buffer), then each thread forms -1/2 Lap8 psi + V psi + that term and stores it.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include "qg_internal.hpp"
+
+namespace qg {
+
+namespace {
+
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+using u4 = __attribute__((ext_vector_type(4))) uint32_t;
+
+template <class T>
+struct alignas(16) Row {
+    T v[16 / sizeof(T)];
+};
+
+__device__ __forceinline__ rsrc_t rsrc(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+template <class T>
+__device__ __forceinline__ Row<T> vload(rsrc_t r, int voff, int soff) {
+    return __builtin_bit_cast(Row<T>, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+template <class T>
+__device__ __forceinline__ void vstore(rsrc_t r, const Row<T>& v, int voff, int soff) {
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, v), r, voff, soff, 0);
+}
+
+__device__ __forceinline__ int xcd_order(int i, int n) {
+    // hardware workgroup ids go round-robin over the 8 XCDs; give each XCD one contiguous
+    // range of logical ids (neighbouring tiles of a band share halo rows in that XCD's L2)
+    const int xcd = i & 7, slot = i >> 3;
+    const int per = n >> 3, rem = n & 7;
+    return ((xcd < rem) ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per) + slot;
+}
+
+template <class T>
+struct Coef8 {
+    T c0;
+    T cx[4], cy[4], cz[4];
+};
+
+// exp(-rr k') with k' = k log2(e) in FP32 (hardware exp2), exp(-rr k) in FP64
+template <class T>
+__device__ __forceinline__ T gauss(T rr, T k) {
+    if constexpr (sizeof(T) == 4) return exp2f(-(rr * k));
+    else return exp(-(rr * k));
+}
+template <class T>
+__device__ __forceinline__ T inv_sqrt(T rr) {
+    if constexpr (sizeof(T) == 4) return __frsqrt_rn(rr);
+    else return 1.0 / sqrt(rr);
+}
+
+}  // namespace
+
+// Per-image staging of the projector values: integer part n of the centre in grid units
+// (shift included) and fractional part f, so the displacement h((i - n) - f) is exact at a
+// nucleus on a grid point; projector range; for the s+p form the scaled amplitudes.
+struct RowsImg {
+    int n[3];
+    int pb, pe;
+    int atom;
+};
+
+struct RowsHpsiArgs {
+    int nx, ny, nz;
+    int ty, tz, ntile, nslab, nbands;
+    int tpb;                    // threads per band (whole wavefronts)
+    int64_t ld_in, ld_out;
+    int add_v;                  // H psi: + V psi
+    int accumulate;             // Laplacian only: + previous out
+    int has_nl;
+    int njs;                    // coefficient slots per atom in the staged E*c (4 or 8)
+    const int* col_off;         // per tile: first column image
+    const int* col_img;         // global image ids
+    const int* it_off;          // per (tile, plane): first item
+    const uint32_t* items;
+    const DevImage* images;
+    const DevProj* projs;
+    int nproj;
+    const double* coef;         // nbands x ncoef, this application's bands
+    const double* coef_energy;
+    const int* atom_coef_off;
+    int ncoef;
+    int max_col;
+    double h[3];
+};
+
+struct RowsProjArgs {
+    int nx, ny, nz;
+    int ty, tz, ntile, nslab, nbands;
+    int64_t ld;
+    const int* col_off;
+    const int* col_img;
+    const int* it_off;
+    const uint32_t* items;
+    const int* pl_base;         // per tile: base of its (plane, column image) item offsets
+    const int* pl_off;          // (plane, image) -> first item of the image in the plane
+    const uint32_t* thr_map;    // per (tile, slab, thread): image | rank << 10 | (count-1) << 18
+    const int* thr_slot;        // per (tile, slab, thread): partial slot of the thread's image
+    const DevImage* images;
+    const DevProj* projs;
+    int nproj;
+    double* part;               // part[(slot * NJS + j) * part_ld + band]
+    int part_ld;
+    int max_col;
+    double h[3];
+};
+
+// Projector-value staging shared by both kernels: image integer/fraction parts, and per
+// projector l, coefficient slot, amplitude with its Y_lm constant, and exponent factor.
+template <class T>
+struct ProjLds {
+    int* im_i;     // [max_col][5]: n.x n.y n.z pb pe
+    T* im_f;       // [max_col][3]: f
+    int* pr_i;     // [nproj][2]: l, coef_rel
+    T* pr_f;       // [nproj][2]: amp * (Y00 or sqrt(3/4pi)), exponent factor
+};
+
+template <class T>
+__device__ __forceinline__ void stage_proj(const ProjLds<T>& L, const DevImage* images, const DevProj* projs,
+                                           int nproj, const int* col_img, int ncol, const double* h, int tid,
+                                           int nthr) {
+    for (int li = tid; li < ncol; li += nthr) {
+        const DevImage& im = images[col_img[li]];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double u = im.R[d] / h[d];
+            const double nn = rint(u);
+            L.im_f[3 * li + d] = (T)(u - nn);
+            L.im_i[5 * li + d] = im.shift[d] + (int)nn;
+        }
+        L.im_i[5 * li + 3] = im.proj_begin;
+        L.im_i[5 * li + 4] = im.proj_end;
+    }
+    for (int p = tid; p < nproj; p += nthr) {
+        const DevProj& d = projs[p];
+        L.pr_i[2 * p] = d.l;
+        L.pr_i[2 * p + 1] = d.coef_rel;
+        const double ylm = d.l == 0 ? 0.28209479177387814 : 0.48860251190291992;   // 1/sqrt(4pi), sqrt(3/4pi)
+        L.pr_f[2 * p] = (T)(d.amp * ylm);
+        L.pr_f[2 * p + 1] = (sizeof(T) == 4) ? (T)(d.inv_2s2 * 1.4426950408889634) : (T)d.inv_2s2;
+    }
+}
+
+// beta_j at point (ix, iy, iz) of column image li, for coefficient slots j < NJS (zero for
+// slots of projectors the image does not hold).  SP4: the image holds exactly an s channel
+// in slot 0 and a p channel in slots 1..3.
+template <class T, int NJS, bool SP4>
+__device__ __forceinline__ void beta_at(const ProjLds<T>& L, int li, T dx, T dy, T dz, T (&beta)[NJS]) {
+    const T rr = dx * dx + (dy * dy + dz * dz);
+    T ir = T(0);
+    if (!(T(1.0e-14) > rr)) ir = inv_sqrt<T>(rr);
+    const int* ii = L.im_i + 5 * li;
+    if constexpr (SP4) {
+        const int pb = ii[3];
+        const T es = L.pr_f[2 * pb] * gauss<T>(rr, L.pr_f[2 * pb + 1]);
+        const T ep = L.pr_f[2 * pb + 2] * gauss<T>(rr, L.pr_f[2 * pb + 3]) * ir;
+        beta[0] = es;
+        beta[1] = ep * dy;
+        beta[2] = ep * dz;
+        beta[3] = ep * dx;
+    } else {
+#pragma unroll
+        for (int j = 0; j < NJS; ++j) beta[j] = T(0);
+        for (int p = ii[3]; p < ii[4]; ++p) {
+            const int l = L.pr_i[2 * p], cr = L.pr_i[2 * p + 1];
+            const T rad = L.pr_f[2 * p] * gauss<T>(rr, L.pr_f[2 * p + 1]);
+            if (l == 0) {
+#pragma unroll
+                for (int j = 0; j < NJS; ++j)
+                    if (j == cr) beta[j] = rad;
+            } else {
+                const T rp = rad * ir;
+#pragma unroll
+                for (int j = 0; j < NJS; ++j) {
+                    if (j == cr) beta[j] = rp * dy;
+                    if (j == cr + 1) beta[j] = rp * dz;
+                    if (j == cr + 2) beta[j] = rp * dx;
+                }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------ projection ----
+constexpr int kProjThreads = kRowsProjThreads;
+constexpr int kProjPV = 8;   // staged vectors per thread per plane
+
+template <class T, int NBP, int NJS, bool SP4>
+__global__ __launch_bounds__(kProjThreads) void rows_project_kernel(RowsProjArgs a, const T* __restrict__ psi) {
+    constexpr int VN = 16 / sizeof(T);
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int wg = xcd_order(blockIdx.x, gridDim.x);
+    const int tile = wg % a.ntile;
+    const int rest = wg / a.ntile;
+    const int slab = rest % a.nslab;
+    const int b0 = (rest / a.nslab) * NBP;
+    const int y0 = tile * a.ty;
+    const int rows = min(a.ty, ny - y0);
+    const int z0 = slab * a.tz, z1 = min(z0 + a.tz, nz);
+    const int tid = threadIdx.x;
+    const int plane = nx * ny;
+
+    // LDS: the tile plane of NBP bands, band fastest [row][x][band]; reused for the reduction
+    const size_t pl_elems = (size_t)a.ty * nx * NBP;
+    const size_t red_elems = (size_t)kProjThreads * NJS * NBP;
+    T* pl = reinterpret_cast<T*>(smem);
+    unsigned char* lp = smem + sizeof(T) * (pl_elems > red_elems ? pl_elems : red_elems);
+    ProjLds<T> L;
+    L.im_f = reinterpret_cast<T*>(lp);
+    lp += sizeof(T) * 3 * a.max_col;
+    L.pr_f = reinterpret_cast<T*>(lp);
+    lp += sizeof(T) * 2 * a.nproj;
+    L.im_i = reinterpret_cast<int*>(lp);
+    lp += sizeof(int) * 5 * a.max_col;
+    L.pr_i = reinterpret_cast<int*>(lp);
+
+    const int c0 = a.col_off[tile];
+    const int ncol = a.col_off[tile + 1] - c0;
+    stage_proj<T>(L, a.images, a.projs, a.nproj, a.col_img + c0, ncol, a.h, tid, kProjThreads);
+
+    const uint32_t tm = a.thr_map[((size_t)tile * a.nslab + slab) * kProjThreads + tid];
+    const bool act = tm != 0xffffffffu;
+    const int li = (int)(tm & 1023u);
+    const int rk = (int)((tm >> 10) & 255u);
+    const int cnt = (int)((tm >> 18) & 255u) + 1;
+
+    // staging assignment: vector v -> band v % NBP, then x-vector, then row; NBP divides the
+    // thread count, so every vector of a thread belongs to band tid % NBP
+    static_assert(kProjThreads % NBP == 0, "bands per workgroup must divide the thread count");
+    const int nq = nx / VN;
+    const int nvec = NBP * rows * nq;
+    const int bs = tid % NBP;
+    // lanes of a wavefront read different bands: plain global loads (a buffer resource must
+    // be wavefront-uniform)
+    const Row<T>* src = reinterpret_cast<const Row<T>*>(psi + (int64_t)min(b0 + bs, a.nbands - 1) * a.ld);
+    int vo[kProjPV], lo[kProjPV];
+#pragma unroll
+    for (int k = 0; k < kProjPV; ++k) {
+        const int v = tid + kProjThreads * k;
+        const bool ok = v < nvec;
+        const int rr = ok ? v / NBP : 0;
+        const int xq = rr % nq, r = rr / nq;
+        vo[k] = ok ? ((y0 + r) * nx + xq * VN) / VN : 0;   // in vectors
+        lo[k] = ok ? ((r * nx + xq * VN) * NBP + bs) : -1;
+    }
+    Row<T> pre[kProjPV];
+    auto load = [&](int z) {
+        const Row<T>* sp = src + (int64_t)z * (plane / VN);
+#pragma unroll
+        for (int k = 0; k < kProjPV; ++k)
+            if (lo[k] >= 0) pre[k] = sp[vo[k]];
+    };
+
+    T acc[NJS][NBP];
+#pragma unroll
+    for (int j = 0; j < NJS; ++j)
+#pragma unroll
+        for (int b = 0; b < NBP; ++b) acc[j][b] = T(0);
+
+    const T hx = (T)a.h[0], hy = (T)a.h[1], hz = (T)a.h[2];
+    const int* plb = a.pl_off + a.pl_base[tile];
+    load(z0);
+    for (int z = z0; z < z1; ++z) {
+        __syncthreads();   // the previous plane's readers are done (and the staging above)
+#pragma unroll
+        for (int k = 0; k < kProjPV; ++k)
+            if (lo[k] >= 0) {
+#pragma unroll
+                for (int i = 0; i < VN; ++i) pl[lo[k] + i * NBP] = pre[k].v[i];
+            }
+        if (z + 1 < z1) load(z + 1);
+        __syncthreads();
+        if (act) {
+            const int ib = a.it_off[(size_t)tile * nz + z];
+            const int* po = plb + (size_t)z * (ncol + 1);
+            const int s = ib + po[li], e = ib + po[li + 1];
+            const int* ii = L.im_i + 5 * li;
+            const T* ff = L.im_f + 3 * li;
+            const T dz = hz * ((T)(z - ii[2]) - ff[2]);
+            for (int i = s + rk; i < e; i += cnt) {
+                const uint32_t it = a.items[i];
+                const int r = (int)(it & 31u), xs = (int)((it >> 5) & 1023u), np = (int)((it >> 15) & 3u) + 1;
+                const T dy = hy * ((T)(y0 + r - ii[1]) - ff[1]);
+                const T* prow = pl + (size_t)(r * nx + xs) * NBP;
+#pragma unroll
+                for (int p = 0; p < 4; ++p) {
+                    if (p < np) {
+                        const T dx = hx * ((T)(xs + p - ii[0]) - ff[0]);
+                        T beta[NJS];
+                        beta_at<T, NJS, SP4>(L, li, dx, dy, dz, beta);
+                        const Row<T>* pv = reinterpret_cast<const Row<T>*>(prow + p * NBP);
+#pragma unroll
+                        for (int q = 0; q < NBP / VN; ++q) {
+                            const Row<T> w = pv[q];
+#pragma unroll
+                            for (int u = 0; u < VN; ++u)
+#pragma unroll
+                                for (int j = 0; j < NJS; ++j) acc[j][q * VN + u] += beta[j] * w.v[u];
+                        }
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    T* red = pl;
+#pragma unroll
+    for (int j = 0; j < NJS; ++j)
+#pragma unroll
+        for (int b = 0; b < NBP; ++b) red[(size_t)tid * NJS * NBP + j * NBP + b] = acc[j][b];
+    __syncthreads();
+    if (act) {
+        const int tb = tid - rk;
+        const int slot = a.thr_slot[((size_t)tile * a.nslab + slab) * kProjThreads + tid];
+        for (int o = rk; o < NJS * NBP; o += cnt) {
+            double sum = 0.0;
+            for (int u = 0; u < cnt; ++u) sum += (double)red[(size_t)(tb + u) * NJS * NBP + o];
+            const int j = o / NBP, b = o - j * NBP;
+            if (b0 + b < a.nbands) a.part[((size_t)slot * NJS + j) * a.part_ld + b0 + b] = sum;
+        }
+    }
+}
+
+// coef[b * ncoef + off(a) + j] = dV * sum of the atom's partials
+__global__ void rows_coef_reduce_kernel(const double* __restrict__ part, int part_ld, int njs,
+                                        const int* __restrict__ atom_slot_off, const int* __restrict__ atom_slot,
+                                        const int* __restrict__ atom_coef_off, int natoms, int nbands, double dv,
+                                        double* __restrict__ coef, int ncoef) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)natoms * njs * nbands) return;
+    const int b = (int)(idx % nbands);
+    const int64_t rest = idx / nbands;
+    const int j = (int)(rest % njs);
+    const int at = (int)(rest / njs);
+    const int cb = atom_coef_off[at];
+    if (j >= atom_coef_off[at + 1] - cb) return;
+    double sum = 0.0;
+    for (int s = atom_slot_off[at]; s < atom_slot_off[at + 1]; ++s)
+        sum += part[((size_t)atom_slot[s] * njs + j) * part_ld + b];
+    coef[(int64_t)b * ncoef + cb + j] = dv * sum;
+}
+
+// -------------------------------------------------------------- stencil + V + projector ----
+constexpr int kRowsMaxThreads = 1024;
+
+template <class T, int NBW, int NJS, bool SP4>
+__global__ __launch_bounds__(NBW >= 4 ? 1024 : 512) void rows_hpsi_kernel(RowsHpsiArgs a, Coef8<T> sc, const T* __restrict__ V,
+                                                                  const T* __restrict__ psi, T* __restrict__ out) {
+    constexpr int VN = 16 / sizeof(T);
+    constexpr int S = sizeof(T);
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int W = nx + 8;
+    const int ty = a.ty;
+    const int lband = (ty + 8) * W;
+    const int nthr = blockDim.x;
+    T* planes = reinterpret_cast<T*>(smem);                    // [2][NBW][ty + 8][W]
+    T* nlb = planes + (size_t)2 * NBW * lband;                 // [NBW][ty][nx]
+    T* ec = nlb + (size_t)NBW * ty * nx;                       // [max_col][NBW][NJS]
+    unsigned char* lp = reinterpret_cast<unsigned char*>(ec + (size_t)a.max_col * NBW * NJS);
+    ProjLds<T> L;
+    L.im_f = reinterpret_cast<T*>(lp);
+    lp += sizeof(T) * 3 * a.max_col;
+    L.pr_f = reinterpret_cast<T*>(lp);
+    lp += sizeof(T) * 2 * a.nproj;
+    L.im_i = reinterpret_cast<int*>(lp);
+    lp += sizeof(int) * 5 * a.max_col;
+    L.pr_i = reinterpret_cast<int*>(lp);
+
+    const int wg = xcd_order(blockIdx.x, gridDim.x);
+    const int tile = wg % a.ntile;
+    const int rest = wg / a.ntile;
+    const int slab = rest % a.nslab;
+    const int b0 = (rest / a.nslab) * NBW;
+    const int y0 = tile * ty;
+    const int rows = min(ty, ny - y0);
+    const int z0 = slab * a.tz, z1 = min(z0 + a.tz, nz);
+    const int tid = threadIdx.x;
+    // tpb is a multiple of 64: every lane of a wavefront works on the same band
+    const int bl = __builtin_amdgcn_readfirstlane(tid / a.tpb);
+    const int lt = tid - bl * a.tpb;
+    const int nq = nx / VN;
+    const int r = lt / nq, xq = lt - r * nq;
+    const int x = xq * VN;
+    const bool own = bl < NBW && r < rows;
+    const bool bok = (b0 + bl) < a.nbands;
+    const int plane = nx * ny;
+    const uint32_t band_bytes = (uint32_t)((int64_t)plane * nz * S);
+    const int band = min(b0 + min(bl, NBW - 1), a.nbands - 1);
+    const rsrc_t rin = rsrc(psi + (int64_t)band * a.ld_in, band_bytes);
+    const rsrc_t rout = rsrc(out + (int64_t)band * a.ld_out, band_bytes);
+    const rsrc_t rv = rsrc(V, band_bytes);
+
+    // own vector: plane offset, LDS slot (and the wrapped copy that pads the x-halo)
+    const int vo = own ? ((y0 + r) * nx + x) * S : 0;
+    const int lc = (r + 4) * W + 4 + x;
+    const int lw = (x < 4) ? lc + nx : ((x >= nx - 4) ? lc - nx : -1);
+    // halo vector of this thread: 8 rows x nq vectors per band (4 above the tile, 4 below)
+    const bool hown = bl < NBW && lt < 8 * nq;
+    int ho = 0, hl = 0, hw = -1;
+    if (hown) {
+        const int hr = lt / nq, hx = (lt - hr * nq) * VN;
+        int gy = (hr < 4) ? y0 - 4 + hr : y0 + rows + (hr - 4);
+        gy = ((gy % ny) + ny) % ny;
+        const int lrow = (hr < 4) ? hr : rows + hr;
+        ho = (gy * nx + hx) * S;
+        hl = lrow * W + 4 + hx;
+        hw = (hx < 4) ? hl + nx : ((hx >= nx - 4) ? hl - nx : -1);
+    }
+    auto zoff = [&](int z) {
+        int zz = z % nz;
+        zz = zz < 0 ? zz + nz : zz;
+        return zz * plane * S;
+    };
+
+    // projector staging: image data of the tile column and E*c of its atoms for the bands
+    const int c0 = a.col_off[tile];
+    const int ncol = a.has_nl ? a.col_off[tile + 1] - c0 : 0;
+    if (a.has_nl) {
+        stage_proj<T>(L, a.images, a.projs, a.nproj, a.col_img + c0, ncol, a.h, tid, nthr);
+        for (int i = tid; i < ncol * NBW * NJS; i += nthr) {
+            const int li = i / (NBW * NJS), b = (i / NJS) % NBW, j = i % NJS;
+            const int at = a.images[a.col_img[c0 + li]].atom;
+            const int cb = a.atom_coef_off[at];
+            T v = T(0);
+            if (j < a.atom_coef_off[at + 1] - cb && b0 + b < a.nbands)
+                v = (T)(a.coef_energy[cb + j] * a.coef[(int64_t)(b0 + b) * a.ncoef + cb + j]);
+            ec[i] = v;
+        }
+        for (int i = tid; i < NBW * ty * nx; i += nthr) nlb[i] = T(0);
+    }
+
+    Row<T> q[9];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+        if (own) q[d] = vload<T>(rin, vo, zoff(z0 - 4 + d));
+        else q[d] = Row<T>{};
+    }
+    Row<T> qn{}, hn{}, vn{};
+    auto prefetch = [&](int z) {
+        if (own) {
+            qn = vload<T>(rin, vo, zoff(z + 4));
+            if (a.add_v) vn = vload<T>(rv, vo, zoff(z));
+            else if (a.accumulate) vn = vload<T>(rout, vo, zoff(z));
+        }
+        if (hown) hn = vload<T>(rin, ho, zoff(z));
+    };
+    prefetch(z0);
+    const T hx = (T)a.h[0], hy = (T)a.h[1], hz = (T)a.h[2];
+
+    for (int z = z0; z < z1; ++z) {
+        const int buf = (z - z0) & 1;
+        T* Lb = planes + ((size_t)buf * NBW + (bl < NBW ? bl : 0)) * lband;
+        q[8] = qn;
+        const Row<T> vcur = vn;
+        if (own) {
+            *reinterpret_cast<Row<T>*>(Lb + lc) = q[4];
+            if (lw >= 0) *reinterpret_cast<Row<T>*>(Lb + lw) = q[4];
+        }
+        if (hown) {
+            *reinterpret_cast<Row<T>*>(Lb + hl) = hn;
+            if (hw >= 0) *reinterpret_cast<Row<T>*>(Lb + hw) = hn;
+        }
+        if (z + 1 < z1) prefetch(z + 1);
+        __syncthreads();
+
+        if (a.has_nl) {
+            // projector term of this plane: items dealt over all threads of the workgroup
+            const int i0 = a.it_off[(size_t)tile * nz + z], i1 = a.it_off[(size_t)tile * nz + z + 1];
+            for (int i = i0 + tid; i < i1; i += nthr) {
+                const uint32_t it = a.items[i];
+                const int ir = (int)(it & 31u), xs = (int)((it >> 5) & 1023u);
+                const int np = (int)((it >> 15) & 3u) + 1, li = (int)(it >> 17);
+                const int* ii = L.im_i + 5 * li;
+                const T* ff = L.im_f + 3 * li;
+                const T dy = hy * ((T)(y0 + ir - ii[1]) - ff[1]);
+                const T dz = hz * ((T)(z - ii[2]) - ff[2]);
+                T e[NBW][NJS];
+#pragma unroll
+                for (int b = 0; b < NBW; ++b) {
+                    const Row<T>* ep = reinterpret_cast<const Row<T>*>(ec + ((size_t)li * NBW + b) * NJS);
+#pragma unroll
+                    for (int q4 = 0; q4 < NJS / VN; ++q4) {
+                        const Row<T> w = ep[q4];
+#pragma unroll
+                        for (int u = 0; u < VN; ++u) e[b][q4 * VN + u] = w.v[u];
+                    }
+                }
+                T* nrow = nlb + (size_t)ir * nx + xs;
+#pragma unroll
+                for (int p = 0; p < 4; ++p) {
+                    if (p < np) {
+                        const T dx = hx * ((T)(xs + p - ii[0]) - ff[0]);
+                        T beta[NJS];
+                        beta_at<T, NJS, SP4>(L, li, dx, dy, dz, beta);
+#pragma unroll
+                        for (int b = 0; b < NBW; ++b) {
+                            T v = T(0);
+#pragma unroll
+                            for (int j = 0; j < NJS; ++j) v += e[b][j] * beta[j];
+                            __hip_atomic_fetch_add(nrow + (size_t)b * ty * nx + p, v, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+
+        if (own) {
+            const T* Lc = Lb + (r + 4) * W + x;   // Lc[4 + i] is point x + i
+            Row<T> res;
+#pragma unroll
+            for (int i = 0; i < VN; ++i) {
+                T base = T(0);
+                if (a.add_v) base = vcur.v[i] * q[4].v[i];
+                else if (a.accumulate) base = vcur.v[i];
+                res.v[i] = base;
+            }
+            if (a.has_nl) {
+                Row<T>* np = reinterpret_cast<Row<T>*>(nlb + ((size_t)bl * ty + r) * nx + x);
+                const Row<T> nlv = *np;
+                *np = Row<T>{};
+#pragma unroll
+                for (int i = 0; i < VN; ++i) res.v[i] += nlv.v[i];
+            }
+            // stencil: centre and z from the register queue, x and y from LDS, accumulated
+            // term by term so few values are live at once
+            T d[VN];
+#pragma unroll
+            for (int i = 0; i < VN; ++i) {
+                d[i] = sc.c0 * q[4].v[i];
+                d[i] += sc.cz[3] * (q[8].v[i] + q[0].v[i]) + sc.cz[2] * (q[7].v[i] + q[1].v[i]) +
+                        sc.cz[1] * (q[6].v[i] + q[2].v[i]) + sc.cz[0] * (q[5].v[i] + q[3].v[i]);
+            }
+            {
+                T xsv[VN + 8];
+#pragma unroll
+                for (int k = 0; k < (VN + 8) / VN; ++k) {
+                    const Row<T> w = *reinterpret_cast<const Row<T>*>(Lc + k * VN);
+#pragma unroll
+                    for (int u = 0; u < VN; ++u) xsv[k * VN + u] = w.v[u];
+                }
+#pragma unroll
+                for (int i = 0; i < VN; ++i)
+                    d[i] += sc.cx[3] * (xsv[i + 8] + xsv[i]) + sc.cx[2] * (xsv[i + 7] + xsv[i + 1]) +
+                            sc.cx[1] * (xsv[i + 6] + xsv[i + 2]) + sc.cx[0] * (xsv[i + 5] + xsv[i + 3]);
+            }
+#pragma unroll
+            for (int m = 3; m >= 0; --m) {
+                const Row<T> ym = *reinterpret_cast<const Row<T>*>(Lc + 4 - (m + 1) * W);
+                const Row<T> yp = *reinterpret_cast<const Row<T>*>(Lc + 4 + (m + 1) * W);
+#pragma unroll
+                for (int i = 0; i < VN; ++i) d[i] += sc.cy[m] * (yp.v[i] + ym.v[i]);
+            }
+#pragma unroll
+            for (int i = 0; i < VN; ++i) res.v[i] += d[i];
+            if (bok) vstore<T>(rout, res, vo, zoff(z));
+        }
+#pragma unroll
+        for (int d = 0; d < 8; ++d) q[d] = q[d + 1];
+    }
+}
+
+// ------------------------------------------------------------------------ host side ----
+namespace {
+
+template <class T>
+Coef8<T> coef8(const qg_grid& g, double coef) {
+    const double w[4] = {8064.0, -1008.0, 128.0, -9.0};
+    Coef8<T> s;
+    s.c0 = (T)(-14350.0 / 5040.0 * coef *
+               (1.0 / (g.h[0] * g.h[0]) + 1.0 / (g.h[1] * g.h[1]) + 1.0 / (g.h[2] * g.h[2])));
+    for (int m = 0; m < 4; ++m) {
+        s.cx[m] = (T)(w[m] * coef / (5040.0 * g.h[0] * g.h[0]));
+        s.cy[m] = (T)(w[m] * coef / (5040.0 * g.h[1] * g.h[1]));
+        s.cz[m] = (T)(w[m] * coef / (5040.0 * g.h[2] * g.h[2]));
+    }
+    return s;
+}
+
+int round64i(int v) { return (v + 63) / 64 * 64; }
+
+}  // namespace
+
+// Tile height for a row of nq vectors: whole wavefronts per band with as few idle lanes as
+// possible (rows of 8-16 preferred), at least 8 rows' worth of threads for the y-halo.
+static int rows_pick_ty(int nq, int ny, int want) {
+    if (want > 0) return std::max(1, std::min(ny, std::min(32, want)));
+    int best = 8, bests = -1;
+    for (int ty : {8, 16, 12, 10, 6, 20, 24, 4, 32}) {
+        if (ty > ny) continue;
+        const int t = round64i(std::max(ty, 8) * nq);
+        const int score = (int)(1000.0 * ty * nq / t) + (ty >= 8 ? 5 : 0);
+        if (score > bests) { bests = score; best = ty; }
+    }
+    return std::min(best, ny);
+}
+
+// Whether the row kernels serve this grid and projector set: x-rows of whole 16-byte vectors,
+// Gaussian projectors with l <= 1 and at most 8 coefficient slots per atom.
+static bool rows_eligible(const qg_grid& g, int dtype, const qg_nonlocal* nl) {
+    if (env_int("QG_ROWS", 1) == 0) return false;
+    const int VN = dtype == QG_F32 ? 4 : 2;
+    if (g.n[0] % VN != 0 || g.n[0] < 8 || g.n[0] > 1024) return false;
+    if (g.ng() * 8 >= (1LL << 31)) return false;
+    if (nl) {
+        if (nl->geo.max_coef_per_atom > 8) return false;
+        for (const DevProj& p : nl->geo.projs)
+            if (p.kind != QG_RADIAL_GAUSSIAN || p.l > 1) return false;
+    }
+    return true;
+}
+
+struct RowsShape {
+    int ty, tz, tzp, nbw, nbp;
+};
+
+static RowsShape rows_shape(const qg_grid& g, int dtype, int nbands) {
+    const int VN = dtype == QG_F32 ? 4 : 2;
+    const int nq = g.n[0] / VN;
+    RowsShape sh{};
+    sh.ty = rows_pick_ty(nq, g.n[1], env_int("QG_ROWS_TY", 0));
+    sh.nbw = env_int("QG_ROWS_NBW", dtype == QG_F32 ? 2 : 1);
+    if (sh.nbw != 1 && sh.nbw != 2 && sh.nbw != 4) sh.nbw = 2;
+    while (sh.nbw > 1 && sh.nbw * round64i(std::max(sh.ty, 8) * nq) > (sh.nbw >= 4 ? 1024 : 512)) sh.nbw /= 2;
+    sh.tz = std::max(1, std::min(g.n[2], env_int("QG_ROWS_TZ", g.n[2])));
+    sh.tzp = std::max(1, std::min(g.n[2], env_int("QG_ROWS_TZP", 16)));
+    sh.nbp = env_int("QG_ROWS_NBP", 8) == 4 ? 4 : 8;
+    (void)nbands;
+    return sh;
+}
+
+static RowsGeo* rows_geometry(qg_nonlocal* nl, int ty, int tzp) {
+    for (auto& c : nl->rows)
+        if (c && c->ty == ty && c->tzp == tzp) return c.get();
+    auto c = std::make_unique<RowsGeo>();
+    if (!build_rows_geometry(nl->gcopy, nl->geo, ty, tzp, *c)) {
+        c->ok = false;
+    } else {
+        c->ok = true;
+        auto up = [](auto& buf, auto& vec) {
+            if (vec.empty()) vec.push_back({});
+            buf.upload(vec.data(), vec.size());
+        };
+        up(c->d_col_off, c->col_off);
+        up(c->d_col_img, c->col_img);
+        up(c->d_it_off, c->it_off);
+        up(c->d_items, c->items);
+        up(c->d_pl_base, c->pl_base);
+        up(c->d_pl_off, c->pl_off);
+        up(c->d_thr_map, c->thr_map);
+        up(c->d_thr_slot, c->thr_slot);
+        up(c->d_atom_slot_off, c->atom_slot_off);
+        up(c->d_atom_slot, c->atom_slot);
+        std::vector<uint32_t>().swap(c->items);
+        std::vector<int>().swap(c->pl_off);
+        std::vector<uint32_t>().swap(c->thr_map);
+        std::vector<int>().swap(c->thr_slot);
+    }
+    nl->rows.push_back(std::move(c));
+    return nl->rows.back().get();
+}
+
+template <class T>
+static size_t rows_proj_lds(const RowsGeo& rg, const qg_grid& g, int nbp, int nproj) {
+    const size_t pl = (size_t)rg.ty * g.n[0] * nbp, red = (size_t)kProjThreads * rg.njs * nbp;
+    return sizeof(T) * std::max(pl, red) + sizeof(T) * (3 * rg.max_col + 2 * nproj) +
+           sizeof(int) * (5 * rg.max_col + 2 * nproj);
+}
+
+template <class T>
+static size_t rows_hpsi_lds(const qg_grid& g, int ty, int nbw, int njs, int max_col, int nproj) {
+    const size_t W = g.n[0] + 8;
+    return sizeof(T) * (2 * (size_t)nbw * (ty + 8) * W + (size_t)nbw * ty * g.n[0] + (size_t)max_col * nbw * njs +
+                        3 * (size_t)max_col + 2 * (size_t)nproj) +
+           sizeof(int) * (5 * (size_t)max_col + 2 * (size_t)nproj);
+}
+
+// Scratch of one application: projector partials and coefficients (grown, never shrunk).
+template <class T, int NJS, bool SP4>
+static void rows_launch_project(qg_nonlocal* nl, const RowsGeo& rg, int nbp, int nbands, const T* psi, int64_t ld,
+                                double* part, double* coef, hipStream_t s) {
+    const qg_grid& g = nl->gcopy;
+    RowsProjArgs a{};
+    a.nx = g.n[0]; a.ny = g.n[1]; a.nz = g.n[2];
+    a.ty = rg.ty; a.tz = rg.tzp; a.ntile = rg.ntile; a.nslab = rg.nslabp; a.nbands = nbands;
+    a.ld = ld;
+    a.col_off = rg.d_col_off.p; a.col_img = rg.d_col_img.p;
+    a.it_off = rg.d_it_off.p; a.items = rg.d_items.p;
+    a.pl_base = rg.d_pl_base.p; a.pl_off = rg.d_pl_off.p;
+    a.thr_map = rg.d_thr_map.p; a.thr_slot = rg.d_thr_slot.p;
+    a.images = nl->d_images.p; a.projs = nl->d_projs.p; a.nproj = (int)nl->geo.projs.size();
+    a.part = part;
+    a.part_ld = nbands;
+    a.max_col = std::max(1, rg.max_col);
+    for (int d = 0; d < 3; ++d) a.h[d] = g.h[d];
+    const int VN = 16 / sizeof(T);
+    QG_REQUIRE((int64_t)nbp * rg.ty * (g.n[0] / VN) <= (int64_t)kProjThreads * kProjPV,
+               "projection tile does not fit the staging registers");
+    const size_t lds = rows_proj_lds<T>(rg, g, nbp, a.nproj);
+    QG_REQUIRE(lds <= 65536, "projection tile does not fit in LDS");
+    const int64_t nwg = (int64_t)rg.ntile * rg.nslabp * ((nbands + nbp - 1) / nbp);
+    QG_REQUIRE(nwg < (1LL << 31), "too many workgroups");
+    if (nbp == 8)
+        hipLaunchKernelGGL((rows_project_kernel<T, 8, NJS, SP4>), dim3((unsigned)nwg), dim3(kProjThreads), lds, s, a, psi);
+    else
+        hipLaunchKernelGGL((rows_project_kernel<T, 4, NJS, SP4>), dim3((unsigned)nwg), dim3(kProjThreads), lds, s, a, psi);
+    QG_HIP(hipGetLastError());
+    const int64_t nred = (int64_t)nl->geo.natoms * NJS * nbands;
+    if (nred > 0) {
+        hipLaunchKernelGGL(rows_coef_reduce_kernel, dim3((unsigned)((nred + 255) / 256)), dim3(256), 0, s, part, nbands,
+                           NJS, rg.d_atom_slot_off.p, rg.d_atom_slot.p, nl->d_atom_coef_off.p, nl->geo.natoms, nbands,
+                           g.dvol(), coef, nl->geo.ncoef);
+        QG_HIP(hipGetLastError());
+    }
+}
+
+template <class T, int NBW, int NJS, bool SP4>
+static void rows_launch_hpsi_t(const qg_grid& g, const RowsShape& sh, const RowsGeo* rg, qg_nonlocal* nl, int nbands,
+                               const T* psi, int64_t ld_in, T* out, int64_t ld_out, double lap_coef, int add_v,
+                               int accumulate, const T* V, const double* coef, hipStream_t s) {
+    constexpr int VN = 16 / sizeof(T);
+    const int nq = g.n[0] / VN;
+    RowsHpsiArgs a{};
+    a.nx = g.n[0]; a.ny = g.n[1]; a.nz = g.n[2];
+    a.ty = sh.ty; a.tz = sh.tz;
+    a.ntile = (g.n[1] + sh.ty - 1) / sh.ty;
+    a.nslab = (g.n[2] + sh.tz - 1) / sh.tz;
+    a.nbands = nbands;
+    a.tpb = round64i(std::max(sh.ty, 8) * nq);
+    a.ld_in = ld_in; a.ld_out = ld_out;
+    a.add_v = add_v; a.accumulate = accumulate;
+    a.has_nl = rg != nullptr;
+    a.njs = NJS;
+    a.max_col = 1;
+    if (rg) {
+        a.col_off = rg->d_col_off.p; a.col_img = rg->d_col_img.p;
+        a.it_off = rg->d_it_off.p; a.items = rg->d_items.p;
+        a.images = nl->d_images.p; a.projs = nl->d_projs.p; a.nproj = (int)nl->geo.projs.size();
+        a.coef = coef; a.coef_energy = nl->d_coef_energy.p; a.atom_coef_off = nl->d_atom_coef_off.p;
+        a.ncoef = nl->geo.ncoef;
+        a.max_col = std::max(1, rg->max_col);
+    }
+    for (int d = 0; d < 3; ++d) a.h[d] = g.h[d];
+    const int nthr = NBW * a.tpb;
+    QG_REQUIRE(nthr <= (NBW >= 4 ? 1024 : 512), "row tile too wide for one workgroup");
+    const size_t lds = rows_hpsi_lds<T>(g, sh.ty, NBW, NJS, a.max_col, a.nproj);
+    QG_REQUIRE(lds <= 65536, "row tile does not fit in LDS");
+    const int64_t nwg = (int64_t)a.ntile * a.nslab * ((nbands + NBW - 1) / NBW);
+    QG_REQUIRE(nwg < (1LL << 31), "too many workgroups");
+    hipLaunchKernelGGL((rows_hpsi_kernel<T, NBW, NJS, SP4>), dim3((unsigned)nwg), dim3(nthr), lds, s, a,
+                       coef8<T>(g, lap_coef), V, psi, out);
+    QG_HIP(hipGetLastError());
+}
+
+template <class T, int NJS, bool SP4>
+static void rows_launch_hpsi(const qg_grid& g, const RowsShape& sh, const RowsGeo* rg, qg_nonlocal* nl, int nbands,
+                             const T* psi, int64_t ld_in, T* out, int64_t ld_out, double lap_coef, int add_v,
+                             int accumulate, const T* V, const double* coef, hipStream_t s) {
+    switch (sh.nbw) {
+        case 4:
+            rows_launch_hpsi_t<T, 4, NJS, SP4>(g, sh, rg, nl, nbands, psi, ld_in, out, ld_out, lap_coef, add_v,
+                                               accumulate, V, coef, s);
+            break;
+        case 2:
+            rows_launch_hpsi_t<T, 2, NJS, SP4>(g, sh, rg, nl, nbands, psi, ld_in, out, ld_out, lap_coef, add_v,
+                                               accumulate, V, coef, s);
+            break;
+        default:
+            rows_launch_hpsi_t<T, 1, NJS, SP4>(g, sh, rg, nl, nbands, psi, ld_in, out, ld_out, lap_coef, add_v,
+                                               accumulate, V, coef, s);
+    }
+}
+
+// The row path of one application.  Returns false (having launched nothing) when it does not
+// serve this grid / projector set, so the caller runs the table kernels instead.
+template <class T>
+static bool rows_apply_t(qg_ham* h, int nbands, const T* psi, int64_t ld, T* hpsi, int64_t ld_out, hipStream_t s,
+                         RowsScratch* scratch) {
+    const qg_grid& g = h->grid;
+    qg_nonlocal* nl = (h->nl && h->nl->geo.ncoef > 0) ? h->nl : nullptr;
+    const int dtype = sizeof(T) == 4 ? QG_F32 : QG_F64;
+    if (!rows_eligible(g, dtype, nl)) return false;
+    const RowsShape sh = rows_shape(g, dtype, nbands);
+    const RowsGeo* rg = nullptr;
+    if (nl) {
+        rg = rows_geometry(nl, sh.ty, sh.tzp);
+        if (!rg->ok) return false;
+        if (rows_proj_lds<T>(*rg, g, sh.nbp, (int)nl->geo.projs.size()) > 65536) return false;
+        if (rows_hpsi_lds<T>(g, sh.ty, sh.nbw, rg->njs, std::max(1, rg->max_col), (int)nl->geo.projs.size()) > 65536)
+            return false;
+    } else if (rows_hpsi_lds<T>(g, sh.ty, sh.nbw, 4, 1, 0) > 65536) {
+        return false;
+    }
+    const T* V = sizeof(T) == 4 ? (const T*)(const void*)h->V32.p : (const T*)(const void*)h->V64.p;
+    double* part = nullptr;
+    double* coef = nullptr;
+    if (nl) {
+        const size_t npart = (size_t)std::max(1, rg->nslot) * rg->njs * nbands;
+        const size_t ncoef = (size_t)nbands * nl->geo.ncoef;
+        if (scratch) {
+            QG_REQUIRE(npart + ncoef <= scratch->capacity, "application exceeds the fixed scratch size");
+            part = scratch->base;
+            coef = scratch->base + npart;
+        } else {
+            h->rows_part.alloc(npart);
+            nl->d_coef.alloc(ncoef);
+            part = h->rows_part.p;
+            coef = nl->d_coef.p;
+        }
+    }
+    if (h->timing) QG_HIP(hipEventRecord(h->tev[0], s));
+    if (nl) {
+        if (rg->njs == 4) {
+            if (rg->sp4) rows_launch_project<T, 4, true>(nl, *rg, sh.nbp, nbands, psi, ld, part, coef, s);
+            else rows_launch_project<T, 4, false>(nl, *rg, sh.nbp, nbands, psi, ld, part, coef, s);
+        } else {
+            rows_launch_project<T, 8, false>(nl, *rg, sh.nbp, nbands, psi, ld, part, coef, s);
+        }
+    }
+    if (h->timing) QG_HIP(hipEventRecord(h->tev[1], s));
+    if (!nl || rg->njs == 4) {
+        if (nl && rg->sp4)
+            rows_launch_hpsi<T, 4, true>(g, sh, rg, nl, nbands, psi, ld, hpsi, ld_out, -0.5, 1, 0, V, coef, s);
+        else
+            rows_launch_hpsi<T, 4, false>(g, sh, rg, nl, nbands, psi, ld, hpsi, ld_out, -0.5, 1, 0, V, coef, s);
+    } else {
+        rows_launch_hpsi<T, 8, false>(g, sh, rg, nl, nbands, psi, ld, hpsi, ld_out, -0.5, 1, 0, V, coef, s);
+    }
+    if (h->timing) QG_HIP(hipEventRecord(h->tev[2], s));
+    return true;
+}
+
+bool rows_hamiltonian_apply(qg_ham* h, int dtype, int nbands, const void* psi, int64_t ld, void* hpsi,
+                            int64_t ld_out, hipStream_t s, RowsScratch* scratch) {
+    if (h->timing) {
+        while (h->tev.size() < 3) {
+            hipEvent_t e;
+            QG_HIP(hipEventCreate(&e));
+            h->tev.push_back(e);
+        }
+    }
+    bool done;
+    if (dtype == QG_F32)
+        done = rows_apply_t<float>(h, nbands, (const float*)psi, ld, (float*)hpsi, ld_out, s, scratch);
+    else
+        done = rows_apply_t<double>(h, nbands, (const double*)psi, ld, (double*)hpsi, ld_out, s, scratch);
+    if (done && h->timing) {
+        h->tchunks = 1;
+        h->timed_valid = true;
+    }
+    return done;
+}
+
+// coef_dev[b * ncoef + j] for a band block through the tile-centric projection; false when
+// the row path does not serve this projector set.
+bool rows_nonlocal_project(qg_nonlocal* nl, int dtype, int nbands, const void* psi, int64_t ld, double* coef,
+                           DevBuf<double>& part_buf, hipStream_t s) {
+    const qg_grid& g = nl->gcopy;
+    if (nl->geo.ncoef == 0 || !rows_eligible(g, dtype, nl)) return false;
+    const RowsShape sh = rows_shape(g, dtype, nbands);
+    const RowsGeo* rg = rows_geometry(nl, sh.ty, sh.tzp);
+    if (!rg->ok) return false;
+    const size_t lds = dtype == QG_F32 ? rows_proj_lds<float>(*rg, g, sh.nbp, (int)nl->geo.projs.size())
+                                       : rows_proj_lds<double>(*rg, g, sh.nbp, (int)nl->geo.projs.size());
+    if (lds > 65536) return false;
+    part_buf.alloc((size_t)std::max(1, rg->nslot) * rg->njs * nbands);
+    auto go = [&](auto tag) {
+        using T = decltype(tag);
+        const T* p = (const T*)psi;
+        if (rg->njs == 4) {
+            if (rg->sp4) rows_launch_project<T, 4, true>(nl, *rg, sh.nbp, nbands, p, ld, part_buf.p, coef, s);
+            else rows_launch_project<T, 4, false>(nl, *rg, sh.nbp, nbands, p, ld, part_buf.p, coef, s);
+        } else {
+            rows_launch_project<T, 8, false>(nl, *rg, sh.nbp, nbands, p, ld, part_buf.p, coef, s);
+        }
+    };
+    if (dtype == QG_F32) go(float{});
+    else go(double{});
+    return true;
+}
+
+// out = (accumulate ? out : 0) + coef Lap8 psi through the row stencil; false when the grid
+// does not suit it.
+bool rows_laplacian_apply(const qg_grid& g, int dtype, int nbands, const void* psi, int64_t ld, void* out,
+                          int64_t ld_out, double coef, int accumulate, hipStream_t s) {
+    if (!rows_eligible(g, dtype, nullptr)) return false;
+    const RowsShape sh = rows_shape(g, dtype, nbands);
+    if (dtype == QG_F32) {
+        if (rows_hpsi_lds<float>(g, sh.ty, sh.nbw, 4, 1, 0) > 65536) return false;
+        rows_launch_hpsi<float, 4, false>(g, sh, nullptr, nullptr, nbands, (const float*)psi, ld, (float*)out, ld_out,
+                                          coef, 0, accumulate, nullptr, nullptr, s);
+    } else {
+        if (rows_hpsi_lds<double>(g, sh.ty, sh.nbw, 4, 1, 0) > 65536) return false;
+        rows_launch_hpsi<double, 4, false>(g, sh, nullptr, nullptr, nbands, (const double*)psi, ld, (double*)out,
+                                           ld_out, coef, 0, accumulate, nullptr, nullptr, s);
+    }
+    return true;
+}
+
+}  // namespace qg

@@ -1,0 +1,171 @@
+// Host harness for tests/test_rows_geometry.py: builds the projector geometry of a diamond
+// cell (optionally a second species with separate s and p cutoffs) with the library's host
+// code (qg_setup.cpp), then the row kernels' geometry (tile-column images, per (tile, plane)
+// items, projection thread map and partial slots), and checks it against the per-image
+// sphere intervals it was derived from:
+//   - the items of each (tile, plane) hold exactly the (point, image) pairs of the intervals,
+//     each once, grouped by image as the per-(plane, image) offsets say;
+//   - a projection replayed with the kernel's bookkeeping (each thread's image, rank and
+//     stride over its image's items, partial slots summed per atom) equals the direct dot
+//     product of the FP64 tables with a random psi.
+// Prints one summary line.
+#include "../../qumasun_amd/csrc/qg_internal.hpp"
+#include <cmath>
+#include <cstdio>
+#include <map>
+#include <random>
+#include <set>
+#include <tuple>
+using namespace qg;
+extern "C" int qg_synth_diamond_atoms(int ncell, double a, double* xyz);
+
+int main(int argc, char** argv) {
+    const int nx = argc > 1 ? atoi(argv[1]) : 40;
+    const int ny = argc > 2 ? atoi(argv[2]) : nx;
+    const int nz = argc > 3 ? atoi(argv[3]) : nx;
+    const int ty = argc > 4 ? atoi(argv[4]) : 8;
+    const int tzp = argc > 5 ? atoi(argv[5]) : 16;
+    const int two_species = argc > 6 ? atoi(argv[6]) : 0;
+    const double A = 6.74;
+    qg_grid g;
+    g.n[0] = nx; g.n[1] = ny; g.n[2] = nz;
+    g.h[0] = A / nx; g.h[1] = A / ny; g.h[2] = A / nz;
+    double xyz[24];
+    qg_synth_diamond_atoms(1, A, xyz);
+    int species[8] = {0};
+    const double sig = 0.6, amp = std::sqrt(4.0 / (std::sqrt(M_PI) * sig * sig * sig));
+    qg_proj_spec sp[4] = {};
+    sp[0].l = 0; sp[0].energy = 1.0; sp[0].cutoff = 2.4; sp[0].radial_kind = QG_RADIAL_GAUSSIAN;
+    sp[0].sigma = sig; sp[0].amplitude = amp;
+    sp[1] = sp[0]; sp[1].l = 1; sp[1].energy = -0.5;
+    int nproj[2] = {2, 2};
+    int nspecies = 1;
+    if (two_species) {
+        // species 1: p channel first (slots 0..2), s channel with a smaller cutoff (slot 3)
+        nspecies = 2;
+        sp[2] = sp[1]; sp[2].sigma = 0.5; sp[2].cutoff = 2.0;
+        sp[3] = sp[0]; sp[3].sigma = 0.7; sp[3].cutoff = 1.6;
+        for (int a = 1; a < 8; a += 2) species[a] = 1;
+        // move one nucleus exactly onto a grid point
+        xyz[3] = 3 * g.h[0]; xyz[4] = 5 * g.h[1]; xyz[5] = 2 * g.h[2];
+    }
+    NlGeometry geo;
+    build_nl_geometry(g, 8, xyz, species, nspecies, nproj, sp, geo);
+    RowsGeo rg;
+    if (!build_rows_geometry(g, geo, ty, tzp, rg)) {
+        printf("geometry rejected\n");
+        return 1;
+    }
+    // reference: (tile, z, point-in-tile, global image) from the intervals
+    std::set<std::tuple<int, int, int, int>> ref;
+    for (int i = 0; i < (int)geo.images.size(); ++i) {
+        const DevImage& im = geo.images[i];
+        const int wy = im.e[1] - im.b[1];
+        for (int z = im.b[2]; z < im.e[2]; ++z)
+            for (int y = im.b[1]; y < im.e[1]; ++y) {
+                const int2 iv = geo.intervals[im.row_base + (y - im.b[1]) + wy * (z - im.b[2])];
+                for (int x = iv.x; x < iv.y; ++x) ref.insert({y / ty, z, (y % ty) * nx + x, i});
+            }
+    }
+    int64_t items = 0, dup = 0, badgroup = 0;
+    std::set<std::tuple<int, int, int, int>> got;
+    for (int t = 0; t < rg.ntile; ++t) {
+        const int nc = rg.col_off[t + 1] - rg.col_off[t];
+        for (int z = 0; z < nz; ++z) {
+            const size_t tp = (size_t)t * nz + z;
+            const int* po = &rg.pl_off[rg.pl_base[t] + (size_t)z * (nc + 1)];
+            for (int k = rg.it_off[tp]; k < rg.it_off[tp + 1]; ++k) {
+                const uint32_t it = rg.items[k];
+                const int r = it & 31, xs = (it >> 5) & 1023, c = ((it >> 15) & 3) + 1, li = it >> 17;
+                const int rel = k - rg.it_off[tp];
+                if (li >= nc || rel < po[li] || rel >= po[li + 1]) ++badgroup;
+                const int gi = rg.col_img[rg.col_off[t] + li];
+                for (int p = 0; p < c; ++p) {
+                    ++items;
+                    if (!got.insert({t, z, r * nx + xs + p, gi}).second) ++dup;
+                }
+            }
+        }
+    }
+    int64_t missing = 0, extra = 0;
+    for (auto& k : ref)
+        if (!got.count(k)) ++missing;
+    for (auto& k : got)
+        if (!ref.count(k)) ++extra;
+    // thread map sanity and the replayed projection
+    std::mt19937 mt(7);
+    std::uniform_real_distribution<double> U(-1.0, 1.0);
+    const int64_t ng = g.ng();
+    std::vector<double> psi(ng);
+    for (auto& v : psi) v = U(mt);
+    const int njs = rg.njs;
+    std::vector<double> part((size_t)std::max(1, rg.nslot) * njs, 0.0);
+    std::vector<int> slot_seen(std::max(1, rg.nslot), 0);
+    int64_t badmap = 0;
+    // table value lookup: (global image, grid index) -> table row
+    std::map<std::pair<int, int64_t>, int64_t> tix;
+    for (int i = 0; i < (int)geo.images.size(); ++i)
+        for (int64_t p = geo.img_tbl_off[i]; p < geo.img_tbl_off[i + 1]; ++p) tix[{i, geo.gidx[p]}] = p;
+    for (int t = 0; t < rg.ntile; ++t) {
+        const int nc = rg.col_off[t + 1] - rg.col_off[t];
+        for (int s = 0; s < rg.nslabp; ++s) {
+            int prev_li = -1, prev_end = 0, used = 0;
+            for (int tid = 0; tid < kRowsProjThreads; ++tid) {
+                const size_t at = ((size_t)t * rg.nslabp + s) * kRowsProjThreads + tid;
+                const uint32_t tm = rg.thr_map[at];
+                if (tm == 0xffffffffu) continue;
+                ++used;
+                const int li = tm & 1023, rk = (tm >> 10) & 255, cnt = ((tm >> 18) & 255) + 1;
+                if (li >= nc || rk >= cnt) { ++badmap; continue; }
+                if (rk == 0) {
+                    if (tid != prev_end) ++badmap;   // ranges contiguous from thread 0
+                    prev_li = li;
+                    prev_end = tid + cnt;
+                    ++slot_seen[rg.thr_slot[at]];
+                } else if (li != prev_li) {
+                    ++badmap;
+                }
+                const int gi = rg.col_img[rg.col_off[t] + li];
+                const int slot = rg.thr_slot[at];
+                for (int z = s * tzp; z < std::min(nz, (s + 1) * tzp); ++z) {
+                    const size_t tp = (size_t)t * nz + z;
+                    const int* po = &rg.pl_off[rg.pl_base[t] + (size_t)z * (nc + 1)];
+                    for (int k = rg.it_off[tp] + po[li] + rk; k < rg.it_off[tp] + po[li + 1]; k += cnt) {
+                        const uint32_t it = rg.items[k];
+                        const int r = it & 31, xs = (it >> 5) & 1023, c = ((it >> 15) & 3) + 1;
+                        for (int p = 0; p < c; ++p) {
+                            const int64_t gidx = (xs + p) + (int64_t)nx * ((t * ty + r) + (int64_t)ny * z);
+                            auto f = tix.find({gi, gidx});
+                            if (f == tix.end()) { ++badmap; continue; }
+                            for (int j = 0; j < geo.njs && j < njs; ++j)
+                                part[(size_t)slot * njs + j] += geo.beta[f->second * geo.njs + j] * psi[gidx];
+                        }
+                    }
+                }
+            }
+            if (used > kRowsProjThreads) ++badmap;
+        }
+    }
+    for (int k = 0; k < rg.nslot; ++k)
+        if (slot_seen[k] != 1) ++badmap;
+    double maxrel = 0.0;
+    for (int a = 0; a < geo.natoms; ++a) {
+        const int cb = geo.atom_coef_off[a], cnt = geo.atom_coef_off[a + 1] - cb;
+        for (int j = 0; j < cnt; ++j) {
+            double direct = 0.0, viaslots = 0.0, scale = 0.0;
+            for (int i = geo.atom_img_off[a]; i < geo.atom_img_off[a + 1]; ++i)
+                for (int64_t p = geo.img_tbl_off[i]; p < geo.img_tbl_off[i + 1]; ++p) {
+                    direct += geo.beta[p * geo.njs + j] * psi[geo.gidx[p]];
+                    scale += std::fabs(geo.beta[p * geo.njs + j] * psi[geo.gidx[p]]);
+                }
+            for (int k = rg.atom_slot_off[a]; k < rg.atom_slot_off[a + 1]; ++k)
+                viaslots += part[(size_t)rg.atom_slot[k] * njs + j];
+            maxrel = std::max(maxrel, std::fabs(direct - viaslots) / std::max(1e-300, scale));
+        }
+    }
+    printf("items %lld refpts %zu missing %lld extra %lld dup %lld badgroup %lld badmap %lld maxrel %.3e "
+           "slots %d maxcol %d sp4 %d njs %d\n",
+           (long long)items, ref.size(), (long long)missing, (long long)extra, (long long)dup, (long long)badgroup,
+           (long long)badmap, maxrel, rg.nslot, rg.max_col, (int)rg.sp4, rg.njs);
+    return 0;
+}
