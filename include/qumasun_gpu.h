@@ -90,6 +90,13 @@ typedef struct qg_nonlocal qg_nonlocal;
 int qg_nonlocal_create(const qg_grid* g, int natoms, const double* xyz_host,
                        const int* species_host, int nspecies, const int* nproj_host,
                        const qg_proj_spec* specs, qg_nonlocal** out);
+/* Move the nuclei of an existing handle: rebuild the projector geometry for natoms nuclei at
+ * xyz_host (species indices into the channels given at creation) in place.  The handle keeps
+ * its address, so every Hamiltonian built on it uses the new positions at its next
+ * application.  Replaces PseudoPotIntegrator_mpi::UpdatePosition on a live integrator
+ * (PseudoPotOperator_mpi_v2.h:231-248, called from qumasun_mpi.move.h:15 at every MD or
+ * relaxation step).  Waits for the device first (no application may be in flight). */
+int qg_nonlocal_update(qg_nonlocal* nl, int natoms, const double* xyz_host, const int* species_host);
 int qg_nonlocal_destroy(qg_nonlocal* nl);
 /* Number of (atom, projector, m) coefficients per band (sum of TotalProjectorLM). */
 int qg_nonlocal_num_coef(const qg_nonlocal* nl);

@@ -138,15 +138,18 @@ class PseudoPotIntegrator:
         pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
         nat = pos.shape[0]
         sp = np.zeros(nat, dtype=np.int32) if species is None else np.ascontiguousarray(species, dtype=np.int32)
-        nproj = np.array([len(s) for s in self.species_projectors], dtype=np.int32)
-        specs, keep = _specs_array(self.species_projectors)
-        h = ctypes.c_void_p()
-        check(lib().qg_nonlocal_create(self.grid._h, nat, dptr(pos),
-                                       sp.ctypes.data_as(N.P(ctypes.c_int)), len(nproj),
-                                       nproj.ctypes.data_as(N.P(ctypes.c_int)), specs, ctypes.byref(h)))
         if self._h:
-            lib().qg_nonlocal_destroy(self._h)
-        self._h = h
+            # move the nuclei of the live handle in place: Hamiltonians built on this
+            # integrator keep a valid handle and see the new positions
+            check(lib().qg_nonlocal_update(self._h, nat, dptr(pos), sp.ctypes.data_as(N.P(ctypes.c_int))))
+        else:
+            nproj = np.array([len(s) for s in self.species_projectors], dtype=np.int32)
+            specs, keep = _specs_array(self.species_projectors)
+            h = ctypes.c_void_p()
+            check(lib().qg_nonlocal_create(self.grid._h, nat, dptr(pos),
+                                           sp.ctypes.data_as(N.P(ctypes.c_int)), len(nproj),
+                                           nproj.ctypes.data_as(N.P(ctypes.c_int)), specs, ctypes.byref(h)))
+            self._h = h
         self.positions = pos
         self.species = sp
 

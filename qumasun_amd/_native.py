@@ -68,6 +68,7 @@ SIGNATURES = {
                                     c_double, c_int, c_void_p]),
     "qg_nonlocal_create": (c_int, [c_void_p, c_int, P(c_double), P(c_int), c_int, P(c_int),
                                    P(ProjSpec), P(c_void_p)]),
+    "qg_nonlocal_update": (c_int, [c_void_p, c_int, P(c_double), P(c_int)]),
     "qg_nonlocal_destroy": (c_int, [c_void_p]),
     "qg_nonlocal_num_coef": (c_int, [c_void_p]),
     "qg_nonlocal_projector_form": (c_int, [c_void_p]),

@@ -67,7 +67,9 @@ struct ConvArgs {
     const float* gb;
 };
 
-__device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+// periodic index for any offset: a tile may overshoot the grid by up to a whole tile (kTX / kTY
+// points) on grids smaller than the tile, so a single wrap is not enough
+__device__ __forceinline__ int wrapi(int i, int n) { return ((i % n) + n) % n; }
 
 // Stage the input tile (CIN channels, 1-point periodic halo) into LDS.
 template <int CIN, int IN_MODE>

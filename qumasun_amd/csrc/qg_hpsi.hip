@@ -1148,7 +1148,8 @@ static void launch_nl_batch(int nb, int ppt, int nthr, size_t lds, int total, co
 }
 
 template <class T>
-static void batch_apply_t(const std::vector<BatchReq>& reqs, hipStream_t s, DevBuf<unsigned char>& scratch) {
+static void batch_apply_t(const std::vector<BatchReq>& reqs, hipStream_t s, DevBuf<unsigned char>& scratch,
+                          DevBuf<double>& coefs) {
     const int dtype = sizeof(T) == 4 ? QG_F32 : QG_F64;
     const int n = (int)reqs.size();
     std::vector<int> cb(n);
@@ -1166,17 +1167,23 @@ static void batch_apply_t(const std::vector<BatchReq>& reqs, hipStream_t s, DevB
         all_split = all_split && sc;
     }
     split = split && all_split;
+    // every request gets its own coefficient region (cells may share one nonlocal handle,
+    // e.g. two spin channels on one integrator)
+    std::vector<size_t> coff(n, 0);
+    size_t ctotal = 0;
     for (int c = 0; c < n; ++c) {
         const BatchReq& r = reqs[c];
         qg_nonlocal* nl = (r.h->nl && r.h->nl->geo.ncoef > 0) ? r.h->nl : nullptr;
         cb[c] = nl ? chunk_bands(r.h->grid, dtype, r.nbands, split) : std::max(1, r.nbands);
         if (nl) {
-            nl->d_coef.alloc((size_t)std::min(cb[c], r.nbands) * nl->geo.ncoef);
+            coff[c] = ctotal;
+            ctotal += (size_t)std::min(cb[c], r.nbands) * nl->geo.ncoef;
             QG_REQUIRE(njs == 0 || njs == nl->geo.njs, "cells of one batch must share the projector slot count");
             njs = nl->geo.njs;
         }
         rounds = std::max(rounds, (r.nbands + cb[c] - 1) / cb[c]);
     }
+    coefs.alloc(std::max<size_t>(1, ctotal));
     if (njs == 0) njs = 4;
     const int NBA = njs == 4 ? 8 : 4;
     // descriptors of every round, packed into one upload
@@ -1234,16 +1241,17 @@ static void batch_apply_t(const std::vector<BatchReq>& reqs, hipStream_t s, DevB
                 a.img_tbl_off = nl->d_img_tbl_off.p;
                 a.gidx = nl->d_gidx.p;
                 a.beta = (sizeof(T) == 4) ? (const void*)nl->d_beta32.p : (const void*)nl->d_beta64.p;
-                a.coef = nl->d_coef.p;
+                a.coef = coefs.p + coff[c];
                 a.ncoef = nl->geo.ncoef;
                 pcs.push_back(ProjCell<T>{a, pin});
                 ppre.push_back(ppre.back() + a.natoms * ((nb + NBA - 1) / NBA));
             }
             qg_nonlocal* fnl = split ? nullptr : nl;
-            const FusedSetup<T> fs = setup_fused<T, MODE_HAM>(g, nb, ng, ng, -0.5, 0, fnl, fnl ? fnl->d_coef.p : nullptr);
+            double* ccoef = nl ? coefs.p + coff[c] : nullptr;
+            const FusedSetup<T> fs = setup_fused<T, MODE_HAM>(g, nb, ng, ng, -0.5, 0, fnl, fnl ? ccoef : nullptr);
             QG_REQUIRE(fs.p.nb == 1, "batched cells use one band per workgroup");
             if (split && nl) {
-                const FusedSetup<T> fn = setup_fused<T, MODE_NL>(g, nb, ng, ng, 0.0, 1, nl, nl->d_coef.p);
+                const FusedSetup<T> fn = setup_fused<T, MODE_NL>(g, nb, ng, ng, 0.0, 1, nl, ccoef);
                 QG_REQUIRE(fn.p.nb == 1 || fn.p.nb == 2 || fn.p.nb == 4 || fn.p.nb == 8,
                            "batched projector pass: unsupported bands per workgroup");
                 QG_REQUIRE(fn.p.ppt == 2 || fn.p.ppt == 4, "batched projector pass: unsupported points per thread");
@@ -1327,10 +1335,10 @@ static void batch_apply_t(const std::vector<BatchReq>& reqs, hipStream_t s, DevB
 }
 
 void batch_hamiltonian_apply(const std::vector<BatchReq>& reqs, int dtype, hipStream_t s,
-                             DevBuf<unsigned char>& scratch) {
+                             DevBuf<unsigned char>& scratch, DevBuf<double>& coefs) {
     if (reqs.empty()) return;
-    if (dtype == QG_F64) batch_apply_t<double>(reqs, s, scratch);
-    else batch_apply_t<float>(reqs, s, scratch);
+    if (dtype == QG_F64) batch_apply_t<double>(reqs, s, scratch, coefs);
+    else batch_apply_t<float>(reqs, s, scratch, coefs);
 }
 
 // ----------------------------------------------------------------- autotuner ----
@@ -1418,6 +1426,41 @@ static void check_block(const qg_grid* g, int dtype, int nbands, const void* psi
     QG_REQUIRE(psi != out, "in-place application is not supported (psi must differ from out)");
 }
 
+// (Re)build the projector geometry of nl for natoms nuclei with the channels saved at
+// creation, in place: the handle keeps its address, so Hamiltonians built on it see the new
+// positions at their next application (the reference's UpdatePosition on a live integrator,
+// PseudoPotOperator_mpi_v2.h:231-248).  Cached per-plan staging is dropped.
+static void nl_build(qg_nonlocal* nl, int natoms, const double* xyz, const int* species) {
+    for (int a = 0; a < natoms; ++a)
+        QG_REQUIRE(species[a] >= 0 && species[a] < nl->nspecies, "species index out of range");
+    NlGeometry geo;
+    build_nl_geometry(nl->gcopy, natoms, xyz, species, nl->nspecies, nl->nproj.data(), nl->specs.data(), geo);
+    nl->geo = std::move(geo);
+    const auto& gm = nl->geo;
+    if (!gm.projs.empty()) nl->d_projs.upload(gm.projs.data(), gm.projs.size());
+    if (!gm.tables.empty()) nl->d_tables.upload(gm.tables.data(), gm.tables.size());
+    if (!gm.images.empty()) nl->d_images.upload(gm.images.data(), gm.images.size());
+    if (!gm.intervals.empty()) nl->d_intervals.upload(gm.intervals.data(), gm.intervals.size());
+    if (!gm.gidx.empty()) {
+        nl->d_gidx.upload(gm.gidx.data(), gm.gidx.size());
+        nl->d_beta64.upload(gm.beta.data(), gm.beta.size());
+        std::vector<float> b32(gm.beta.begin(), gm.beta.end());
+        nl->d_beta32.upload(b32.data(), b32.size());
+    }
+    nl->d_img_tbl_off.upload(gm.img_tbl_off.data(), gm.img_tbl_off.size());
+    if (!gm.coef_energy.empty()) nl->d_coef_energy.upload(gm.coef_energy.data(), gm.coef_energy.size());
+    nl->d_atom_img_off.upload(gm.atom_img_off.data(), gm.atom_img_off.size());
+    nl->d_atom_coef_off.upload(gm.atom_coef_off.data(), gm.atom_coef_off.size());
+    for (auto& t : nl->tiles) t.ty = t.tz = -1;
+    nl->rows.clear();
+    // Gaussian projectors can be evaluated inside the projector pass instead of read from
+    // per-point tables (QG_NL_ANALYTIC=0 keeps the tables everywhere)
+    bool gauss = !gm.projs.empty();
+    for (const DevProj& d : gm.projs) gauss = gauss && d.kind == QG_RADIAL_GAUSSIAN && d.l <= 1;
+    nl->analytic_level = env_int("QG_NL_ANALYTIC", 1);
+    nl->analytic = gauss && nl->gcopy.n[0] <= 1024 && nl->analytic_level != 0;
+}
+
 extern "C" {
 
 int qg_laplacian8_apply(const qg_grid* g, int dtype, int nbands, const void* psi, int64_t ld_psi,
@@ -1446,33 +1489,37 @@ int qg_nonlocal_create(const qg_grid* g, int natoms, const double* xyz, const in
         try {
             nl->gcopy = *g;
             nl->grid = &nl->gcopy;
-            build_nl_geometry(*g, natoms, xyz, species, nspecies, nproj, specs, nl->geo);
-            const auto& geo = nl->geo;
-            if (!geo.projs.empty()) nl->d_projs.upload(geo.projs.data(), geo.projs.size());
-            if (!geo.tables.empty()) nl->d_tables.upload(geo.tables.data(), geo.tables.size());
-            if (!geo.images.empty()) nl->d_images.upload(geo.images.data(), geo.images.size());
-            if (!geo.intervals.empty()) nl->d_intervals.upload(geo.intervals.data(), geo.intervals.size());
-            if (!geo.gidx.empty()) {
-                nl->d_gidx.upload(geo.gidx.data(), geo.gidx.size());
-                nl->d_beta64.upload(geo.beta.data(), geo.beta.size());
-                std::vector<float> b32(geo.beta.begin(), geo.beta.end());
-                nl->d_beta32.upload(b32.data(), b32.size());
+            nl->nspecies = nspecies;
+            int total = 0;
+            for (int sp = 0; sp < nspecies; ++sp) {
+                QG_REQUIRE(nproj[sp] >= 0, "negative projector count");
+                nl->nproj.push_back(nproj[sp]);
+                total += nproj[sp];
             }
-            nl->d_img_tbl_off.upload(geo.img_tbl_off.data(), geo.img_tbl_off.size());
-            if (!geo.coef_energy.empty()) nl->d_coef_energy.upload(geo.coef_energy.data(), geo.coef_energy.size());
-            nl->d_atom_img_off.upload(geo.atom_img_off.data(), geo.atom_img_off.size());
-            nl->d_atom_coef_off.upload(geo.atom_coef_off.data(), geo.atom_coef_off.size());
-            // Gaussian projectors can be evaluated inside the projector pass instead of read
-            // from per-point tables (QG_NL_ANALYTIC=0 keeps the tables everywhere)
-            bool gauss = !geo.projs.empty();
-            for (const DevProj& d : geo.projs) gauss = gauss && d.kind == QG_RADIAL_GAUSSIAN && d.l <= 1;
-            nl->analytic_level = env_int("QG_NL_ANALYTIC", 1);
-            nl->analytic = gauss && g->n[0] <= 1024 && nl->analytic_level != 0;
+            nl->specs.assign(specs, specs + total);
+            nl->spec_tables.resize(total);
+            for (int k = 0; k < total; ++k)
+                if (specs[k].radial_kind == QG_RADIAL_TABLE && specs[k].table && specs[k].n_table > 0) {
+                    nl->spec_tables[k].assign(specs[k].table, specs[k].table + specs[k].n_table);
+                    nl->specs[k].table = nl->spec_tables[k].data();
+                }
+            nl_build(nl, natoms, xyz, species);
         } catch (...) {
             delete nl;
             throw;
         }
         *out = nl;
+    });
+}
+
+int qg_nonlocal_update(qg_nonlocal* nl, int natoms, const double* xyz, const int* species) {
+    return guarded([&] {
+        QG_REQUIRE(nl, "null nonlocal handle");
+        QG_REQUIRE(natoms >= 0, "negative atom count");
+        QG_REQUIRE(natoms == 0 || (xyz && species), "null arrays");
+        // device work that may still read the old geometry finishes first
+        QG_HIP(hipDeviceSynchronize());
+        nl_build(nl, natoms, xyz, species);
     });
 }
 
@@ -1593,9 +1640,10 @@ int qg_batch_hamiltonian_apply(int ncells, qg_ham* const* hams, int dtype, const
         }
         // descriptor upload area of this entry point (kept for the life of the process)
         static DevBuf<unsigned char>* scratch = new DevBuf<unsigned char>();
+        static DevBuf<double>* coefs = new DevBuf<double>();
         static std::mutex mu;
         std::lock_guard<std::mutex> lk(mu);
-        batch_hamiltonian_apply(reqs, dtype, as_stream(stream), *scratch);
+        batch_hamiltonian_apply(reqs, dtype, as_stream(stream), *scratch, *coefs);
     });
 }
 

@@ -7,6 +7,7 @@
 #include <vector>
 #include <cstdlib>
 #include <memory>
+#include <mutex>
 #include "../../include/qumasun_gpu.h"
 
 namespace qg {
@@ -197,6 +198,11 @@ struct RowsScratch {
 struct qg_nonlocal {
     const qg_grid* grid = nullptr;
     qg_grid gcopy;
+    // the projector channels given at creation (tables copied), reused by qg_nonlocal_update
+    int nspecies = 0;
+    std::vector<int> nproj;
+    std::vector<qg_proj_spec> specs;
+    std::vector<std::vector<double>> spec_tables;
     qg::NlGeometry geo;
     qg::DevBuf<qg::DevProj> d_projs;
     qg::DevBuf<double> d_tables;
@@ -230,6 +236,7 @@ struct qg_nonlocal {
     qg::DevBuf<int64_t> d_img_tbl_off;
     // row-kernel geometry, one per (tile rows, projection slab) shape in use
     std::vector<std::unique_ptr<qg::RowsGeo>> rows;
+    std::mutex rows_mu;             // guards the creation of entries of `rows`
     qg::DevBuf<double> rows_part;   // projector partials of qg_nonlocal_project
 };
 
@@ -260,6 +267,7 @@ struct qg_ham {
     bool timed_valid = false;
     std::vector<hipEvent_t> tev;   // per band chunk: before projection, before fused, after fused
     qg::DevBuf<double> rows_part;  // projector partials of the row path
+    qg::DevBuf<double> rows_coef;  // projector coefficients of the row path
     int tchunks = 0;
     ~qg_ham() {
         for (auto& e : tev)

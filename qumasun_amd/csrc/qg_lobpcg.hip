@@ -45,7 +45,7 @@ struct BatchReq {
     void* out;
 };
 void batch_hamiltonian_apply(const std::vector<BatchReq>& reqs, int dtype, hipStream_t s,
-                             DevBuf<unsigned char>& scratch);
+                             DevBuf<unsigned char>& scratch, DevBuf<double>& coefs);
 
 // Rendezvous of the cells of a batched solve: each cell's solver thread posts its H psi
 // request and waits; the last thread to arrive runs every pending request as one batched
@@ -61,19 +61,23 @@ struct BatchService {
         int dtype;
     };
     std::vector<Req> pending;
-    std::string error;
+    std::string error;   // failure of the last generation only (cleared when the next one runs)
     DevBuf<unsigned char> scratch;
+    DevBuf<double> coefs;
     int launches = 0;
 
-    // runs with mu held; every active thread is waiting, so pending is stable
+    // runs with mu held; every active thread is waiting, so pending is stable.  A waiting
+    // thread wakes before any later generation can run (that needs its own next request),
+    // so the error it reads is its own generation's.
     void execute(hipStream_t s) {
+        error.clear();
         try {
             for (int dt : {QG_F32, QG_F64}) {
                 std::vector<BatchReq> v;
                 for (auto& q : pending)
                     if (q.dtype == dt) v.push_back(q.r);
                 if (!v.empty()) {
-                    batch_hamiltonian_apply(v, dt, s, scratch);
+                    batch_hamiltonian_apply(v, dt, s, scratch, coefs);
                     ++launches;
                 }
             }

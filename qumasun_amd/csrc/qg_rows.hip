@@ -669,6 +669,7 @@ static RowsShape rows_shape(const qg_grid& g, int dtype, int nbands) {
 }
 
 static RowsGeo* rows_geometry(qg_nonlocal* nl, int ty, int tzp) {
+    std::lock_guard<std::mutex> lk(nl->rows_mu);
     for (auto& c : nl->rows)
         if (c && c->ty == ty && c->tzp == tzp) return c.get();
     auto c = std::make_unique<RowsGeo>();
@@ -842,9 +843,9 @@ static bool rows_apply_t(qg_ham* h, int nbands, const T* psi, int64_t ld, T* hps
             coef = scratch->base + npart;
         } else {
             h->rows_part.alloc(npart);
-            nl->d_coef.alloc(ncoef);
+            h->rows_coef.alloc(ncoef);
             part = h->rows_part.p;
-            coef = nl->d_coef.p;
+            coef = h->rows_coef.p;
         }
     }
     if (h->timing) QG_HIP(hipEventRecord(h->tev[0], s));

@@ -224,3 +224,155 @@ def test_hamiltonian_ragged_padded(gpu, oracle, prec, nb):
     err = rel(got, ref)
     assert err <= (TOL64 if prec == "f64" else TOL32), err
     assert bool((out[:, ng:] == 7.0).all())
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_update_position_with_live_hamiltonian(gpu, oracle, prec):
+    """UpdatePosition on an integrator that a Hamiltonian was built on (the reference moves the
+    nuclei of its one integrator at every MD / relaxation step, qumasun_mpi.move.h:15): the
+    handle is rebuilt in place, and the Hamiltonian's next application uses the new positions
+    (oracle parity for the last set)."""
+    import torch
+    print(f"update position prec={prec}", flush=True)
+    s = _system(1, 40)
+    n = (s["n"],) * 3
+    h = (s["h"],) * 3
+    ng = n[0] ** 3
+    g = Q.Grid(*n, *h)
+    integ = Q.PseudoPotIntegrator(g, s["projectors"])
+    integ.UpdatePosition(s["positions"])
+    H = Q.Hamiltonian(g, s["V"], integ)
+    dt = torch.float64 if prec == "f64" else torch.float32
+    X = Q.random_states(ng, 5, h[0] ** 3)
+    xt = _t(X, dt, gpu)
+    out = torch.empty_like(xt)
+    H.apply(xt, out)
+    rng = np.random.default_rng(5)
+    pos = s["positions"]
+    for step in range(2):
+        pos = pos + rng.uniform(-0.3, 0.3, pos.shape)
+        integ.UpdatePosition(pos)
+        H.apply(xt, out)
+    torch.cuda.synchronize()
+    onl = oracle.OracleNL(n, h, pos, np.zeros(len(pos), np.int32), s["projectors"])
+    ref = oracle.hamiltonian_block(n, h, s["V"], onl, X)
+    err = rel(_np(out), ref)
+    assert err <= (TOL64 if prec == "f64" else TOL32), err
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_batch_two_hamiltonians_one_integrator(gpu, oracle, prec):
+    """Two Hamiltonians (different local potentials, e.g. two spin channels) built on one
+    integrator, applied in one batched launch: each equals its own application and the oracle."""
+    import torch
+    print(f"batch shared integrator prec={prec}", flush=True)
+    s = _system(1, 40)
+    n = (s["n"],) * 3
+    h = (s["h"],) * 3
+    ng = n[0] ** 3
+    g = Q.Grid(*n, *h)
+    integ = Q.PseudoPotIntegrator(g, s["projectors"])
+    integ.UpdatePosition(s["positions"])
+    V2 = s["V"] + 0.3 * np.sin(np.arange(ng) * 0.01)
+    Ha = Q.Hamiltonian(g, s["V"], integ)
+    Hb = Q.Hamiltonian(g, V2, integ)
+    dt = torch.float64 if prec == "f64" else torch.float32
+    Xa = Q.random_states(ng, 4, h[0] ** 3, seed=11)
+    Xb = Q.random_states(ng, 6, h[0] ** 3, seed=12)
+    xa, xb = _t(Xa, dt, gpu), _t(Xb, dt, gpu)
+    ya, yb = torch.empty_like(xa), torch.empty_like(xb)
+    Q.batch_apply([Ha, Hb], [xa, xb], [ya, yb])
+    torch.cuda.synchronize()
+    onl = oracle.OracleNL(n, h, s["positions"], np.zeros(len(s["positions"]), np.int32), s["projectors"])
+    tol = TOL64 if prec == "f64" else TOL32
+    assert rel(_np(ya), oracle.hamiltonian_block(n, h, s["V"], onl, Xa)) <= tol
+    assert rel(_np(yb), oracle.hamiltonian_block(n, h, V2, onl, Xb)) <= tol
+
+
+def _log_table(l, energy, sigma, cutoff, xi_min, dxi, npts):
+    import math
+    amp = math.sqrt(4.0 / (math.sqrt(math.pi) * sigma ** 3))
+    xi = xi_min + dxi * np.arange(npts)
+    vals = amp * np.exp(-np.exp(2 * xi) / (2 * sigma * sigma))
+    return Q.table_projector(l, energy, cutoff, xi_min, dxi, vals)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_table_projectors_parity(gpu, oracle, prec):
+    """Tabulated radial projectors (QG_RADIAL_TABLE, a Gaussian sampled on a log grid): H psi and
+    the projector coefficients against the oracle.  The tables are short enough that sphere
+    points fall in every interpolation branch of GetValueByXi (RadialGrid2.h:81-136), below
+    r_min (table[0]) and beyond the last table point (zero) inside the cutoff; a second species
+    has s and p channels of different cutoffs, and one nucleus sits on a grid point."""
+    import torch
+    print(f"table projectors prec={prec}", flush=True)
+    import math
+    n = (28, 24, 32)
+    L = (6.0, 5.6, 6.4)
+    h = tuple(L[d] / n[d] for d in range(3))
+    projs = [[_log_table(0, 1.0, 0.6, 2.4, math.log(0.3), 0.25, 9), _log_table(1, -0.5, 0.6, 2.4, math.log(0.3), 0.25, 9)],
+             [_log_table(1, 0.8, 0.5, 2.0, math.log(0.2), 0.3, 12), _log_table(0, -0.3, 0.7, 1.6, math.log(0.25), 0.2, 10)]]
+    pos = np.array([[0.0, 0.0, 0.0], [2.9137, 3.0411, 2.8213], [10 * h[0], 12 * h[1], 9 * h[2]], [5.7, 0.3, 6.1]])
+    species = np.array([0, 1, 1, 0], np.int32)
+    ng = n[0] * n[1] * n[2]
+    rng = np.random.default_rng(21)
+    V = rng.uniform(-1.0, 0.5, ng)
+    onl = oracle.OracleNL(n, h, pos, species, projs)
+    X = Q.random_states(ng, 6, h[0] * h[1] * h[2], seed=77)
+    ref = oracle.hamiltonian_block(n, h, V, onl, X)
+    g = Q.Grid(*n, *h)
+    integ = Q.PseudoPotIntegrator(g, projs)
+    integ.UpdatePosition(pos, species)
+    H = Q.Hamiltonian(g, V, integ)
+    dt = torch.float64 if prec == "f64" else torch.float32
+    xt = _t(X, dt, gpu)
+    out = torch.empty_like(xt)
+    H.apply(xt, out)
+    coef = integ.InnerNonlocalPsi(xt).cpu().numpy()
+    torch.cuda.synchronize()
+    tol = TOL64 if prec == "f64" else TOL32
+    assert rel(_np(out), ref) <= tol
+    coef_ref = np.stack([onl.project(X[b].copy()) for b in range(len(X))])
+    assert rel(coef, coef_ref) <= (TOL64 if prec == "f64" else 2e-6)
+    loc = oracle.hamiltonian_block(n, h, V, None, X)
+    assert rel(_np(out) - loc, ref - loc) <= (1e-11 if prec == "f64" else 2e-5)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("nb", [1, 5, 16])
+def test_rows_general_projector_sets(gpu, oracle, prec, nb):
+    """The row kernels' general projector form (images that do not hold exactly one s and one p
+    channel): two species, one with s + p (cutoff 2.4) and one with p first and a separate
+    s cutoff (2.0 / 1.6, so an atom has two images of different channel sets), an orthorhombic
+    cell with x-rows of whole 16-byte vectors, atoms on cell faces and one on a grid point."""
+    import torch
+    print(f"rows general prec={prec} nb={nb}", flush=True)
+    n = (32, 28, 24)
+    L = (6.4, 5.6, 5.1)
+    h = tuple(L[d] / n[d] for d in range(3))
+    pos = np.array([[0.0, 0.0, 0.0], [3.1, 2.7, 2.45], [6.35, 0.2, 5.0], [10 * h[0], 7 * h[1], 3 * h[2]],
+                    [4.4, 1.3, 3.3]])
+    species = np.array([0, 1, 0, 1, 1], np.int32)
+    projs = [[Q.gaussian_projector(0, 1.0, 0.6, 2.4), Q.gaussian_projector(1, -0.5, 0.6, 2.4)],
+             [Q.gaussian_projector(1, 0.8, 0.5, 2.0), Q.gaussian_projector(0, -0.3, 0.7, 1.6)]]
+    ng = n[0] * n[1] * n[2]
+    rng = np.random.default_rng(31)
+    V = rng.uniform(-1.0, 0.5, ng)
+    onl = oracle.OracleNL(n, h, pos, species, projs)
+    X = Q.random_states(ng, nb, h[0] * h[1] * h[2], seed=5 + nb)
+    ref = oracle.hamiltonian_block(n, h, V, onl, X)
+    g = Q.Grid(*n, *h)
+    integ = Q.PseudoPotIntegrator(g, projs)
+    integ.UpdatePosition(pos, species)
+    H = Q.Hamiltonian(g, V, integ)
+    dt = torch.float64 if prec == "f64" else torch.float32
+    xt = _t(X, dt, gpu)
+    out = torch.empty_like(xt)
+    H.apply(xt, out)
+    coef = integ.InnerNonlocalPsi(xt).cpu().numpy()
+    torch.cuda.synchronize()
+    assert rel(_np(out), ref) <= (TOL64 if prec == "f64" else TOL32)
+    coef_ref = np.stack([onl.project(X[b].copy()) for b in range(nb)])
+    assert rel(coef, coef_ref) <= (TOL64 if prec == "f64" else 2e-6)
+    loc = oracle.hamiltonian_block(n, h, V, None, X)
+    assert rel(_np(out) - loc, ref - loc) <= (1e-11 if prec == "f64" else 2e-5)
