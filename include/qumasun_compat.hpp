@@ -1,21 +1,38 @@
-// qumasun_compat.hpp — C++ call surface of QUMASUN's H·psi + LOBPCG path on top of the C ABI
-// (qumasun_gpu.h), so the reference's call sites can switch to the MI355X engine with the
-// same argument meaning.  Header-only; errors become std::runtime_error (the reference
-// returns void and prints; the engine reports every failure).
+// qumasun_compat.hpp — the reference's C++ call surface of QUMASUN's H·psi + LOBPCG path on
+// top of the C ABI (qumasun_gpu.h), so QUMASUN call sites switch to the MI355X engine with
+// the same arguments and meaning.  Header-only; errors become std::runtime_error (the
+// reference returns void and prints; the engine reports every failure).
 //
-//   Laplasian8th<T>(GridRange, out, p, coef, dx, dy, dz)   GridDifference8th.h:177-185
-//   PseudoPotIntegratorGPU::UpdatePosition / ProjectionPP   PseudoPotOperator_mpi_v2.h:231, :465
-//   PseudoPotIntegratorGPU::ProjectionPP_bundle             PseudoPotOperator_mpi_v2.h:789
-//   HamiltonianGPU::Apply  (mHamiltonianMatrix_ddm)         qumasun_mpi.matrix.h:21-43
-//   LOBPCG::Eigen_d_multi                                   lobpcg_d_multi_mpi.h:882-890
+//   Laplasian8th<T>(l_grid, out, p, coef, dx, dy, dz)               GridDifference8th.h:177-185
+//   PseudoPotIntegratorGPU::InitializeGrid / Load / UpdatePosition   PseudoPotOperator.h:93,
+//                                                                    PseudoPotOperator_vlocal.h:80,
+//                                                                    PseudoPotOperator_mpi_v2.h:231
+//   PseudoPotIntegratorGPU::ProjectionPP(Hp, psi, l_grid, nuclei, n) PseudoPotOperator_mpi_v2.h:465
+//   PseudoPotIntegratorGPU::ProjectionPP_bundle                      PseudoPotOperator_mpi_v2.h:789
+//   HamiltonianGPU (mHamiltonianMatrix_ddm), usable as OpeA          qumasun_mpi.matrix.h:21-43
+//   LOBPCG::Eigen_d_multi_mpi(l_grid, dVol, N, iter_max, ev, X, keep_P, work,
+//                             keep_S_matrix, SCF_step, OpeA, watch)  lobpcg_d_multi_mpi.h:882-890
+//   EigenLOBPCG_d_multi(N, dVol, num_solution, iter_max, ev, X, keep, work,
+//                       is_first_time, OpeA, watch)                  lobpcg_d_multi.h:23-27
+//   GramSchmidt(N, num_solution, psi, dVol)                          GramSchmidt.h:6-38
 //
-// Pointers named *_dev are device memory; the *Host overloads stage host arrays through
-// device memory (one copy each way) for drop-in use from host-resident code.
+// Memory: every array argument may be host or device memory (checked per call with
+// hipPointerGetAttributes).  Host arrays are staged through device memory, one copy each way,
+// as a drop-in for host-resident code; device arrays are used in place.  A user operator OpeA
+// receives pointers in the memory space of the caller's eigen_vectors.
+// Grid types: any type with SizeX()/SizeY()/SizeZ()/Size3D() (the reference's GridRange and
+// GridRangeMPI both qualify); one process holds the whole periodic grid.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdint>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <tuple>
+#include <type_traits>
 #include <vector>
 #include "qumasun_gpu.h"
 
@@ -28,8 +45,7 @@ inline void hip_check(hipError_t e, const char* what) {
     if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
 }
 
-// The reference's box of grid indices [begin, end) (GridRange.h:14-46); one process holds the
-// whole periodic grid, so begin_* = 0.
+// The reference's box of grid indices [begin, end) (GridRange.h:14-46).
 struct GridRange {
     int begin_x, begin_y, begin_z;
     int end_x, end_y, end_z;
@@ -49,12 +65,56 @@ template <class T> struct dtype_of;
 template <> struct dtype_of<double> { static constexpr int value = QG_F64; };
 template <> struct dtype_of<float> { static constexpr int value = QG_F32; };
 
+// Whether p is device memory (allocations of hipMalloc / hipMallocManaged).
+inline bool is_device_ptr(const void* p) {
+    if (!p) return false;
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();   // plain host memory is not registered: clear the error
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+// Device copy of a caller array, copied back on sync_back() when the caller's is host memory.
+template <class T>
+class Staged {
+public:
+    Staged(T* user, size_t n, bool copy_in) : user_(user), n_(n) {
+        if (is_device_ptr(user) || n == 0) {
+            dev_ = user;
+            return;
+        }
+        hip_check(hipMalloc(&dev_, n * sizeof(T)), "hipMalloc");
+        own_ = true;
+        if (copy_in) hip_check(hipMemcpy(dev_, user, n * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy");
+    }
+    ~Staged() {
+        if (own_) (void)hipFree(dev_);
+    }
+    Staged(const Staged&) = delete;
+    Staged& operator=(const Staged&) = delete;
+    T* get() const { return dev_; }
+    bool host() const { return own_; }
+    void sync_back() {
+        if (own_) hip_check(hipMemcpy(user_, dev_, n_ * sizeof(T), hipMemcpyDeviceToHost), "hipMemcpy");
+    }
+
+private:
+    T* user_;
+    T* dev_ = nullptr;
+    size_t n_;
+    bool own_ = false;
+};
+
 // RAII grid handle
 class Grid {
 public:
-    Grid(const GridRange& g, double dx, double dy, double dz) {
-        check(qg_grid_create(g.SizeX(), g.SizeY(), g.SizeZ(), dx, dy, dz, &h_), "qg_grid_create");
+    Grid(int nx, int ny, int nz, double dx, double dy, double dz) {
+        check(qg_grid_create(nx, ny, nz, dx, dy, dz, &h_), "qg_grid_create");
     }
+    template <class G>
+    Grid(const G& g, double dx, double dy, double dz) : Grid(g.SizeX(), g.SizeY(), g.SizeZ(), dx, dy, dz) {}
     ~Grid() { qg_grid_destroy(h_); }
     Grid(const Grid&) = delete;
     Grid& operator=(const Grid&) = delete;
@@ -64,58 +124,72 @@ private:
     qg_grid* h_ = nullptr;
 };
 
-// out += coef * Lap8 p (periodic), one band, device pointers — Laplasian8th<T>.
-template <class T>
-void Laplasian8th(const GridRange& g, T* out_dev, const T* p_dev, double coef, double dx, double dy, double dz,
-                  hipStream_t stream = nullptr) {
-    Grid grid(g, dx, dy, dz);
-    const int64_t ng = (int64_t)g.Size3D();
-    check(qg_laplacian8_apply(grid.get(), dtype_of<T>::value, 1, p_dev, ng, out_dev, ng, coef, 1, stream),
-          "qg_laplacian8_apply");
-    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+// Grid handles of the free functions, kept per (shape, spacing) for the life of the process.
+inline qg_grid* cached_grid(int nx, int ny, int nz, double dx, double dy, double dz) {
+    static std::mutex mu;
+    static std::map<std::tuple<int, int, int, double, double, double>, std::unique_ptr<Grid>> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto& g = cache[std::make_tuple(nx, ny, nz, dx, dy, dz)];
+    if (!g) g.reset(new Grid(nx, ny, nz, dx, dy, dz));
+    return g->get();
 }
 
-// Host-pointer overload (stages through device memory).
-template <class T>
-void Laplasian8thHost(const GridRange& g, T* out, const T* p, double coef, double dx, double dy, double dz) {
-    const size_t bytes = g.Size3D() * sizeof(T);
-    T *dp = nullptr, *dout = nullptr;
-    hip_check(hipMalloc(&dp, bytes), "hipMalloc");
-    hip_check(hipMalloc(&dout, bytes), "hipMalloc");
-    hip_check(hipMemcpy(dp, p, bytes, hipMemcpyHostToDevice), "hipMemcpy");
-    hip_check(hipMemcpy(dout, out, bytes, hipMemcpyHostToDevice), "hipMemcpy");
-    Laplasian8th<T>(g, dout, dp, coef, dx, dy, dz);
-    hip_check(hipMemcpy(out, dout, bytes, hipMemcpyDeviceToHost), "hipMemcpy");
-    (void)hipFree(dp);
-    (void)hipFree(dout);
+// out += coef * Lap8 p (periodic) for one band: Laplasian8th<T>(l_grid, out, p, coef, dx, dy,
+// dz).  Device arrays run on the null stream without a host wait (HIP order); host arrays are
+// staged and the call returns with out updated.
+template <class T, class G>
+void Laplasian8th(const G& l_grid, T* out, const T* p, double coef, double dx, double dy, double dz) {
+    qg_grid* g = cached_grid(l_grid.SizeX(), l_grid.SizeY(), l_grid.SizeZ(), dx, dy, dz);
+    const size_t ng = (size_t)l_grid.Size3D();
+    Staged<T> so(out, ng, true);
+    Staged<T> sp(const_cast<T*>(p), ng, true);
+    check(qg_laplacian8_apply(g, dtype_of<T>::value, 1, sp.get(), (int64_t)ng, so.get(), (int64_t)ng, coef, 1,
+                              nullptr),
+          "qg_laplacian8_apply");
+    so.sync_back();
+}
+
+// Host-array form kept for existing callers of this header.
+template <class T, class G>
+void Laplasian8thHost(const G& g, T* out, const T* p, double coef, double dx, double dy, double dz) {
+    Laplasian8th<T>(g, out, p, coef, dx, dy, dz);
 }
 
 // Nonlocal KB projectors for one process holding the whole grid (PseudoPotIntegrator_mpi).
-// Load() takes the projector channels of an element directly (the .vps loader is not part of
-// this engine): qg_proj_spec with a Gaussian or a log-grid table radial part.
+// Load() takes the projector channels of an element directly (qg_proj_spec, Gaussian or log-grid
+// table radial part); LoadVps() reads them from an MBK .vps file (qg_vps_*).
 class PseudoPotIntegratorGPU {
 public:
-    ~PseudoPotIntegratorGPU() { release(); }
-    void InitializeGrid(const GridRange& g, double dx, double dy, double dz) {
-        g_ = g;
+    ~PseudoPotIntegratorGPU() {
+        if (nl_) qg_nonlocal_destroy(nl_);
+    }
+    // InitializeGrid(grid, dx, dy, dz) (PseudoPotOperator.h:93)
+    template <class G>
+    void InitializeGrid(const G& g, double dx, double dy, double dz) {
+        nx_ = g.SizeX(); ny_ = g.SizeY(); nz_ = g.SizeZ();
         d_[0] = dx; d_[1] = dy; d_[2] = dz;
-        grid_.reset(new Grid(g, dx, dy, dz));
+        grid_.reset(new Grid(nx_, ny_, nz_, dx, dy, dz));
     }
     void Load(int Z, const std::vector<qg_proj_spec>& projectors) {
+        if (nl_) throw std::runtime_error("Load before the first UpdatePosition");
         for (auto& e : elements_)
             if (e.first == Z) { e.second = projectors; return; }
         elements_.emplace_back(Z, projectors);
     }
-    // UpdatePosition(nuclei, n, l_grid) (PseudoPotOperator_mpi_v2.h:231-248)
+    // UpdatePosition(nuclei, num_nuclei, l_grid) (PseudoPotOperator_mpi_v2.h:231-248): the first
+    // call builds the projector geometry, later calls move the nuclei of the same handle (every
+    // HamiltonianGPU built on this integrator follows).
+    template <class G>
+    void UpdatePosition(const Nucleus* nuclei, int num_nuclei, const G& l_grid) {
+        if (!grid_) throw std::runtime_error("InitializeGrid first");
+        if (l_grid.SizeX() != nx_ || l_grid.SizeY() != ny_ || l_grid.SizeZ() != nz_)
+            throw std::runtime_error("UpdatePosition: grid differs from InitializeGrid");
+        UpdatePosition(nuclei, num_nuclei);
+    }
     void UpdatePosition(const Nucleus* nuclei, int n) {
         if (!grid_) throw std::runtime_error("InitializeGrid first");
         std::vector<double> xyz(3 * (size_t)n);
-        std::vector<int> species(n), nproj;
-        std::vector<qg_proj_spec> specs;
-        for (auto& e : elements_) {
-            nproj.push_back((int)e.second.size());
-            specs.insert(specs.end(), e.second.begin(), e.second.end());
-        }
+        std::vector<int> species(n);
         for (int a = 0; a < n; ++a) {
             xyz[3 * a] = nuclei[a].Rx; xyz[3 * a + 1] = nuclei[a].Ry; xyz[3 * a + 2] = nuclei[a].Rz;
             species[a] = -1;
@@ -123,39 +197,59 @@ public:
                 if (elements_[s].first == nuclei[a].Z) species[a] = (int)s;
             if (species[a] < 0) throw std::runtime_error("ERROR: VPS is not loaded: Z = " + std::to_string(nuclei[a].Z));
         }
-        release();
-        check(qg_nonlocal_create(grid_->get(), n, xyz.data(), species.data(), (int)nproj.size(), nproj.data(),
-                                 specs.data(), &nl_),
-              "qg_nonlocal_create");
+        if (nl_) {
+            check(qg_nonlocal_update(nl_, n, xyz.data(), species.data()), "qg_nonlocal_update");
+        } else {
+            std::vector<int> nproj;
+            std::vector<qg_proj_spec> specs;
+            for (auto& e : elements_) {
+                nproj.push_back((int)e.second.size());
+                specs.insert(specs.end(), e.second.begin(), e.second.end());
+            }
+            check(qg_nonlocal_create(grid_->get(), n, xyz.data(), species.data(), (int)nproj.size(), nproj.data(),
+                                     specs.data(), &nl_),
+                  "qg_nonlocal_create");
+        }
+        natoms_ = n;
     }
-    // Hp += V_NL psi for one band (device pointers) — ProjectionPP.
-    void ProjectionPP(double* Hp_dev, const double* psi_dev, hipStream_t stream = nullptr) {
-        ProjectionPP_bundle(1, Hp_dev, psi_dev, stream);
+    // ProjectionPP(Hp, psi, l_grid, nuclei, num_nuclei) (PseudoPotOperator_mpi_v2.h:465):
+    // Hp += V_NL psi for one band, with the nuclei of the last UpdatePosition.
+    template <class G>
+    void ProjectionPP(double* Hp, const double* psi, const G& l_grid, const Nucleus* nuclei, int num_nuclei) {
+        (void)nuclei;
+        if ((size_t)l_grid.Size3D() != size())
+            throw std::runtime_error("ProjectionPP: grid differs from InitializeGrid");
+        if (num_nuclei != natoms_) throw std::runtime_error("ProjectionPP: nuclei differ from UpdatePosition");
+        ProjectionPP_bundle(1, Hp, psi);
     }
-    // Band-block form; band-major layout (the engine reads band-major blocks directly, so the
-    // reference's transposes around ProjectionPP_bundle are not needed).
-    void ProjectionPP_bundle(int nbands, double* Hp_dev, const double* psi_dev, hipStream_t stream = nullptr) {
-        const int64_t ng = (int64_t)g_.Size3D();
-        check(qg_nonlocal_apply(nl_, QG_F64, nbands, psi_dev, ng, Hp_dev, ng, 1, stream), "qg_nonlocal_apply");
-        hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    // Band-block form on band-major blocks (the engine reads band-major blocks directly, so the
+    // reference's transposes around ProjectionPP_bundle, qumasun_mpi.matrix.h:60-66, are not
+    // needed).
+    void ProjectionPP_bundle(int nbands, double* Hp, const double* psi) {
+        if (!nl_) throw std::runtime_error("UpdatePosition first");
+        const size_t ng = size();
+        Staged<double> sh(Hp, ng * nbands, true);
+        Staged<double> sp(const_cast<double*>(psi), ng * nbands, true);
+        check(qg_nonlocal_apply(nl_, QG_F64, nbands, sp.get(), (int64_t)ng, sh.get(), (int64_t)ng, 1, nullptr),
+              "qg_nonlocal_apply");
+        hip_check(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
+        sh.sync_back();
     }
     qg_nonlocal* handle() const { return nl_; }
-    const GridRange& grid() const { return g_; }
     qg_grid* grid_handle() const { return grid_->get(); }
+    size_t size() const { return (size_t)nx_ * ny_ * nz_; }
+    double dvol() const { return d_[0] * d_[1] * d_[2]; }
 
 private:
-    void release() {
-        if (nl_) qg_nonlocal_destroy(nl_);
-        nl_ = nullptr;
-    }
-    GridRange g_{};
+    int nx_ = 0, ny_ = 0, nz_ = 0, natoms_ = 0;
     double d_[3] = {0, 0, 0};
     std::unique_ptr<Grid> grid_;
     std::vector<std::pair<int, std::vector<qg_proj_spec>>> elements_;
     qg_nonlocal* nl_ = nullptr;
 };
 
-// H = V + V_NL - 1/2 Lap8 (QUMASUN_MPI::mHamiltonianMatrix_ddm) for band blocks.
+// H = V + V_NL - 1/2 Lap8 (QUMASUN_MPI::mHamiltonianMatrix_ddm) for band blocks.  As OpeA of
+// Eigen_d_multi_mpi it is applied on the device to the whole block at once.
 class HamiltonianGPU {
 public:
     HamiltonianGPU(PseudoPotIntegratorGPU& pp, const double* Vtot_host) : pp_(pp) {
@@ -165,14 +259,25 @@ public:
     HamiltonianGPU(const HamiltonianGPU&) = delete;
     HamiltonianGPU& operator=(const HamiltonianGPU&) = delete;
     void SetPotential(const double* Vtot_host) { check(qg_hamiltonian_set_potential(h_, Vtot_host), "set_potential"); }
-    // Hp = H p for nbands band-major device columns (FP64 or FP32)
+    // Hp = H p for nbands band-major columns (FP64 or FP32, host or device)
     template <class T>
-    void Apply(int nbands, T* Hp_dev, const T* p_dev, hipStream_t stream = nullptr) {
-        const int64_t ng = (int64_t)pp_.grid().Size3D();
-        check(qg_hamiltonian_apply(h_, dtype_of<T>::value, nbands, p_dev, ng, Hp_dev, ng, stream),
+    void Apply(int nbands, T* Hp, const T* p, hipStream_t stream = nullptr) {
+        const size_t ng = pp_.size();
+        Staged<T> sh(Hp, ng * nbands, false);
+        Staged<T> sp(const_cast<T*>(p), ng * nbands, true);
+        check(qg_hamiltonian_apply(h_, dtype_of<T>::value, nbands, sp.get(), (int64_t)ng, sh.get(), (int64_t)ng,
+                                   stream),
               "qg_hamiltonian_apply");
+        if (sh.host()) {
+            hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+            sh.sync_back();
+        }
     }
+    // OpeA(Ax, x) of the reference (one band)
+    void operator()(double* Ax, double* x) { Apply<double>(1, Ax, x); }
     qg_ham* get() const { return h_; }
+    size_t size() const { return pp_.size(); }
+    double dvol() const { return pp_.dvol(); }
 
 private:
     PseudoPotIntegratorGPU& pp_;
@@ -180,18 +285,134 @@ private:
 };
 
 // GramSchmidt(N, num_solution, psi_set, dVol) (GramSchmidt.h:6-38) / GramSchmidt_ddm
-// (GramSchmidt_mpi.h:9-53) on a num_solution x N FP64 band-major device block, in place.
-inline void GramSchmidt(int64_t N, int num_solution, double* psi_dev, double dVol) {
-    check(qg_gram_schmidt(QG_F64, N, num_solution, psi_dev, N, dVol, nullptr), "qg_gram_schmidt");
+// (GramSchmidt_mpi.h:9-53) on a num_solution x N FP64 band-major block, in place.
+inline void GramSchmidt(int64_t N, int num_solution, double* psi, double dVol) {
+    Staged<double> sp(psi, (size_t)N * num_solution, true);
+    check(qg_gram_schmidt(QG_F64, N, num_solution, sp.get(), N, dVol, nullptr), "qg_gram_schmidt");
+    sp.sync_back();
 }
+
+namespace detail {
+
+template <class WATCH>
+void watch_thunk(void* user, int sector, int64_t count) {
+    static_cast<WATCH*>(user)->Record(sector, count);
+}
+
+// A user operator OpeA(double* Ax, double* x) for one band, called from the engine's operator
+// callback per band, on pointers in the caller's memory space.
+template <class OperationA>
+struct UserOp {
+    OperationA* op;
+    size_t ng;
+    bool host;                    // the caller's arrays are host memory: stage each block
+    std::vector<double> hx, hax;
+    static void call(void* user, int dtype, int nbands, const void* x, void* ax, void* stream) {
+        UserOp* c = static_cast<UserOp*>(user);
+        (void)dtype;   // the compat solvers run in FP64
+        hip_check(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+        const size_t n = c->ng * (size_t)nbands;
+        if (c->host) {
+            c->hx.resize(n);
+            c->hax.resize(n);
+            hip_check(hipMemcpy(c->hx.data(), x, n * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+            for (int k = 0; k < nbands; ++k) (*c->op)(c->hax.data() + k * c->ng, c->hx.data() + k * c->ng);
+            hip_check(hipMemcpy(ax, c->hax.data(), n * 8, hipMemcpyHostToDevice), "hipMemcpy");
+        } else {
+            for (int k = 0; k < nbands; ++k)
+                (*c->op)(static_cast<double*>(ax) + k * c->ng,
+                         const_cast<double*>(static_cast<const double*>(x)) + k * c->ng);
+            hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        }
+    }
+};
+
+// A Hamiltonian of V = 0 without projectors on an (nx, ny, nz) grid of cell dVol: the grid the
+// engine's solver needs when OpeA is a foreign operator.
+struct GridOnlyHam {
+    std::unique_ptr<Grid> g;
+    qg_ham* h = nullptr;
+    GridOnlyHam(int nx, int ny, int nz, double dvol) {
+        const double d = std::cbrt(dvol);
+        g.reset(new Grid(nx, ny, nz, d, d, d));
+        std::vector<double> v((size_t)nx * ny * nz, 0.0);
+        check(qg_hamiltonian_create(g->get(), v.data(), nullptr, &h), "qg_hamiltonian_create");
+    }
+    ~GridOnlyHam() { qg_hamiltonian_destroy(h); }
+};
+
+template <class OperationA>
+inline constexpr bool is_engine_op = std::is_same<std::decay_t<OperationA>, HamiltonianGPU*>::value ||
+                                     std::is_same<std::decay_t<OperationA>, std::reference_wrapper<HamiltonianGPU>>::value;
+
+inline HamiltonianGPU& engine_of(HamiltonianGPU* h) { return *h; }
+inline HamiltonianGPU& engine_of(std::reference_wrapper<HamiltonianGPU> h) { return h.get(); }
+
+// One solve: eigen_vectors / keep_P of num_solution x ng (host or device), OpeA the engine's
+// Hamiltonian (applied to whole blocks on the device) or a per-band user operator.
+template <class OperationA, class WATCH>
+void solve(int nx, int ny, int nz, double dVol, int num_solution, int iter_max, double* eigen_values,
+           double* eigen_vectors, double* keep_P, int SCF_step, double tol, OperationA OpeA, WATCH& watch) {
+    const size_t ng = (size_t)nx * ny * nz;
+    qg_lobpcg_opts o;
+    qg_lobpcg_default_opts(&o);
+    o.max_iter = iter_max;
+    o.tol = tol;
+    o.precision = QG_PREC_F64;
+    o.watch = &watch_thunk<WATCH>;
+    o.watch_user = &watch;
+    Staged<double> sx(eigen_vectors, ng * num_solution, true);
+    Staged<double> sp(keep_P, ng * num_solution, SCF_step != 0);
+    qg_lobpcg_stats st{};
+    if constexpr (is_engine_op<OperationA>) {
+        HamiltonianGPU& H = engine_of(OpeA);
+        if (H.size() != ng) throw std::runtime_error("Eigen_d_multi_mpi: grid differs from the Hamiltonian's");
+        if (std::fabs(H.dvol() - dVol) > 1e-12 * dVol)
+            throw std::runtime_error("Eigen_d_multi_mpi: dVol differs from the Hamiltonian's grid");
+        check(qg_lobpcg_d_multi(H.get(), &o, num_solution, eigen_values, sx.get(), sp.get(), SCF_step, &st),
+              "qg_lobpcg_d_multi");
+    } else {
+        GridOnlyHam gh(nx, ny, nz, dVol);
+        UserOp<OperationA> u{&OpeA, ng, sx.host(), {}, {}};
+        o.op = &UserOp<OperationA>::call;
+        o.op_user = &u;
+        check(qg_lobpcg_d_multi(gh.h, &o, num_solution, eigen_values, sx.get(), sp.get(), SCF_step, &st),
+              "qg_lobpcg_d_multi");
+    }
+    sx.sync_back();
+    sp.sync_back();
+}
+
+}  // namespace detail
 
 namespace LOBPCG {
 
 // Eigen_d_multi_mpi(l_grid, dVol, num_solution, iter_max, eigen_values, eigen_vectors, keep_P,
-// work, keep_S_matrix, SCF_step, OpeA, watch) with OpeA = the device Hamiltonian.  dVol is
-// taken from the grid; the work block is owned by the engine; eigen_vectors_dev and
-// keep_P_dev are num_solution x Ng FP64 device blocks.  Options default to the reference's
-// behaviour (FP64, fixed iter_max, refresh every 10 iterations, rollback at 1e-3).
+// work, keep_S_matrix, SCF_step, OpeA, watch) (lobpcg_d_multi_mpi.h:882-890): the reference's
+// block LOBPCG for iter_max iterations (2N basis on SCF_step 0, refresh every 10 iterations,
+// rollback at |1 - ||x'||^2| > 1e-3), on the device in FP64.
+//   OpeA: a HamiltonianGPU* (or std::ref of one), applied to whole blocks on the device, or any
+//         callable OpeA(double* Ax, double* x) for one band (the per-band lambda of
+//         QUMASUN_MPI::mSolveLOBPCG, qumasun_mpi.lobpcg.h:33-35), given pointers in the memory
+//         space of eigen_vectors.
+//   work, keep_S_matrix: accepted for the reference's signature; the engine keeps its work
+//         blocks and Gram state in the Hamiltonian's solver state (the reference reads
+//         keep_S_matrix only under S_MATRIX_RELOAD, lobpcg_d_multi_mpi.h:46, which it leaves off).
+//   watch: WATCH::Record(sector, count) with the reference's sector ids (10, 11, 15, 16, 17).
+template <class G, class OperationA, class WATCH>
+void Eigen_d_multi_mpi(const G& l_grid, double dVol, int num_solution, int iter_max, double* eigen_values,
+                       double* eigen_vectors, double* keep_P, double* work, double* keep_S_matrix, int SCF_step,
+                       OperationA OpeA, WATCH& watch) {
+    (void)work;
+    (void)keep_S_matrix;
+    detail::solve(l_grid.SizeX(), l_grid.SizeY(), l_grid.SizeZ(), dVol, num_solution, iter_max, eigen_values,
+                  eigen_vectors, keep_P, SCF_step, 0.0, OpeA, watch);
+}
+
+// Work size of Eigen_d_multi_mpi in doubles (lobpcg_d_multi_mpi.h:1575-1577).
+inline size_t WorkSize_d_multi_mpi(int local_size, int num_solution) { return 5 * (size_t)local_size * num_solution; }
+
+// Earlier engine-shaped forms (a Hamiltonian and device blocks), kept for existing callers.
 inline qg_lobpcg_stats Eigen_d_multi(HamiltonianGPU& H, int num_solution, int iter_max, double* eigen_values,
                                      double* eigen_vectors_dev, double* keep_P_dev, int SCF_step,
                                      const qg_lobpcg_opts* opts = nullptr) {
@@ -205,37 +426,55 @@ inline qg_lobpcg_stats Eigen_d_multi(HamiltonianGPU& H, int num_solution, int it
     return st;
 }
 
-// The reference's form with a user operator: OpeA(double* Ax_dev, double* x_dev) applies A to
-// one band (device pointers, FP64), as the per-band lambda of QUMASUN_MPI::mSolveLOBPCG
-// (qumasun_mpi.lobpcg.h:33-35); H supplies the grid.  The solve runs in FP64.
-template <class OperationA>
-inline qg_lobpcg_stats Eigen_d_multi(HamiltonianGPU& H, int num_solution, int iter_max, double* eigen_values,
-                                     double* eigen_vectors_dev, double* keep_P_dev, int SCF_step, OperationA OpeA,
-                                     int64_t grid_size, const qg_lobpcg_opts* opts = nullptr) {
-    struct Ctx {
-        OperationA* op;
-        int64_t ng;
-    } ctx{&OpeA, grid_size};
-    qg_lobpcg_opts o;
-    qg_lobpcg_default_opts(&o);
-    if (opts) o = *opts;
-    o.max_iter = iter_max;
-    o.precision = QG_PREC_F64;
-    o.op_user = &ctx;
-    o.op = [](void* user, int dtype, int nbands, const void* x, void* ax, void* stream) {
-        Ctx* c = static_cast<Ctx*>(user);
-        (void)dtype;
-        (void)hipStreamSynchronize((hipStream_t)stream);
-        for (int k = 0; k < nbands; ++k)
-            (*c->op)(static_cast<double*>(ax) + (int64_t)k * c->ng,
-                     const_cast<double*>(static_cast<const double*>(x)) + (int64_t)k * c->ng);
-        (void)hipDeviceSynchronize();
-    };
-    qg_lobpcg_stats st{};
-    check(qg_lobpcg_d_multi(H.get(), &o, num_solution, eigen_values, eigen_vectors_dev, keep_P_dev, SCF_step, &st),
-          "qg_lobpcg_d_multi");
-    return st;
+}  // namespace LOBPCG
+
+// EigenLOBPCG_d_multi(N, dVol, num_solution, iter_max, eigen_values, eigen_vectors, keep, work,
+// is_first_time, OpeA, watch) (lobpcg_d_multi.h:23-27), the reference's single-process form:
+// N grid points (a cube of side N^(1/3) is assumed when OpeA is a user operator; with a
+// HamiltonianGPU the Hamiltonian's grid is used), keep = the conjugate block, is_first_time
+// = SCF step 0.  Stops as the reference does when every band's dVol-weighted squared residual
+// is below 1e-10 dVol (lobpcg_d_multi.h:59, :533), or after iter_max iterations.
+template <class OperationA, class WATCH>
+void EigenLOBPCG_d_multi(int N, double dVol, int num_solution, int iter_max, double* eigen_values,
+                         double* eigen_vectors, double* keep, double* work, bool is_first_time, OperationA OpeA,
+                         WATCH& watch) {
+    (void)work;
+    int nx, ny, nz;
+    if constexpr (detail::is_engine_op<OperationA>) {
+        (void)N;
+        nx = (int)detail::engine_of(OpeA).size();
+        ny = nz = 1;
+    } else {
+        nx = (int)std::lround(std::cbrt((double)N));
+        if ((int64_t)nx * nx * nx != N) throw std::runtime_error("EigenLOBPCG_d_multi: N is not a cube; use Eigen_d_multi_mpi");
+        ny = nz = nx;
+    }
+    // max_k dVol sum r_k^2 < 1e-10 dVol  <=>  max_k ||r_k|| (dVol-normalised) < sqrt(1e-10 dVol)
+    const double tol = std::sqrt(1.0e-10 * dVol);
+    if constexpr (detail::is_engine_op<OperationA>) {
+        HamiltonianGPU& H = detail::engine_of(OpeA);
+        qg_lobpcg_opts o;
+        qg_lobpcg_default_opts(&o);
+        o.max_iter = iter_max;
+        o.tol = tol;
+        o.watch = &detail::watch_thunk<WATCH>;
+        o.watch_user = &watch;
+        const size_t ng = H.size();
+        Staged<double> sx(eigen_vectors, ng * num_solution, true);
+        Staged<double> sp(keep, ng * num_solution, !is_first_time);
+        qg_lobpcg_stats st{};
+        check(qg_lobpcg_d_multi(H.get(), &o, num_solution, eigen_values, sx.get(), sp.get(), is_first_time ? 0 : 1,
+                                &st),
+              "qg_lobpcg_d_multi");
+        sx.sync_back();
+        sp.sync_back();
+    } else {
+        detail::solve(nx, ny, nz, dVol, num_solution, iter_max, eigen_values, eigen_vectors, keep,
+                      is_first_time ? 0 : 1, tol, OpeA, watch);
+    }
 }
 
-}  // namespace LOBPCG
+// Work size of EigenLOBPCG_d_multi in doubles (lobpcg_d_multi.h:548-551).
+inline size_t WorkSizeLOBPCG_d_multi(int line_size, int num_solution) { return 5 * (size_t)line_size * num_solution; }
+
 }  // namespace qumasun_gpu

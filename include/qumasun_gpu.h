@@ -184,6 +184,15 @@ typedef struct qg_lobpcg_opts {
     int basis;               /* QG_BASIS_*: used for FP64 runs without a preconditioner */
     double fallback_shift;   /* QG_PRECOND_LEARNED: shift of the kinetic preconditioner that
                                 takes over after a stall (0: precond_shift) */
+    /* Optional sector clock, the reference's WATCH::Record(idx, count) (StopWatch.h:34): called
+     * on the solving thread once the device work of a sector has completed, with the sector ids
+     * the reference's solver and operator use (qumasun_mpi.print.h:78-86,
+     * lobpcg_d_multi_mpi.h:978-1571): 10 others (also called when a sector starts, so the time
+     * between sectors is "others"), 11 H psi (kinetic, local and nonlocal in one application;
+     * count = bands), 15 small eigensolve (LAPACK), 16 block updates (GEMM2), 17 Gram
+     * products (GEMM1).  NULL: no calls. */
+    void (*watch)(void* user, int sector, int64_t count);
+    void* watch_user;
 } qg_lobpcg_opts;
 
 typedef struct qg_lobpcg_stats {

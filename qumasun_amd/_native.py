@@ -38,6 +38,8 @@ class ProjSpec(ctypes.Structure):
 
 # void (*op)(void* user, int dtype, int nbands, const void* x, void* ax, void* stream)
 OP_CALLBACK = ctypes.CFUNCTYPE(None, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p)
+# void (*watch)(void* user, int sector, int64_t count)
+WATCH_CALLBACK = ctypes.CFUNCTYPE(None, c_void_p, c_int, c_int64)
 
 
 class LobpcgOpts(ctypes.Structure):
@@ -46,7 +48,7 @@ class LobpcgOpts(ctypes.Structure):
                 ("refresh_period", c_int), ("rollback_limit", c_double), ("precond", c_int),
                 ("precond_shift", c_double), ("verbose", c_int), ("switch_tol", c_double),
                 ("stall_window", c_int), ("op", OP_CALLBACK), ("op_user", c_void_p), ("basis", c_int),
-                ("fallback_shift", c_double)]
+                ("fallback_shift", c_double), ("watch", WATCH_CALLBACK), ("watch_user", c_void_p)]
 
 
 class LobpcgStats(ctypes.Structure):

@@ -62,5 +62,21 @@ def build(verbose=False):
     return lib
 
 
+HARNESS = os.path.join(os.path.dirname(HERE), "tests", "harness")
+COMPAT_DRIVER = os.path.join(LIBDIR, "compat_drive")
+
+
+def compat_driver(rebuild=False):
+    """Build (when missing or asked) the test driver of the reference-shaped C++ surface,
+    tests/harness/compat_drive.cpp, linked against the library; returns its path."""
+    if rebuild or not os.path.exists(COMPAT_DRIVER):
+        inc = os.path.join(os.path.dirname(HERE), "include")
+        tmp = COMPAT_DRIVER + ".tmp"
+        _run([HIPCC, "-O2", "-std=c++17", "-I" + inc, os.path.join(HARNESS, "compat_drive.cpp"), "-o", tmp,
+              "-L" + LIBDIR, "-l:" + LIBNAME, "-Wl,-rpath," + LIBDIR])
+        os.replace(tmp, COMPAT_DRIVER)
+    return COMPAT_DRIVER
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))

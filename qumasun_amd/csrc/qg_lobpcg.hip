@@ -419,9 +419,20 @@ static void run_lobpcg(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_ho
     const int gsz = grid_for(KN);
     auto now = [] { return std::chrono::steady_clock::now(); };
     auto secs = [](auto a, auto b) { return std::chrono::duration<double>(b - a).count(); };
+    // sector clock: "others" up to the start of a sector, then the sector itself
+    // (the reference's WATCH::Record ids)
+    auto sector_begin = [&] {
+        if (o.watch) o.watch(o.watch_user, 10, 1);
+        return now();
+    };
+    auto sector_end = [&](std::chrono::steady_clock::time_point t0, int idx, int64_t count) {
+        const double d = secs(t0, now());
+        if (o.watch) o.watch(o.watch_user, idx, count);
+        return d;
+    };
     auto sync = [&] { QG_HIP(hipStreamSynchronize(s)); };
     auto hpsi = [&](const T* in, T* outp) {
-        auto t0 = now();
+        auto t0 = sector_begin();
         if (o.op)
             o.op(o.op_user, dtype, N, in, outp, (void*)s);
         else if (h->batch)
@@ -429,7 +440,7 @@ static void run_lobpcg(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_ho
         else
             hamiltonian_apply(h, dtype, N, in, K, outp, K, s);
         sync();
-        stats.t_hpsi += secs(t0, now());
+        stats.t_hpsi += sector_end(t0, 11, N);
         stats.hpsi_calls += N;
     };
 
@@ -483,14 +494,14 @@ static void run_lobpcg(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_ho
     double best_res = maxres;
     int best_iter = 0;
     for (iter = 0; iter < o.max_iter && !finished; ++iter) {
-        auto t0 = now();
+        auto t0 = sector_begin();
         if (precond) {
             if (learned)
                 cnn_precond_apply(h->cnn, dtype, N, R, K, s);
             else
                 precond_apply(h, QG_PRECOND_KINETIC, kinetic_shift(o), dtype, N, R, K, s);
             sync();
-            stats.t_precond += secs(t0, now());
+            stats.t_precond += sector_end(t0, 10, 1);
         }
         const bool use_p = p_valid;
         if (robust) {
@@ -505,7 +516,7 @@ static void run_lobpcg(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_ho
         }
         hpsi(R, AR);
         // MakeMatrix_d_blas (:192-250)
-        t0 = now();
+        t0 = sector_begin();
         const bool full = robust || (iter == 0) || (o.refresh_period > 0 && iter % o.refresh_period == 0) || large_diff;
         if (full) {
             op.gram(X, AX, blkS(st.Sa, 0, 0), n3);
@@ -525,10 +536,10 @@ static void run_lobpcg(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_ho
         op.gram(R, AP, blkS(st.Sa, 1, 2), n3);
         op.gram(R, R, blkS(st.Sb, 1, 1), n3, true);
         sync();
-        stats.t_gram += secs(t0, now());
+        stats.t_gram += sector_end(t0, 17, 1);
 
         // Rayleigh-Ritz: dsygvd on the upper nb x nb problem (lapack_DSYGVX, w_dsygvx.h:25-103)
-        t0 = now();
+        t0 = sector_begin();
         const int nb = use_p ? n3 : 2 * N;
         copy_sub(nb, nb, st.A3.p, nb, st.Sa.p, n3, s);
         copy_sub(nb, nb, st.B3.p, nb, st.Sb.p, n3, s);
@@ -538,7 +549,7 @@ static void run_lobpcg(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_ho
         QG_HIP(hipMemcpyAsync(&info, st.info.p, sizeof(int), hipMemcpyDeviceToHost, s));
         QG_HIP(hipMemcpyAsync(ev.data(), st.D.p, nb * 8, hipMemcpyDeviceToHost, s));
         sync();
-        stats.t_eig += secs(t0, now());
+        stats.t_eig += sector_end(t0, 15, 1);
         if (info != 0 && o.verbose)
             std::fprintf(stderr, "lobpcg: dsygvd info = %d at iteration %d%s\n", info, iter,
                          soft_restart ? " (restart without P)" : " (ignored, as the reference)");
@@ -570,7 +581,7 @@ static void run_lobpcg(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_ho
         }
         const bool final_step = (iter + 1 == o.max_iter);
         // NextVectors2(x, r, p, tmp): T = R Cr (+ P Cp); P = T; T += X Cx
-        t0 = now();
+        t0 = sector_begin();
         op.update(R, Cr, Cr32, nb, 0.0, Tm);
         if (use_p) op.update(P, Cp, Cp32, nb, 1.0, Tm);
         QG_HIP(hipMemcpyAsync(P, Tm, blk, hipMemcpyDeviceToDevice, s));
@@ -578,7 +589,7 @@ static void run_lobpcg(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_ho
         hipLaunchKernelGGL((coldot_kernel<T>), dim3(N), dim3(256), 0, s, Tm, Tm, K, K, st.norm_a.p);
         QG_HIP(hipMemcpyAsync(nx.data(), st.norm_a.p, N * 8, hipMemcpyDeviceToHost, s));
         sync();
-        stats.t_update += secs(t0, now());
+        stats.t_update += sector_end(t0, 16, 1);
         double maxd = 0.0;
         for (int k = 0; k < N; ++k) maxd = std::max(maxd, std::fabs(1.0 - nx[k]));
         large_diff = !robust && maxd > o.rollback_limit;
@@ -607,7 +618,7 @@ static void run_lobpcg(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_ho
         QG_HIP(hipMemcpyAsync(st.eig.p, st.D.p, N * 8, hipMemcpyDeviceToDevice, s));
         bool stop = final_step;
         if (!final_step) {
-            t0 = now();
+            t0 = sector_begin();
             // NextVectors2(Ax, Ar, Ap, tmp); AX = T
             op.update(AR, Cr, Cr32, nb, 0.0, Tm);
             if (use_p) op.update(AP, Cp, Cp32, nb, 1.0, Tm);
@@ -619,8 +630,8 @@ static void run_lobpcg(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_ho
                                    st.norm_a.p, K, K);
             hipLaunchKernelGGL((residual_kernel<T>), dim3(N), dim3(256), 0, s, AX, X, st.eig.p, R, K, K, st.res2.p);
             sync();
-            stats.t_update += secs(t0, now());
-            t0 = now();
+            stats.t_update += sector_end(t0, 16, 1);
+            t0 = sector_begin();
             // NextMatrix_d: recurrences for the S and H blocks, then XtR / RtP directly
             next_matrix(st.blas, s, N, st.Sb.p, Cx, Cr, Cp, nb, use_p);
             next_matrix(st.blas, s, N, st.Sa.p, Cx, Cr, Cp, nb, use_p);
@@ -633,7 +644,7 @@ static void run_lobpcg(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_ho
             }
             QG_HIP(hipMemcpyAsync(r2.data(), st.res2.p, N * 8, hipMemcpyDeviceToHost, s));
             sync();
-            stats.t_other += secs(t0, now());
+            stats.t_other += sector_end(t0, 10, 1);
             maxres = 0.0;
             for (int k = 0; k < N; ++k) maxres = std::max(maxres, std::sqrt(r2[k]));
             stats.max_residual = maxres;
@@ -791,9 +802,20 @@ static void run_lobpcg_ortho(qg_ham* h, const qg_lobpcg_opts& o, int N, double* 
     const int gsz = grid_for(KN);
     auto now = [] { return std::chrono::steady_clock::now(); };
     auto secs = [](auto a, auto b) { return std::chrono::duration<double>(b - a).count(); };
+    // sector clock: "others" up to the start of a sector, then the sector itself
+    // (the reference's WATCH::Record ids)
+    auto sector_begin = [&] {
+        if (o.watch) o.watch(o.watch_user, 10, 1);
+        return now();
+    };
+    auto sector_end = [&](std::chrono::steady_clock::time_point t0, int idx, int64_t count) {
+        const double d = secs(t0, now());
+        if (o.watch) o.watch(o.watch_user, idx, count);
+        return d;
+    };
     auto sync = [&] { QG_HIP(hipStreamSynchronize(s)); };
     auto hpsi = [&](const T* in, T* outp) {
-        auto t0 = now();
+        auto t0 = sector_begin();
         if (o.op)
             o.op(o.op_user, dtype, N, in, outp, (void*)s);
         else if (h->batch)
@@ -801,7 +823,7 @@ static void run_lobpcg_ortho(qg_ham* h, const qg_lobpcg_opts& o, int N, double* 
         else
             hamiltonian_apply(h, dtype, N, in, K, outp, K, s);
         sync();
-        stats.t_hpsi += secs(t0, now());
+        stats.t_hpsi += sector_end(t0, 11, N);
         stats.hpsi_calls += N;
     };
     // Out = alpha A C + beta Out, C: N x N FP64 (ld ldc), A: K x N
@@ -818,12 +840,12 @@ static void run_lobpcg_ortho(qg_ham* h, const qg_lobpcg_opts& o, int N, double* 
         }
     };
     auto eigh = [&](double* A, int n, int lda, double* w) -> int {
-        auto t0 = now();
+        auto t0 = sector_begin();
         QG_BLAS(rocsolver_dsyevd(st.blas, rocblas_evect_original, rocblas_fill_upper, n, A, lda, w, st.E.p, st.info.p));
         int info = 0;
         QG_HIP(hipMemcpyAsync(&info, st.info.p, sizeof(int), hipMemcpyDeviceToHost, s));
         sync();
-        stats.t_eig += secs(t0, now());
+        stats.t_eig += sector_end(t0, 15, 1);
         return info;
     };
     // drop threshold of SVQB: well above the relative rounding of the Gram matrix (FP32 Grams
@@ -834,23 +856,23 @@ static void run_lobpcg_ortho(qg_ham* h, const qg_lobpcg_opts& o, int N, double* 
     // orthogonalise once more).
     std::vector<double> theta(N);
     auto svqb = [&](T*& V, int* keep_out, bool drop) -> double {
-        auto t0 = now();
+        auto t0 = sector_begin();
         op.gram(V, V, Gm.p, N, true);
         hipLaunchKernelGGL(svqb_diag_kernel, dim3((N + 255) / 256), dim3(256), 0, s, Gm.p, N, N, dsc.p);
         hipLaunchKernelGGL(svqb_scale_kernel, dim3((N + 255) / 256, N), dim3(256), 0, s, Gm.p, N, N,
                            (const double*)dsc.p);
         sync();
-        stats.t_gram += secs(t0, now());
+        stats.t_gram += sector_end(t0, 17, 1);
         const int info = eigh(Gm.p, N, N, st.D.p);
         QG_REQUIRE(info == 0, "SVQB eigensolve failed");
         hipLaunchKernelGGL(svqb_transform_kernel, dim3((N + 255) / 256, N), dim3(256), 0, s, Gm.p, N, N, st.D.p,
                            dsc.p, drop ? tau : 0.0, Tr.p, N, keep_out);
         QG_HIP(hipMemcpyAsync(theta.data(), st.D.p, N * 8, hipMemcpyDeviceToHost, s));
-        t0 = now();
+        t0 = sector_begin();
         upd(V, Tr.p, N, 1.0, 0.0, Tm);
         std::swap(V, Tm);
         sync();
-        stats.t_update += secs(t0, now());
+        stats.t_update += sector_end(t0, 16, 1);
         const double tmax = theta[N - 1];
         double tmin = tmax;
         for (int j = 0; j < N; ++j)
@@ -917,19 +939,19 @@ static void run_lobpcg_ortho(qg_ham* h, const qg_lobpcg_opts& o, int N, double* 
         const double big = 10.0 * (emax + 1.0e3);
         if (o.tol > 0.0 && maxres <= o.tol) break;
         // W = K R, orthogonal to X, orthonormal
-        auto t0 = now();
+        auto t0 = sector_begin();
         if (o.precond != QG_PRECOND_NONE) {
             if (learned) cnn_precond_apply(h->cnn, dtype, N, W, K, s);
             else precond_apply(h, QG_PRECOND_KINETIC, kinetic_shift(o), dtype, N, W, K, s);
             sync();
-            stats.t_precond += secs(t0, now());
+            stats.t_precond += sector_end(t0, 10, 1);
         }
         for (int pass = 0; pass < 2; ++pass) {
-            t0 = now();
+            t0 = sector_begin();
             op.gram(X, W, Gm.p, N, true);
             upd(X, Gm.p, N, -1.0, 1.0, W);
             sync();
-            stats.t_gram += secs(t0, now());
+            stats.t_gram += sector_end(t0, 17, 1);
             if (svqb(W, keep.p + N, true) > reortho) break;
         }
         hpsi(W, AW);
@@ -937,19 +959,19 @@ static void run_lobpcg_ortho(qg_ham* h, const qg_lobpcg_opts& o, int N, double* 
             // P against X and W, orthonormal; AP recomputed (a transformed AP would carry the
             // SVQB amplification of its rounding into the Rayleigh-Ritz matrix)
             for (int pass = 0; pass < 2; ++pass) {
-                t0 = now();
+                t0 = sector_begin();
                 op.gram(X, P, Gm.p, N, true);
                 upd(X, Gm.p, N, -1.0, 1.0, P);
                 op.gram(W, P, Gm.p, N, true);
                 upd(W, Gm.p, N, -1.0, 1.0, P);
                 sync();
-                stats.t_gram += secs(t0, now());
+                stats.t_gram += sector_end(t0, 17, 1);
                 if (svqb(P, keep.p + 2 * N, true) > reortho) break;
             }
             hpsi(P, AP);
         }
         // Rayleigh-Ritz: upper blocks of [X W P]^T [AX AW AP]
-        t0 = now();
+        t0 = sector_begin();
         const int nb = p_valid ? n3 : 2 * N;
         double* A = st.A3.p;
         auto blkA = [&](int bi, int bj) { return A + (int64_t)N * bi + (int64_t)nb * N * bj; };
@@ -961,7 +983,7 @@ static void run_lobpcg_ortho(qg_ham* h, const qg_lobpcg_opts& o, int N, double* 
             hipLaunchKernelGGL(mask_diag_kernel, dim3((N + 255) / 256), dim3(256), 0, s, A, nb, keep.p + 2 * N, 2 * N,
                                N, big);
         sync();
-        stats.t_gram += secs(t0, now());
+        stats.t_gram += sector_end(t0, 17, 1);
         const int info = eigh(A, nb, nb, st.D.p);
         if (info != 0) {
             // drop P and retry the iteration from X
@@ -973,7 +995,7 @@ static void run_lobpcg_ortho(qg_ham* h, const qg_lobpcg_opts& o, int N, double* 
         }
         // X' = X Cx + W Cw + P Cp,  P' = W Cw + P Cp; AX' = H X' (one H psi costs less than the
         // three GEMMs of the recurrence, and carries no accumulated rounding)
-        t0 = now();
+        t0 = sector_begin();
         const double* Cx = A;
         const double* Cw = A + N;
         const double* Cp = A + 2 * N;
@@ -989,7 +1011,7 @@ static void run_lobpcg_ortho(qg_ham* h, const qg_lobpcg_opts& o, int N, double* 
         hipLaunchKernelGGL((colscale_kernel<T>), dim3(grid_for(K) / 4 + 1, N), dim3(256), 0, s, X, (T*)nullptr,
                            st.norm_a.p, K, K);
         sync();
-        stats.t_update += secs(t0, now());
+        stats.t_update += sector_end(t0, 16, 1);
         hpsi(X, AX);
         p_valid = true;
         residual(maxres);
@@ -1130,6 +1152,8 @@ void qg_lobpcg_default_opts(qg_lobpcg_opts* o) {
     o->op_user = nullptr;
     o->basis = QG_BASIS_REFERENCE;
     o->fallback_shift = 0.0;
+    o->watch = nullptr;
+    o->watch_user = nullptr;
 }
 
 int qg_lobpcg_d_multi(qg_ham* h, const qg_lobpcg_opts* opts, int nbands, double* eig, double* X, double* P,

@@ -88,30 +88,17 @@ def test_vloc_and_atoms():
 
 
 def test_compat_header_compiles_and_links(tmp_path):
-    """include/qumasun_compat.hpp (the reference-shaped C++ surface) instantiates every
-    wrapper and links against the library; the program makes no GPU call."""
+    """include/qumasun_compat.hpp (the reference-shaped C++ surface): the driver program of
+    tests/harness/compat_drive.cpp, written against it alone, compiles and links against the
+    library (it is run on the GPU by tests/test_gpu_compat.py)."""
     import shutil
     import subprocess
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
-    src = tmp_path / "compat.cpp"
-    src.write_text(
-        '#include "qumasun_compat.hpp"\n'
-        "using namespace qumasun_gpu;\n"
-        "int main(int argc, char**) {\n"
-        "  if (argc < 5) return 0;\n"
-        "  GridRange g{0, 0, 0, 8, 8, 8};\n"
-        "  Laplasian8th<double>(g, nullptr, nullptr, 1.0, 0.1, 0.1, 0.1);\n"
-        "  Laplasian8th<float>(g, nullptr, nullptr, 1.0, 0.1, 0.1, 0.1);\n"
-        "  Laplasian8thHost<double>(g, nullptr, nullptr, 1.0, 0.1, 0.1, 0.1);\n"
-        "  PseudoPotIntegratorGPU pp; pp.InitializeGrid(g, 0.1, 0.1, 0.1); pp.Load(6, {});\n"
-        "  Nucleus n{6, 0, 0, 0}; pp.UpdatePosition(&n, 1); pp.ProjectionPP(nullptr, nullptr);\n"
-        "  HamiltonianGPU H(pp, nullptr); H.Apply<float>(1, nullptr, nullptr);\n"
-        "  H.Apply<double>(1, nullptr, nullptr);\n"
-        "  double e[1]; LOBPCG::Eigen_d_multi(H, 1, 1, e, nullptr, nullptr, 0);\n"
-        "  LOBPCG::Eigen_d_multi(H, 1, 1, e, nullptr, nullptr, 0, [](double* ax, double* x) { (void)ax; (void)x; }, 8);\n"
-        "  return 0;\n}\n")
     libdir = os.path.dirname(N.LIB_PATH)
     exe = tmp_path / "compat"
-    subprocess.run([hipcc, "-std=c++17", "-I" + os.path.join(ROOT, "include"), str(src), "-o", str(exe),
-                    "-L" + libdir, "-lqumasun_gpu", "-Wl,-rpath," + libdir], check=True, capture_output=True)
-    subprocess.run([str(exe)], check=True)
+    r = subprocess.run([hipcc, "-std=c++17", "-I" + os.path.join(ROOT, "include"),
+                        os.path.join(ROOT, "tests", "harness", "compat_drive.cpp"), "-o", str(exe),
+                        "-L" + libdir, "-lqumasun_gpu", "-Wl,-rpath," + libdir],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout[-3000:]
+    assert os.path.exists(exe)

@@ -1,0 +1,76 @@
+"""The reference-shaped C++ surface (include/qumasun_compat.hpp) driven as a QUMASUN call site
+would drive it: tests/harness/compat_drive.cpp calls Laplasian8th, ProjectionPP(Hp, psi,
+l_grid, nuclei, n), UpdatePosition, the Hamiltonian, LOBPCG::Eigen_d_multi_mpi(l_grid, dVol,
+N, iter_max, ev, X, keep_P, work, keep_S_matrix, SCF_step, OpeA, watch) with the engine's
+Hamiltonian and with a per-band user operator, and EigenLOBPCG_d_multi, all on host arrays.
+Every result is compared with the FP64 oracle on the same seeded inputs; the watch must have
+been driven with the reference's sector ids."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import qumasun_amd as Q
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def test_compat_surface_against_oracle(gpu, oracle, tmp_path):
+    from qumasun_amd import build as B
+    print("compat driver", flush=True)
+    exe = B.compat_driver()
+    out = tmp_path / "compat.bin"
+    r = subprocess.run([exe, str(out)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout[-3000:]
+    s = Q.diamond_system(1, 24)
+    n = (s["n"],) * 3
+    h = (s["h"],) * 3
+    ng = n[0] ** 3
+    nb = 8
+    d = np.fromfile(str(out), dtype=np.float64)
+    sizes = [ng, ng, 4 * ng, nb, nb * ng, nb, ng, nb]
+    assert d.size == sum(sizes), (d.size, sum(sizes))
+    parts = np.split(d, np.cumsum(sizes)[:-1])
+    lap, ppv, hp, ev, X, ev_user, hp_moved, ev_single = parts
+    X0 = Q.random_states(ng, nb, h[0] ** 3)
+    nat = len(s["positions"])
+    onl = oracle.OracleNL(n, h, s["positions"], np.zeros(nat, np.int32), s["projectors"])
+    ref_lap = np.zeros(ng)
+    oracle.laplacian8(n, h, -0.5, X0[0].copy(), ref_lap)
+    assert rel(lap, ref_lap) <= 1e-12
+    ref_pp = np.zeros(ng)
+    onl.apply(X0[0].copy(), ref_pp)
+    assert rel(ppv, ref_pp) <= 1e-12
+    assert rel(hp.reshape(4, ng), oracle.hamiltonian_block(n, h, s["V"], onl, X0[:4])) <= 1e-12
+    Xo, Po = X0.copy(), np.zeros_like(X0)
+    eo, so = oracle.lobpcg_hamiltonian(n, h, s["V"], onl, Xo, Po, 12)
+    assert so["iterations"] == 12
+    assert np.abs(ev - eo).max() < 1e-9, np.abs(ev - eo).max()
+    ov = np.abs(np.sum(X.reshape(nb, ng) * Xo, axis=1)) * h[0] ** 3
+    assert np.all(np.abs(ov - 1) < 1e-6), ov
+    assert np.abs(ev_user - ev).max() < 1e-10, np.abs(ev_user - ev).max()
+    # UpdatePosition on the live integrator: the Hamiltonian follows the moved nuclei
+    moved = s["positions"].copy()
+    for i in range(8):
+        moved[i] += [0.11 * (i % 3), -0.07 * (i % 2), 0.05]
+    onl_m = oracle.OracleNL(n, h, moved, np.zeros(nat, np.int32), s["projectors"])
+    assert rel(hp_moved, oracle.hamiltonian_block(n, h, s["V"], onl_m, X0[:1])[0]) <= 1e-12
+    # the single-process form: the reference's stop rule, dVol sum r^2 < 1e-10 dVol
+    Xs, Ps = X0.copy(), np.zeros_like(X0)
+    es, ss = oracle.lobpcg_hamiltonian(n, h, s["V"], onl, Xs, Ps, 40, tol=np.sqrt(1e-10 * h[0] ** 3))
+    assert np.abs(ev_single - es).max() < 1e-8, np.abs(ev_single - es).max()
+    for tag in ("watch", "watch_user"):
+        m = re.search(rf"^{tag} (.*)$", r.stdout, re.M)
+        assert m, r.stdout
+        cnt = dict(tuple(int(v) for v in kv.split(":")) for kv in m[1].split())
+        for sector in (10, 11, 15, 16, 17):
+            assert cnt[sector] > 0, (tag, cnt)
+        assert cnt[11] >= 12 * nb, (tag, cnt)   # H psi counted in bands: 2N at entry + N per iteration
