@@ -286,7 +286,8 @@ int qg_hamiltonian_set_cnn(qg_ham* h, qg_cnn* c);
  * psi[c], hpsi[c] band-major device blocks with ld == Ng of that cell) in persistent launches:
  * one projection launch and one fused stencil/V/projector launch per round of Infinity-Cache
  * sized band chunks, covering every cell.  Same results as qg_hamiltonian_apply per cell.
- * The cells must share the projector slot count and the points-per-thread plan.  No
+ * The cells must share the projector slot count (cells with different tile plans run in
+ * one stencil launch per plan).  No
  * counterpart in the reference (which runs one cell per process). */
 int qg_batch_hamiltonian_apply(int ncells, qg_ham* const* hams, int dtype, const int* nbands,
                                const void* const* psi, void* const* hpsi, void* stream);

@@ -1188,11 +1188,17 @@ static void batch_apply_t(const std::vector<BatchReq>& reqs, hipStream_t s, DevB
     const int NBA = njs == 4 ? 8 : 4;
     // descriptors of every round, packed into one upload
     struct RoundDesc {
-        size_t pc, pp, fc, fp;
-        int nproj, nfused, proj_total, fused_total, ppt, nthr;
-        size_t lds;
+        size_t pc, pp;
+        int nproj, proj_total;
     };
     std::vector<RoundDesc> rd(rounds);
+    // stencil cells grouped by points per thread (the autotuner may pick it per grid shape)
+    struct FusedGroupDesc {
+        size_t fc, fp;
+        int ppt, ncells, total, nthr;
+        size_t lds;
+    };
+    std::vector<std::vector<FusedGroupDesc>> fgd(rounds);
     struct NlGroupDesc {
         size_t fc, fp;
         int nb, ppt, ncells, total, nthr;
@@ -1209,10 +1215,13 @@ static void batch_apply_t(const std::vector<BatchReq>& reqs, hipStream_t s, DevB
     for (int r = 0; r < rounds; ++r) {
         std::vector<ProjCell<T>> pcs;
         std::vector<int> ppre{0};
-        std::vector<BatchCell<T>> fcs;
-        std::vector<int> fpre{0};
-        int ppt = -1, nthr = 0;
-        size_t lds = 0;
+        struct FusedGroup {
+            std::vector<BatchCell<T>> cells;
+            std::vector<int> pre{0};
+            int nthr = 0;
+            size_t lds = 0;
+        };
+        std::map<int, FusedGroup> fgroups;
         struct NlGroup {
             std::vector<BatchCell<T>> cells;
             std::vector<int> pre{0};
@@ -1262,27 +1271,31 @@ static void batch_apply_t(const std::vector<BatchReq>& reqs, hipStream_t s, DevB
                 gr.nthr = std::max(gr.nthr, fn.p.nthr);
                 gr.lds = std::max(gr.lds, fn.lds);
             }
-            QG_REQUIRE(ppt < 0 || ppt == fs.p.ppt, "cells of one batch must share the points-per-thread plan");
-            ppt = fs.p.ppt;
-            nthr = std::max(nthr, fs.p.nthr);
-            lds = std::max(lds, fs.lds);
+            FusedGroup& fg = fgroups[fs.p.ppt];
+            fg.nthr = std::max(fg.nthr, fs.p.nthr);
+            fg.lds = std::max(fg.lds, fs.lds);
             const T* V = (sizeof(T) == 4) ? (const T*)q.h->V32.p : (const T*)q.h->V64.p;
-            fcs.push_back(BatchCell<T>{fs.a, fs.sc, V, pin, pout});
-            QG_REQUIRE((int64_t)fpre.back() + fs.nwg < (1LL << 31), "batch too large");
-            fpre.push_back(fpre.back() + (int)fs.nwg);
+            fg.cells.push_back(BatchCell<T>{fs.a, fs.sc, V, pin, pout});
+            QG_REQUIRE((int64_t)fg.pre.back() + fs.nwg < (1LL << 31), "batch too large");
+            fg.pre.push_back(fg.pre.back() + (int)fs.nwg);
         }
         RoundDesc& d = rd[r];
         d.nproj = (int)pcs.size();
-        d.nfused = (int)fcs.size();
         d.proj_total = ppre.back();
-        d.fused_total = fpre.back();
-        d.ppt = ppt;
-        d.nthr = nthr;
-        d.lds = lds;
         d.pc = pcs.empty() ? 0 : push(pcs.data(), pcs.size() * sizeof(ProjCell<T>));
         d.pp = push(ppre.data(), ppre.size() * sizeof(int));
-        d.fc = push(fcs.data(), fcs.size() * sizeof(BatchCell<T>));
-        d.fp = push(fpre.data(), fpre.size() * sizeof(int));
+        for (auto& kv : fgroups) {
+            FusedGroup& fg = kv.second;
+            FusedGroupDesc gd{};
+            gd.ppt = kv.first;
+            gd.ncells = (int)fg.cells.size();
+            gd.total = fg.pre.back();
+            gd.nthr = fg.nthr;
+            gd.lds = fg.lds;
+            gd.fc = push(fg.cells.data(), fg.cells.size() * sizeof(BatchCell<T>));
+            gd.fp = push(fg.pre.data(), fg.pre.size() * sizeof(int));
+            fgd[r].push_back(gd);
+        }
         for (auto& kv : groups) {
             NlGroup& gr = kv.second;
             NlGroupDesc gd{};
@@ -1312,13 +1325,14 @@ static void batch_apply_t(const std::vector<BatchReq>& reqs, hipStream_t s, DevB
                                    (const ProjCell<T>*)(b + d.pc), (const int*)(b + d.pp), d.nproj);
             QG_HIP(hipGetLastError());
         }
-        if (d.nfused > 0 && d.fused_total > 0) {
+        for (const FusedGroupDesc& gd : fgd[r]) {
+            if (gd.total <= 0) continue;
             if (njs == 4)
-                launch_fused_batch<T, 4>(d.ppt, d.nthr, d.lds, d.fused_total, (const BatchCell<T>*)(b + d.fc),
-                                         (const int*)(b + d.fp), d.nfused, s);
+                launch_fused_batch<T, 4>(gd.ppt, gd.nthr, gd.lds, gd.total, (const BatchCell<T>*)(b + gd.fc),
+                                         (const int*)(b + gd.fp), gd.ncells, s);
             else
-                launch_fused_batch<T, 8>(d.ppt, d.nthr, d.lds, d.fused_total, (const BatchCell<T>*)(b + d.fc),
-                                         (const int*)(b + d.fp), d.nfused, s);
+                launch_fused_batch<T, 8>(gd.ppt, gd.nthr, gd.lds, gd.total, (const BatchCell<T>*)(b + gd.fc),
+                                         (const int*)(b + gd.fp), gd.ncells, s);
         }
         for (const NlGroupDesc& gd : nlg[r]) {
             if (gd.total <= 0) continue;
@@ -1453,6 +1467,7 @@ static void nl_build(qg_nonlocal* nl, int natoms, const double* xyz, const int* 
     nl->d_atom_coef_off.upload(gm.atom_coef_off.data(), gm.atom_coef_off.size());
     for (auto& t : nl->tiles) t.ty = t.tz = -1;
     nl->rows.clear();
+    rows_prepare(nl);
     // Gaussian projectors can be evaluated inside the projector pass instead of read from
     // per-point tables (QG_NL_ANALYTIC=0 keeps the tables everywhere)
     bool gauss = !gm.projs.empty();

@@ -173,6 +173,7 @@ void build_tile_nl(const qg_grid& g, const NlGeometry& geo, int ty, int tz, bool
 struct RowsGeo {
     int ty = 0, tzp = 0, ntile = 0, nslabp = 0;
     int max_col = 0;     // most images of one tile column
+    int max_items = 0;   // most items of one (tile, plane)
     int nslot = 0;       // projection partial slots (one per active (tile, slab, image))
     int njs = 4;         // coefficient slots per atom (4 or 8)
     bool sp4 = false;    // every image: one s channel (slot 0) and one p channel (slots 1..3)
@@ -252,6 +253,7 @@ bool rows_nonlocal_project(qg_nonlocal* nl, int dtype, int nbands, const void* p
                            DevBuf<double>& part_buf, hipStream_t s);
 bool rows_laplacian_apply(const qg_grid& g, int dtype, int nbands, const void* psi, int64_t ld, void* out,
                           int64_t ld_out, double coef, int accumulate, hipStream_t s);
+void rows_prepare(qg_nonlocal* nl);
 }  // namespace qg
 
 struct qg_ham {

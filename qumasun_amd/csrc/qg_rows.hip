@@ -109,6 +109,7 @@ struct RowsHpsiArgs {
     const int* col_img;         // global image ids
     const int* it_off;          // per (tile, plane): first item
     const uint32_t* items;
+    int nitems;                 // total items (item loads are clamped to the array)
     const DevImage* images;
     const DevProj* projs;
     int nproj;
@@ -128,6 +129,7 @@ struct RowsProjArgs {
     const int* col_img;
     const int* it_off;
     const uint32_t* items;
+    int nitems;                 // total items (item loads are clamped to the array)
     const int* pl_base;         // per tile: base of its (plane, column image) item offsets
     const int* pl_off;          // (plane, image) -> first item of the image in the plane
     const uint32_t* thr_map;    // per (tile, slab, thread): image | rank << 10 | (count-1) << 18
@@ -219,7 +221,11 @@ __device__ __forceinline__ void beta_at(const ProjLds<T>& L, int li, T dx, T dy,
 
 // ------------------------------------------------------------------------ projection ----
 constexpr int kProjThreads = kRowsProjThreads;
-constexpr int kProjPV = 8;   // staged vectors per thread per plane
+constexpr int kProjPV = 8;      // staged psi vectors per thread per plane
+constexpr int kItemsPT = 4;     // staged items per thread per plane (both kernels)
+
+// LDS offset rounded up to 16 bytes
+__host__ __device__ constexpr size_t align16(size_t b) { return (b + 15) & ~size_t(15); }
 
 template <class T, int NBP, int NJS, bool SP4>
 __global__ __launch_bounds__(kProjThreads) void rows_project_kernel(RowsProjArgs a, const T* __restrict__ psi) {
@@ -237,11 +243,17 @@ __global__ __launch_bounds__(kProjThreads) void rows_project_kernel(RowsProjArgs
     const int tid = threadIdx.x;
     const int plane = nx * ny;
 
-    // LDS: the tile plane of NBP bands, band fastest [row][x][band]; reused for the reduction
+    // LDS: the tile plane of NBP bands, band fastest [row][x][band], reused for the reduction;
+    // the plane's items; the slab's per-plane item offsets; the projector staging
     const size_t pl_elems = (size_t)a.ty * nx * NBP;
     const size_t red_elems = (size_t)kProjThreads * NJS * NBP;
     T* pl = reinterpret_cast<T*>(smem);
-    unsigned char* lp = smem + sizeof(T) * (pl_elems > red_elems ? pl_elems : red_elems);
+    size_t off = align16(sizeof(T) * (pl_elems > red_elems ? pl_elems : red_elems));
+    uint32_t* itl = reinterpret_cast<uint32_t*>(smem + off);
+    off += sizeof(uint32_t) * kProjThreads * kItemsPT;
+    int* itoffl = reinterpret_cast<int*>(smem + off);
+    off = align16(off + sizeof(int) * (a.tz + 1));
+    unsigned char* lp = smem + off;
     ProjLds<T> L;
     L.im_f = reinterpret_cast<T*>(lp);
     lp += sizeof(T) * 3 * a.max_col;
@@ -254,6 +266,7 @@ __global__ __launch_bounds__(kProjThreads) void rows_project_kernel(RowsProjArgs
     const int c0 = a.col_off[tile];
     const int ncol = a.col_off[tile + 1] - c0;
     stage_proj<T>(L, a.images, a.projs, a.nproj, a.col_img + c0, ncol, a.h, tid, kProjThreads);
+    for (int i = tid; i <= z1 - z0; i += kProjThreads) itoffl[i] = a.it_off[(size_t)tile * nz + z0 + i];
 
     const uint32_t tm = a.thr_map[((size_t)tile * a.nslab + slab) * kProjThreads + tid];
     const bool act = tm != 0xffffffffu;
@@ -280,12 +293,27 @@ __global__ __launch_bounds__(kProjThreads) void rows_project_kernel(RowsProjArgs
         vo[k] = ok ? ((y0 + r) * nx + xq * VN) / VN : 0;   // in vectors
         lo[k] = ok ? ((r * nx + xq * VN) * NBP + bs) : -1;
     }
+    const int* plb = a.pl_off + a.pl_base[tile];
+    // next plane, held in registers until it is staged: psi vectors, items (dealt over the
+    // workgroup), and the item range of this thread's image.  No global load is waited for
+    // inside the compute phase, so these loads stay in flight across it.
     Row<T> pre[kProjPV];
+    uint32_t ipre[kItemsPT];
+    int nxt_s = 0, nxt_e = 0;
+    // every load is issued by every lane (indices clamped into the arrays; unused values are
+    // dropped), so the compiler waits for exactly the loads it needs
+    const int lim = max(a.nitems - 1, 0);
+    const int pli0 = act ? li : 0, pli1 = act ? li + 1 : 0;
     auto load = [&](int z) {
         const Row<T>* sp = src + (int64_t)z * (plane / VN);
 #pragma unroll
-        for (int k = 0; k < kProjPV; ++k)
-            if (lo[k] >= 0) pre[k] = sp[vo[k]];
+        for (int k = 0; k < kProjPV; ++k) pre[k] = sp[vo[k]];
+        const int ib = itoffl[z - z0];
+#pragma unroll
+        for (int k = 0; k < kItemsPT; ++k) ipre[k] = a.items[min(ib + tid + kProjThreads * k, lim)];
+        const int* po = plb + (size_t)z * (ncol + 1);
+        nxt_s = po[pli0];
+        nxt_e = po[pli1];
     };
 
     T acc[NJS][NBP];
@@ -295,27 +323,33 @@ __global__ __launch_bounds__(kProjThreads) void rows_project_kernel(RowsProjArgs
         for (int b = 0; b < NBP; ++b) acc[j][b] = T(0);
 
     const T hx = (T)a.h[0], hy = (T)a.h[1], hz = (T)a.h[2];
-    const int* plb = a.pl_off + a.pl_base[tile];
+    __syncthreads();   // itoffl and the projector staging
     load(z0);
     for (int z = z0; z < z1; ++z) {
-        __syncthreads();   // the previous plane's readers are done (and the staging above)
+        __syncthreads();   // the previous plane's readers are done
 #pragma unroll
         for (int k = 0; k < kProjPV; ++k)
             if (lo[k] >= 0) {
 #pragma unroll
                 for (int i = 0; i < VN; ++i) pl[lo[k] + i * NBP] = pre[k].v[i];
             }
+        {
+            const int n = itoffl[z - z0 + 1] - itoffl[z - z0];
+#pragma unroll
+            for (int k = 0; k < kItemsPT; ++k) {
+                const int i = tid + kProjThreads * k;
+                if (i < n) itl[i] = ipre[k];
+            }
+        }
+        const int cs = nxt_s, ce = nxt_e;
         if (z + 1 < z1) load(z + 1);
         __syncthreads();
         if (act) {
-            const int ib = a.it_off[(size_t)tile * nz + z];
-            const int* po = plb + (size_t)z * (ncol + 1);
-            const int s = ib + po[li], e = ib + po[li + 1];
             const int* ii = L.im_i + 5 * li;
             const T* ff = L.im_f + 3 * li;
             const T dz = hz * ((T)(z - ii[2]) - ff[2]);
-            for (int i = s + rk; i < e; i += cnt) {
-                const uint32_t it = a.items[i];
+            for (int i = cs + rk; i < ce; i += cnt) {
+                const uint32_t it = itl[i];
                 const int r = (int)(it & 31u), xs = (int)((it >> 5) & 1023u), np = (int)((it >> 15) & 3u) + 1;
                 const T dy = hy * ((T)(y0 + r - ii[1]) - ff[1]);
                 const T* prow = pl + (size_t)(r * nx + xs) * NBP;
@@ -379,10 +413,15 @@ __global__ void rows_coef_reduce_kernel(const double* __restrict__ part, int par
 
 // -------------------------------------------------------------- stencil + V + projector ----
 constexpr int kRowsMaxThreads = 1024;
+// buffer offset of lanes with nothing to load or store: beyond every band's num_records, so
+// the hardware returns zero for the load and drops the store
+constexpr int kOOB = 0x7ffffff0;
 
-template <class T, int NBW, int NJS, bool SP4>
-__global__ __launch_bounds__(NBW >= 4 ? 1024 : 512) void rows_hpsi_kernel(RowsHpsiArgs a, Coef8<T> sc, const T* __restrict__ V,
-                                                                  const T* __restrict__ psi, T* __restrict__ out) {
+template <class T, int NBW, int NJS, bool SP4, bool NL>
+__global__ __launch_bounds__(NBW >= 4 ? 1024 : 512) void rows_hpsi_kernel(RowsHpsiArgs a, Coef8<T> sc,
+                                                                          const T* __restrict__ V,
+                                                                          const T* __restrict__ psi,
+                                                                          T* __restrict__ out) {
     constexpr int VN = 16 / sizeof(T);
     constexpr int S = sizeof(T);
     extern __shared__ __align__(16) unsigned char smem[];
@@ -391,10 +430,17 @@ __global__ __launch_bounds__(NBW >= 4 ? 1024 : 512) void rows_hpsi_kernel(RowsHp
     const int ty = a.ty;
     const int lband = (ty + 8) * W;
     const int nthr = blockDim.x;
+    // LDS layout (rows_hpsi_lds on the host): planes, projector term, E*c, items, the slab's
+    // per-plane item offsets, projector staging
     T* planes = reinterpret_cast<T*>(smem);                    // [2][NBW][ty + 8][W]
     T* nlb = planes + (size_t)2 * NBW * lband;                 // [NBW][ty][nx]
     T* ec = nlb + (size_t)NBW * ty * nx;                       // [max_col][NBW][NJS]
-    unsigned char* lp = reinterpret_cast<unsigned char*>(ec + (size_t)a.max_col * NBW * NJS);
+    size_t off = sizeof(T) * ((size_t)2 * NBW * lband + (size_t)NBW * ty * nx + (size_t)a.max_col * NBW * NJS);
+    uint32_t* itl = reinterpret_cast<uint32_t*>(smem + off);   // [nthr * kItemsPT]
+    off += sizeof(uint32_t) * (size_t)nthr * kItemsPT;
+    int* itoffl = reinterpret_cast<int*>(smem + off);          // [tz + 1]
+    off = align16(off + sizeof(int) * (a.tz + 1));
+    unsigned char* lp = smem + off;
     ProjLds<T> L;
     L.im_f = reinterpret_cast<T*>(lp);
     lp += sizeof(T) * 3 * a.max_col;
@@ -426,15 +472,20 @@ __global__ __launch_bounds__(NBW >= 4 ? 1024 : 512) void rows_hpsi_kernel(RowsHp
     const int band = min(b0 + min(bl, NBW - 1), a.nbands - 1);
     const rsrc_t rin = rsrc(psi + (int64_t)band * a.ld_in, band_bytes);
     const rsrc_t rout = rsrc(out + (int64_t)band * a.ld_out, band_bytes);
-    const rsrc_t rv = rsrc(V, band_bytes);
+    // second operand of the point term: V (H psi), the previous output (accumulating
+    // Laplacian), or nothing (a resource of zero records reads zeros)
+    const rsrc_t rsec = a.add_v ? rsrc(V, band_bytes) : (a.accumulate ? rout : rsrc(psi, 0));
 
-    // own vector: plane offset, LDS slot (and the wrapped copy that pads the x-halo)
-    const int vo = own ? ((y0 + r) * nx + x) * S : 0;
+    // own vector: plane offset (kOOB for lanes without a point), LDS slot and the wrapped copy
+    // that pads the x-halo
+    const int vo_pt = ((y0 + r) * nx + x) * S;
+    const int vo = own ? vo_pt : kOOB;
+    const int vst = (own && bok) ? vo_pt : kOOB;
     const int lc = (r + 4) * W + 4 + x;
     const int lw = (x < 4) ? lc + nx : ((x >= nx - 4) ? lc - nx : -1);
     // halo vector of this thread: 8 rows x nq vectors per band (4 above the tile, 4 below)
     const bool hown = bl < NBW && lt < 8 * nq;
-    int ho = 0, hl = 0, hw = -1;
+    int ho = kOOB, hl = 0, hw = -1;
     if (hown) {
         const int hr = lt / nq, hx = (lt - hr * nq) * VN;
         int gy = (hr < 4) ? y0 - 4 + hr : y0 + rows + (hr - 4);
@@ -450,10 +501,11 @@ __global__ __launch_bounds__(NBW >= 4 ? 1024 : 512) void rows_hpsi_kernel(RowsHp
         return zz * plane * S;
     };
 
-    // projector staging: image data of the tile column and E*c of its atoms for the bands
-    const int c0 = a.col_off[tile];
-    const int ncol = a.has_nl ? a.col_off[tile + 1] - c0 : 0;
-    if (a.has_nl) {
+    // projector staging: image data of the tile column and E*c of its atoms for the bands, the
+    // slab's per-plane item offsets
+    if constexpr (NL) {
+        const int c0 = a.col_off[tile];
+        const int ncol = a.col_off[tile + 1] - c0;
         stage_proj<T>(L, a.images, a.projs, a.nproj, a.col_img + c0, ncol, a.h, tid, nthr);
         for (int i = tid; i < ncol * NBW * NJS; i += nthr) {
             const int li = i / (NBW * NJS), b = (i / NJS) % NBW, j = i % NJS;
@@ -465,29 +517,47 @@ __global__ __launch_bounds__(NBW >= 4 ? 1024 : 512) void rows_hpsi_kernel(RowsHp
             ec[i] = v;
         }
         for (int i = tid; i < NBW * ty * nx; i += nthr) nlb[i] = T(0);
+        for (int i = tid; i <= z1 - z0; i += nthr) itoffl[i] = a.it_off[(size_t)tile * nz + z0 + i];
+        __syncthreads();
     }
 
     Row<T> q[9];
 #pragma unroll
-    for (int d = 0; d < 8; ++d) {
-        if (own) q[d] = vload<T>(rin, vo, zoff(z0 - 4 + d));
-        else q[d] = Row<T>{};
-    }
+    for (int d = 0; d < 8; ++d) q[d] = vload<T>(rin, vo, zoff(z0 - 4 + d));
+    // the eight warm-up planes complete here, once: the loop's waits then cover only the
+    // loads issued inside it
+    __builtin_amdgcn_s_waitcnt(0);
+    // next plane, held in registers until it is staged: psi (4 planes ahead in z), V or the
+    // accumulated output, the y-halo, and the plane's projector items dealt over the
+    // workgroup.  Every lane issues every load (out-of-range offsets or clamped indices where
+    // it has nothing to load), so the compiler can wait for exactly the loads it needs, and
+    // the projector phase reads LDS only: these loads stay in flight across the plane's work.
     Row<T> qn{}, hn{}, vn{};
+    uint32_t ipre[kItemsPT];
+    const int lim = max(a.nitems - 1, 0);
     auto prefetch = [&](int z) {
-        if (own) {
-            qn = vload<T>(rin, vo, zoff(z + 4));
-            if (a.add_v) vn = vload<T>(rv, vo, zoff(z));
-            else if (a.accumulate) vn = vload<T>(rout, vo, zoff(z));
+        qn = vload<T>(rin, vo, zoff(z + 4));
+        vn = vload<T>(rsec, vo, zoff(z));
+        hn = vload<T>(rin, ho, zoff(z));
+        if constexpr (NL) {
+            const int ib = itoffl[z - z0];
+#pragma unroll
+            for (int k = 0; k < kItemsPT; ++k) ipre[k] = a.items[min(ib + tid + nthr * k, lim)];
         }
-        if (hown) hn = vload<T>(rin, ho, zoff(z));
     };
     prefetch(z0);
     const T hx = (T)a.h[0], hy = (T)a.h[1], hz = (T)a.h[2];
+    // a plane's result is stored at the top of the next iteration, after that iteration's
+    // loads were issued one iteration earlier: every iteration starts with the same memory
+    // operations outstanding (the prefetch, then one store), so waiting for the prefetch leaves
+    // the store in flight.  The first iteration's store is a dropped out-of-range one.
+    Row<T> res{};
+    int st_off = kOOB, st_z = 0;
 
     for (int z = z0; z < z1; ++z) {
         const int buf = (z - z0) & 1;
         T* Lb = planes + ((size_t)buf * NBW + (bl < NBW ? bl : 0)) * lband;
+        vstore<T>(rout, res, st_off, st_z);
         q[8] = qn;
         const Row<T> vcur = vn;
         if (own) {
@@ -498,14 +568,23 @@ __global__ __launch_bounds__(NBW >= 4 ? 1024 : 512) void rows_hpsi_kernel(RowsHp
             *reinterpret_cast<Row<T>*>(Lb + hl) = hn;
             if (hw >= 0) *reinterpret_cast<Row<T>*>(Lb + hw) = hn;
         }
+        int nit = 0;
+        if constexpr (NL) {
+            // this plane's items (the previous plane's readers passed its second barrier)
+            nit = itoffl[z - z0 + 1] - itoffl[z - z0];
+#pragma unroll
+            for (int k = 0; k < kItemsPT; ++k) {
+                const int i = tid + nthr * k;
+                if (i < nit) itl[i] = ipre[k];
+            }
+        }
         if (z + 1 < z1) prefetch(z + 1);
         __syncthreads();
 
-        if (a.has_nl) {
+        if constexpr (NL) {
             // projector term of this plane: items dealt over all threads of the workgroup
-            const int i0 = a.it_off[(size_t)tile * nz + z], i1 = a.it_off[(size_t)tile * nz + z + 1];
-            for (int i = i0 + tid; i < i1; i += nthr) {
-                const uint32_t it = a.items[i];
+            for (int i = tid; i < nit; i += nthr) {
+                const uint32_t it = itl[i];
                 const int ir = (int)(it & 31u), xs = (int)((it >> 5) & 1023u);
                 const int np = (int)((it >> 15) & 3u) + 1, li = (int)(it >> 17);
                 const int* ii = L.im_i + 5 * li;
@@ -546,15 +625,9 @@ __global__ __launch_bounds__(NBW >= 4 ? 1024 : 512) void rows_hpsi_kernel(RowsHp
 
         if (own) {
             const T* Lc = Lb + (r + 4) * W + x;   // Lc[4 + i] is point x + i
-            Row<T> res;
 #pragma unroll
-            for (int i = 0; i < VN; ++i) {
-                T base = T(0);
-                if (a.add_v) base = vcur.v[i] * q[4].v[i];
-                else if (a.accumulate) base = vcur.v[i];
-                res.v[i] = base;
-            }
-            if (a.has_nl) {
+            for (int i = 0; i < VN; ++i) res.v[i] = a.add_v ? vcur.v[i] * q[4].v[i] : vcur.v[i];
+            if constexpr (NL) {
                 Row<T>* np = reinterpret_cast<Row<T>*>(nlb + ((size_t)bl * ty + r) * nx + x);
                 const Row<T> nlv = *np;
                 *np = Row<T>{};
@@ -592,11 +665,14 @@ __global__ __launch_bounds__(NBW >= 4 ? 1024 : 512) void rows_hpsi_kernel(RowsHp
             }
 #pragma unroll
             for (int i = 0; i < VN; ++i) res.v[i] += d[i];
-            if (bok) vstore<T>(rout, res, vo, zoff(z));
         }
+        // every lane stores (next iteration): lanes without a point, or past the last band, use kOOB
+        st_off = vst;
+        st_z = zoff(z);
 #pragma unroll
         for (int d = 0; d < 8; ++d) q[d] = q[d + 1];
     }
+    vstore<T>(rout, res, st_off, st_z);
 }
 
 // ------------------------------------------------------------------------ host side ----
@@ -620,18 +696,34 @@ int round64i(int v) { return (v + 63) / 64 * 64; }
 
 }  // namespace
 
-// Tile height for a row of nq vectors: whole wavefronts per band with as few idle lanes as
-// possible (rows of 8-16 preferred), at least 8 rows' worth of threads for the y-halo.
+// Tile height of the apply kernel for a row of nq vectors: whole wavefronts per band with as
+// few idle lanes as possible (rows of 8-16 preferred), at least 8 rows' worth of threads for
+// the y-halo, and at most kRowsBandThreads threads per band (the kernel's launch bound with one
+// band per workgroup).  0 when even 8 rows exceed that.
+constexpr int kRowsBandThreads = 512;
 static int rows_pick_ty(int nq, int ny, int want) {
-    if (want > 0) return std::max(1, std::min(ny, std::min(32, want)));
+    if (round64i(8 * nq) > kRowsBandThreads) return 0;
+    if (want > 0) {
+        int ty = std::max(1, std::min(ny, std::min(32, want)));
+        while (ty > 8 && round64i(ty * nq) > kRowsBandThreads) --ty;
+        return ty;
+    }
     int best = 8, bests = -1;
     for (int ty : {8, 16, 12, 10, 6, 20, 24, 4, 32}) {
         if (ty > ny) continue;
         const int t = round64i(std::max(ty, 8) * nq);
+        if (t > kRowsBandThreads) continue;
         const int score = (int)(1000.0 * ty * nq / t) + (ty >= 8 ? 5 : 0);
         if (score > bests) { bests = score; best = ty; }
     }
     return std::min(best, ny);
+}
+
+// Tile height of the projection kernel: the largest (at most 32 rows, at most ny) whose plane
+// of nbp bands fits the staging registers (kProjPV vectors per thread); 0 when one row does not.
+static int rows_pick_typ(int nq, int ny, int nbp) {
+    const int cap = kProjThreads * kProjPV / std::max(1, nbp * nq);
+    return std::max(0, std::min(std::min(32, ny), cap));
 }
 
 // Whether the row kernels serve this grid and projector set: x-rows of whole 16-byte vectors,
@@ -640,7 +732,8 @@ static bool rows_eligible(const qg_grid& g, int dtype, const qg_nonlocal* nl) {
     if (env_int("QG_ROWS", 1) == 0) return false;
     const int VN = dtype == QG_F32 ? 4 : 2;
     if (g.n[0] % VN != 0 || g.n[0] < 8 || g.n[0] > 1024) return false;
-    if (g.ng() * 8 >= (1LL << 31)) return false;
+    if (g.ng() * 8 > (int64_t)kOOB - 64) return false;   // band bytes below the out-of-range offset
+    if (rows_pick_ty(g.n[0] / VN, g.n[1], 0) == 0) return false;
     if (nl) {
         if (nl->geo.max_coef_per_atom > 8) return false;
         for (const DevProj& p : nl->geo.projs)
@@ -649,11 +742,18 @@ static bool rows_eligible(const qg_grid& g, int dtype, const qg_nonlocal* nl) {
     return true;
 }
 
+static int rows_nthr(const qg_grid& g, int dtype, int ty, int nbw) {
+    const int VN = dtype == QG_F32 ? 4 : 2;
+    return nbw * round64i(std::max(ty, 8) * (g.n[0] / VN));
+}
+
 struct RowsShape {
-    int ty, tz, tzp, nbw, nbp;
+    int ty, tz;     // apply kernel: tile rows, slab planes
+    int typ, tzp;   // projection kernel: tile rows, slab planes
+    int nbw, nbp;   // bands per workgroup of the apply and projection kernels
 };
 
-static RowsShape rows_shape(const qg_grid& g, int dtype, int nbands) {
+static RowsShape rows_shape(const qg_grid& g, int dtype) {
     const int VN = dtype == QG_F32 ? 4 : 2;
     const int nq = g.n[0] / VN;
     RowsShape sh{};
@@ -663,15 +763,20 @@ static RowsShape rows_shape(const qg_grid& g, int dtype, int nbands) {
     while (sh.nbw > 1 && sh.nbw * round64i(std::max(sh.ty, 8) * nq) > (sh.nbw >= 4 ? 1024 : 512)) sh.nbw /= 2;
     sh.tz = std::max(1, std::min(g.n[2], env_int("QG_ROWS_TZ", g.n[2])));
     sh.tzp = std::max(1, std::min(g.n[2], env_int("QG_ROWS_TZP", 16)));
-    sh.nbp = env_int("QG_ROWS_NBP", 8) == 4 ? 4 : 8;
-    (void)nbands;
+    // FP64 projection with 4 bands per workgroup: its reduction buffer (256 threads x 4 slots
+    // x bands) stays at 32 KB
+    sh.nbp = env_int("QG_ROWS_NBP", dtype == QG_F32 ? 8 : 4) == 4 ? 4 : 8;
+    sh.typ = rows_pick_typ(nq, g.n[1], sh.nbp);
     return sh;
 }
 
-static RowsGeo* rows_geometry(qg_nonlocal* nl, int ty, int tzp) {
+// The row geometry of nl for tile rows ty and projection slabs of tzp planes, built (and
+// uploaded) on first use; with build == false only a cached one is returned (nullptr if none).
+static RowsGeo* rows_geometry(qg_nonlocal* nl, int ty, int tzp, bool build = true) {
     std::lock_guard<std::mutex> lk(nl->rows_mu);
     for (auto& c : nl->rows)
         if (c && c->ty == ty && c->tzp == tzp) return c.get();
+    if (!build) return nullptr;
     auto c = std::make_unique<RowsGeo>();
     if (!build_rows_geometry(nl->gcopy, nl->geo, ty, tzp, *c)) {
         c->ok = false;
@@ -700,18 +805,24 @@ static RowsGeo* rows_geometry(qg_nonlocal* nl, int ty, int tzp) {
     return nl->rows.back().get();
 }
 
+// LDS bytes of a projection workgroup (layout of rows_project_kernel)
 template <class T>
 static size_t rows_proj_lds(const RowsGeo& rg, const qg_grid& g, int nbp, int nproj) {
     const size_t pl = (size_t)rg.ty * g.n[0] * nbp, red = (size_t)kProjThreads * rg.njs * nbp;
-    return sizeof(T) * std::max(pl, red) + sizeof(T) * (3 * rg.max_col + 2 * nproj) +
-           sizeof(int) * (5 * rg.max_col + 2 * nproj);
+    size_t off = align16(sizeof(T) * std::max(pl, red));
+    off += sizeof(uint32_t) * kProjThreads * kItemsPT;
+    off = align16(off + sizeof(int) * (rg.tzp + 1));
+    return off + sizeof(T) * (3 * rg.max_col + 2 * nproj) + sizeof(int) * (5 * rg.max_col + 2 * nproj);
 }
 
+// LDS bytes of an apply workgroup of nthr threads (layout of rows_hpsi_kernel)
 template <class T>
-static size_t rows_hpsi_lds(const qg_grid& g, int ty, int nbw, int njs, int max_col, int nproj) {
+static size_t rows_hpsi_lds(const qg_grid& g, int ty, int tz, int nbw, int njs, int max_col, int nproj, int nthr) {
     const size_t W = g.n[0] + 8;
-    return sizeof(T) * (2 * (size_t)nbw * (ty + 8) * W + (size_t)nbw * ty * g.n[0] + (size_t)max_col * nbw * njs +
-                        3 * (size_t)max_col + 2 * (size_t)nproj) +
+    size_t off = sizeof(T) * (2 * (size_t)nbw * (ty + 8) * W + (size_t)nbw * ty * g.n[0] + (size_t)max_col * nbw * njs);
+    off += sizeof(uint32_t) * (size_t)nthr * kItemsPT;
+    off = align16(off + sizeof(int) * (tz + 1));
+    return off + sizeof(T) * (3 * (size_t)max_col + 2 * (size_t)nproj) +
            sizeof(int) * (5 * (size_t)max_col + 2 * (size_t)nproj);
 }
 
@@ -726,6 +837,7 @@ static void rows_launch_project(qg_nonlocal* nl, const RowsGeo& rg, int nbp, int
     a.ld = ld;
     a.col_off = rg.d_col_off.p; a.col_img = rg.d_col_img.p;
     a.it_off = rg.d_it_off.p; a.items = rg.d_items.p;
+    a.nitems = rg.it_off.empty() ? 0 : rg.it_off.back();
     a.pl_base = rg.d_pl_base.p; a.pl_off = rg.d_pl_off.p;
     a.thr_map = rg.d_thr_map.p; a.thr_slot = rg.d_thr_slot.p;
     a.images = nl->d_images.p; a.projs = nl->d_projs.p; a.nproj = (int)nl->geo.projs.size();
@@ -736,6 +848,7 @@ static void rows_launch_project(qg_nonlocal* nl, const RowsGeo& rg, int nbp, int
     const int VN = 16 / sizeof(T);
     QG_REQUIRE((int64_t)nbp * rg.ty * (g.n[0] / VN) <= (int64_t)kProjThreads * kProjPV,
                "projection tile does not fit the staging registers");
+    QG_REQUIRE(rg.max_items <= kProjThreads * kItemsPT, "a tile plane holds more projector items than staged");
     const size_t lds = rows_proj_lds<T>(rg, g, nbp, a.nproj);
     QG_REQUIRE(lds <= 65536, "projection tile does not fit in LDS");
     const int64_t nwg = (int64_t)rg.ntile * rg.nslabp * ((nbands + nbp - 1) / nbp);
@@ -775,6 +888,7 @@ static void rows_launch_hpsi_t(const qg_grid& g, const RowsShape& sh, const Rows
     if (rg) {
         a.col_off = rg->d_col_off.p; a.col_img = rg->d_col_img.p;
         a.it_off = rg->d_it_off.p; a.items = rg->d_items.p;
+        a.nitems = rg->it_off.empty() ? 0 : rg->it_off.back();
         a.images = nl->d_images.p; a.projs = nl->d_projs.p; a.nproj = (int)nl->geo.projs.size();
         a.coef = coef; a.coef_energy = nl->d_coef_energy.p; a.atom_coef_off = nl->d_atom_coef_off.p;
         a.ncoef = nl->geo.ncoef;
@@ -783,12 +897,17 @@ static void rows_launch_hpsi_t(const qg_grid& g, const RowsShape& sh, const Rows
     for (int d = 0; d < 3; ++d) a.h[d] = g.h[d];
     const int nthr = NBW * a.tpb;
     QG_REQUIRE(nthr <= (NBW >= 4 ? 1024 : 512), "row tile too wide for one workgroup");
-    const size_t lds = rows_hpsi_lds<T>(g, sh.ty, NBW, NJS, a.max_col, a.nproj);
+    QG_REQUIRE(!rg || rg->max_items <= nthr * kItemsPT, "a tile plane holds more projector items than staged");
+    const size_t lds = rows_hpsi_lds<T>(g, sh.ty, sh.tz, NBW, NJS, a.max_col, a.nproj, nthr);
     QG_REQUIRE(lds <= 65536, "row tile does not fit in LDS");
     const int64_t nwg = (int64_t)a.ntile * a.nslab * ((nbands + NBW - 1) / NBW);
     QG_REQUIRE(nwg < (1LL << 31), "too many workgroups");
-    hipLaunchKernelGGL((rows_hpsi_kernel<T, NBW, NJS, SP4>), dim3((unsigned)nwg), dim3(nthr), lds, s, a,
-                       coef8<T>(g, lap_coef), V, psi, out);
+    if (rg)
+        hipLaunchKernelGGL((rows_hpsi_kernel<T, NBW, NJS, SP4, true>), dim3((unsigned)nwg), dim3(nthr), lds, s, a,
+                           coef8<T>(g, lap_coef), V, psi, out);
+    else
+        hipLaunchKernelGGL((rows_hpsi_kernel<T, NBW, 4, false, false>), dim3((unsigned)nwg), dim3(nthr), lds, s, a,
+                           coef8<T>(g, lap_coef), V, psi, out);
     QG_HIP(hipGetLastError());
 }
 
@@ -815,32 +934,41 @@ static void rows_launch_hpsi(const qg_grid& g, const RowsShape& sh, const RowsGe
 // serve this grid / projector set, so the caller runs the table kernels instead.
 template <class T>
 static bool rows_apply_t(qg_ham* h, int nbands, const T* psi, int64_t ld, T* hpsi, int64_t ld_out, hipStream_t s,
-                         RowsScratch* scratch) {
+                         RowsScratch* ws) {
     const qg_grid& g = h->grid;
     qg_nonlocal* nl = (h->nl && h->nl->geo.ncoef > 0) ? h->nl : nullptr;
     const int dtype = sizeof(T) == 4 ? QG_F32 : QG_F64;
     if (!rows_eligible(g, dtype, nl)) return false;
-    const RowsShape sh = rows_shape(g, dtype, nbands);
-    const RowsGeo* rg = nullptr;
+    const RowsShape sh = rows_shape(g, dtype);
+    const RowsGeo* rg = nullptr;    // apply kernel's geometry (tile rows ty)
+    const RowsGeo* rgp = nullptr;   // projection kernel's geometry (tile rows typ)
     if (nl) {
-        rg = rows_geometry(nl, sh.ty, sh.tzp);
-        if (!rg->ok) return false;
-        if (rows_proj_lds<T>(*rg, g, sh.nbp, (int)nl->geo.projs.size()) > 65536) return false;
-        if (rows_hpsi_lds<T>(g, sh.ty, sh.nbw, rg->njs, std::max(1, rg->max_col), (int)nl->geo.projs.size()) > 65536)
+        if (sh.typ == 0) return false;
+        // with caller scratch nothing may be allocated: the geometry must exist already
+        rg = rows_geometry(nl, sh.ty, sh.tzp, ws == nullptr);
+        rgp = sh.typ == sh.ty ? rg : rows_geometry(nl, sh.typ, sh.tzp, ws == nullptr);
+        QG_REQUIRE(rg && rgp, "the projector geometry of this launch shape is not prepared (qg_nonlocal_create and "
+                              "qg_nonlocal_update prepare the default shapes)");
+        if (!rg->ok || !rgp->ok) return false;
+        const int nthr = rows_nthr(g, dtype, sh.ty, sh.nbw);
+        if (rgp->max_items > kProjThreads * kItemsPT || rg->max_items > nthr * kItemsPT) return false;
+        if (rows_proj_lds<T>(*rgp, g, sh.nbp, (int)nl->geo.projs.size()) > 65536) return false;
+        if (rows_hpsi_lds<T>(g, sh.ty, sh.tz, sh.nbw, rg->njs, std::max(1, rg->max_col), (int)nl->geo.projs.size(),
+                             nthr) > 65536)
             return false;
-    } else if (rows_hpsi_lds<T>(g, sh.ty, sh.nbw, 4, 1, 0) > 65536) {
+    } else if (rows_hpsi_lds<T>(g, sh.ty, sh.tz, sh.nbw, 4, 1, 0, rows_nthr(g, dtype, sh.ty, sh.nbw)) > 65536) {
         return false;
     }
     const T* V = sizeof(T) == 4 ? (const T*)(const void*)h->V32.p : (const T*)(const void*)h->V64.p;
     double* part = nullptr;
     double* coef = nullptr;
     if (nl) {
-        const size_t npart = (size_t)std::max(1, rg->nslot) * rg->njs * nbands;
+        const size_t npart = (size_t)std::max(1, rgp->nslot) * rgp->njs * nbands;
         const size_t ncoef = (size_t)nbands * nl->geo.ncoef;
-        if (scratch) {
-            QG_REQUIRE(npart + ncoef <= scratch->capacity, "application exceeds the fixed scratch size");
-            part = scratch->base;
-            coef = scratch->base + npart;
+        if (ws) {
+            QG_REQUIRE(npart + ncoef <= ws->capacity, "application exceeds the fixed scratch size");
+            part = ws->base;
+            coef = ws->base + npart;
         } else {
             h->rows_part.alloc(npart);
             h->rows_coef.alloc(ncoef);
@@ -848,16 +976,17 @@ static bool rows_apply_t(qg_ham* h, int nbands, const T* psi, int64_t ld, T* hps
             coef = h->rows_coef.p;
         }
     }
-    if (h->timing) QG_HIP(hipEventRecord(h->tev[0], s));
+    const bool timed = h->timing && ws == nullptr;
+    if (timed) QG_HIP(hipEventRecord(h->tev[0], s));
     if (nl) {
-        if (rg->njs == 4) {
-            if (rg->sp4) rows_launch_project<T, 4, true>(nl, *rg, sh.nbp, nbands, psi, ld, part, coef, s);
-            else rows_launch_project<T, 4, false>(nl, *rg, sh.nbp, nbands, psi, ld, part, coef, s);
+        if (rgp->njs == 4) {
+            if (rgp->sp4) rows_launch_project<T, 4, true>(nl, *rgp, sh.nbp, nbands, psi, ld, part, coef, s);
+            else rows_launch_project<T, 4, false>(nl, *rgp, sh.nbp, nbands, psi, ld, part, coef, s);
         } else {
-            rows_launch_project<T, 8, false>(nl, *rg, sh.nbp, nbands, psi, ld, part, coef, s);
+            rows_launch_project<T, 8, false>(nl, *rgp, sh.nbp, nbands, psi, ld, part, coef, s);
         }
     }
-    if (h->timing) QG_HIP(hipEventRecord(h->tev[1], s));
+    if (timed) QG_HIP(hipEventRecord(h->tev[1], s));
     if (!nl || rg->njs == 4) {
         if (nl && rg->sp4)
             rows_launch_hpsi<T, 4, true>(g, sh, rg, nl, nbands, psi, ld, hpsi, ld_out, -0.5, 1, 0, V, coef, s);
@@ -866,12 +995,12 @@ static bool rows_apply_t(qg_ham* h, int nbands, const T* psi, int64_t ld, T* hps
     } else {
         rows_launch_hpsi<T, 8, false>(g, sh, rg, nl, nbands, psi, ld, hpsi, ld_out, -0.5, 1, 0, V, coef, s);
     }
-    if (h->timing) QG_HIP(hipEventRecord(h->tev[2], s));
+    if (timed) QG_HIP(hipEventRecord(h->tev[2], s));
     return true;
 }
 
 bool rows_hamiltonian_apply(qg_ham* h, int dtype, int nbands, const void* psi, int64_t ld, void* hpsi,
-                            int64_t ld_out, hipStream_t s, RowsScratch* scratch) {
+                            int64_t ld_out, hipStream_t s, RowsScratch* ws) {
     if (h->timing) {
         while (h->tev.size() < 3) {
             hipEvent_t e;
@@ -881,9 +1010,9 @@ bool rows_hamiltonian_apply(qg_ham* h, int dtype, int nbands, const void* psi, i
     }
     bool done;
     if (dtype == QG_F32)
-        done = rows_apply_t<float>(h, nbands, (const float*)psi, ld, (float*)hpsi, ld_out, s, scratch);
+        done = rows_apply_t<float>(h, nbands, (const float*)psi, ld, (float*)hpsi, ld_out, s, ws);
     else
-        done = rows_apply_t<double>(h, nbands, (const double*)psi, ld, (double*)hpsi, ld_out, s, scratch);
+        done = rows_apply_t<double>(h, nbands, (const double*)psi, ld, (double*)hpsi, ld_out, s, ws);
     if (done && h->timing) {
         h->tchunks = 1;
         h->timed_valid = true;
@@ -897,9 +1026,10 @@ bool rows_nonlocal_project(qg_nonlocal* nl, int dtype, int nbands, const void* p
                            DevBuf<double>& part_buf, hipStream_t s) {
     const qg_grid& g = nl->gcopy;
     if (nl->geo.ncoef == 0 || !rows_eligible(g, dtype, nl)) return false;
-    const RowsShape sh = rows_shape(g, dtype, nbands);
-    const RowsGeo* rg = rows_geometry(nl, sh.ty, sh.tzp);
-    if (!rg->ok) return false;
+    const RowsShape sh = rows_shape(g, dtype);
+    if (sh.typ == 0) return false;
+    const RowsGeo* rg = rows_geometry(nl, sh.typ, sh.tzp);
+    if (!rg->ok || rg->max_items > kProjThreads * kItemsPT) return false;
     const size_t lds = dtype == QG_F32 ? rows_proj_lds<float>(*rg, g, sh.nbp, (int)nl->geo.projs.size())
                                        : rows_proj_lds<double>(*rg, g, sh.nbp, (int)nl->geo.projs.size());
     if (lds > 65536) return false;
@@ -924,17 +1054,32 @@ bool rows_nonlocal_project(qg_nonlocal* nl, int dtype, int nbands, const void* p
 bool rows_laplacian_apply(const qg_grid& g, int dtype, int nbands, const void* psi, int64_t ld, void* out,
                           int64_t ld_out, double coef, int accumulate, hipStream_t s) {
     if (!rows_eligible(g, dtype, nullptr)) return false;
-    const RowsShape sh = rows_shape(g, dtype, nbands);
+    const RowsShape sh = rows_shape(g, dtype);
     if (dtype == QG_F32) {
-        if (rows_hpsi_lds<float>(g, sh.ty, sh.nbw, 4, 1, 0) > 65536) return false;
+        if (rows_hpsi_lds<float>(g, sh.ty, sh.tz, sh.nbw, 4, 1, 0, rows_nthr(g, dtype, sh.ty, sh.nbw)) > 65536)
+            return false;
         rows_launch_hpsi<float, 4, false>(g, sh, nullptr, nullptr, nbands, (const float*)psi, ld, (float*)out, ld_out,
                                           coef, 0, accumulate, nullptr, nullptr, s);
     } else {
-        if (rows_hpsi_lds<double>(g, sh.ty, sh.nbw, 4, 1, 0) > 65536) return false;
+        if (rows_hpsi_lds<double>(g, sh.ty, sh.tz, sh.nbw, 4, 1, 0, rows_nthr(g, dtype, sh.ty, sh.nbw)) > 65536)
+            return false;
         rows_launch_hpsi<double, 4, false>(g, sh, nullptr, nullptr, nbands, (const double*)psi, ld, (double*)out,
                                            ld_out, coef, 0, accumulate, nullptr, nullptr, s);
     }
     return true;
+}
+
+// Builds the row geometry of the default launch shapes (FP32 and FP64) of nl, so that
+// applications find it ready (and applications on caller scratch need not allocate).
+void rows_prepare(qg_nonlocal* nl) {
+    if (nl->geo.ncoef == 0) return;
+    for (int dtype : {QG_F32, QG_F64}) {
+        if (!rows_eligible(nl->gcopy, dtype, nl)) continue;
+        const RowsShape sh = rows_shape(nl->gcopy, dtype);
+        if (sh.typ == 0) continue;
+        rows_geometry(nl, sh.ty, sh.tzp);
+        rows_geometry(nl, sh.typ, sh.tzp);
+    }
 }
 
 }  // namespace qg

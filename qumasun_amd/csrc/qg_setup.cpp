@@ -449,6 +449,7 @@ bool build_rows_geometry(const qg_grid& g, const NlGeometry& geo, int ty, int tz
                 }
             }
             po[nc] = n;
+            o.max_items = std::max(o.max_items, n);
             o.it_off[tp + 1] = o.it_off[tp] + n;
             if ((int64_t)o.it_off[tp] + n >= (1LL << 31) - 1) return false;
         }
