@@ -130,6 +130,16 @@ int qg_hamiltonian_set_potential(qg_ham* h, const double* V_host);
  * Composition order of mHamiltonianMatrix_ddm (zero, V*psi, +ProjectionPP, +Laplacian). */
 int qg_hamiltonian_apply(qg_ham* h, int dtype, int nbands, const void* psi, int64_t ld,
                          void* hpsi, int64_t ld_out, void* stream);
+/* Same result as qg_hamiltonian_apply with no device memory allocated or freed, for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  It runs the row kernels only (Nx a
+ * multiple of 4 in FP32 or of 2 in FP64, 8 <= Nx <= 1024, Gaussian projectors with l <= 1)
+ * and keeps the projector partials and coefficients in one fixed 64 MiB device array per GPU,
+ * processing the bands in chunks that fit it; QG_ERR_ARG when one band does not fit or the
+ * projector geometry is not prepared (qg_nonlocal_create/update prepare it).  Calls on one GPU
+ * share that array: calls that overlap on one GPU (other streams, other threads) race. */
+int qg_hamiltonian_apply_noalloc(qg_ham* h, int dtype, int nbands, const void* psi, int64_t ld,
+                                 void* hpsi, int64_t ld_out, void* stream);
 /* Sector timers of qg_hamiltonian_apply on this handle (hipEvents recorded on the apply's
  * stream), the counterpart of the reference's StopWatch sectors 13 (PP nonlocal) and
  * 11/12 (kinetic, V) (qumasun_mpi.matrix.h:30-41): t_project_ms is the total time of the

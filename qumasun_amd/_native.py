@@ -83,6 +83,8 @@ SIGNATURES = {
     "qg_hamiltonian_set_potential": (c_int, [c_void_p, P(c_double)]),
     "qg_hamiltonian_apply": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, c_void_p, c_int64,
                                      c_void_p]),
+    "qg_hamiltonian_apply_noalloc": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, c_void_p, c_int64,
+                                             c_void_p]),
     "qg_hamiltonian_set_timing": (c_int, [c_void_p, c_int]),
     "qg_hamiltonian_last_times": (c_int, [c_void_p, P(c_double), P(c_double)]),
     "qg_hamiltonian_last_chunks": (c_int, [c_void_p, P(c_int)]),

@@ -1600,6 +1600,8 @@ int qg_hamiltonian_create(const qg_grid* g, const double* V, qg_nonlocal* nl, qg
         if (nl)
             QG_REQUIRE(nl->gcopy.n[0] == g->n[0] && nl->gcopy.n[1] == g->n[1] && nl->gcopy.n[2] == g->n[2],
                        "nonlocal handle was built for another grid");
+        // device address of the allocation-free entry's scratch on this device (set-up, once per device)
+        (void)scratch_address();
         qg_ham* h = new qg_ham;
         h->grid = *g;
         h->nl = nl;
@@ -1638,6 +1640,16 @@ int qg_hamiltonian_apply(qg_ham* h, int dtype, int nbands, const void* psi, int6
         check_block(&h->grid, dtype, nbands, psi, ld, hpsi, ld_out);
         if (nbands == 0) return;
         hamiltonian_apply(h, dtype, nbands, psi, ld, hpsi, ld_out, as_stream(stream));
+    });
+}
+
+int qg_hamiltonian_apply_noalloc(qg_ham* h, int dtype, int nbands, const void* psi, int64_t ld, void* hpsi,
+                                 int64_t ld_out, void* stream) {
+    return guarded([&] {
+        QG_REQUIRE(h, "null hamiltonian");
+        check_block(&h->grid, dtype, nbands, psi, ld, hpsi, ld_out);
+        if (nbands == 0) return;
+        rows_apply_noalloc(h, dtype, nbands, psi, ld, hpsi, ld_out, as_stream(stream));
     });
 }
 

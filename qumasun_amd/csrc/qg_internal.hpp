@@ -254,6 +254,9 @@ bool rows_nonlocal_project(qg_nonlocal* nl, int dtype, int nbands, const void* p
 bool rows_laplacian_apply(const qg_grid& g, int dtype, int nbands, const void* psi, int64_t ld, void* out,
                           int64_t ld_out, double coef, int accumulate, hipStream_t s);
 void rows_prepare(qg_nonlocal* nl);
+double* scratch_address();
+void rows_apply_noalloc(qg_ham* h, int dtype, int nbands, const void* psi, int64_t ld, void* hpsi, int64_t ld_out,
+                        hipStream_t s);
 }  // namespace qg
 
 struct qg_ham {

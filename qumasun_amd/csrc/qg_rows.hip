@@ -1082,4 +1082,72 @@ void rows_prepare(qg_nonlocal* nl) {
     }
 }
 
+// ------------------------------------------------ allocation-free application ----
+// Fixed device scratch of qg_hamiltonian_apply_noalloc: the FP64 projector partials and
+// coefficients of one band chunk.  Every GPU the library runs on holds one copy of it (64 MiB)
+// until the process ends; calls on one GPU share it, so calls that overlap on one GPU race.
+constexpr size_t kScratchDoubles = size_t(8) << 20;
+__device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+double* g_scratch_addr[kMaxDevices] = {};
+}  // namespace
+
+// Device address of `scratch` on the current device, looked up once per device (the array's
+// name in host code is only a host-side stand-in for the symbol).
+double* scratch_address() {
+    int dev = 0;
+    QG_HIP(hipGetDevice(&dev));
+    QG_REQUIRE(dev >= 0 && dev < kMaxDevices, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void* p = nullptr;
+        QG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        QG_REQUIRE(p != nullptr, "no device address for the scratch array");
+        g_scratch_addr[dev] = static_cast<double*>(p);
+    }
+    return g_scratch_addr[dev];
+}
+
+// H psi through the row kernels with the fixed scratch: band chunks sized so that one chunk's
+// partials and coefficients fit it, launched in stream order.  No device memory is allocated
+// or freed; the projector geometry must have been prepared (qg_nonlocal_create/update do).
+void rows_apply_noalloc(qg_ham* h, int dtype, int nbands, const void* psi, int64_t ld, void* hpsi, int64_t ld_out,
+                        hipStream_t s) {
+    const qg_grid& g = h->grid;
+    qg_nonlocal* nl = (h->nl && h->nl->geo.ncoef > 0) ? h->nl : nullptr;
+    QG_REQUIRE(rows_eligible(g, dtype, nl),
+               "the allocation-free H psi runs the row kernels only: x rows of whole 16-byte vectors (Nx a multiple "
+               "of 4 in FP32, of 2 in FP64, 8 <= Nx <= 1024), Gaussian projectors with l <= 1, at most 8 "
+               "coefficients per atom");
+    RowsScratch ws{scratch_address(), kScratchDoubles};
+    int chunk = nbands;
+    if (nl) {
+        const RowsShape sh = rows_shape(g, dtype);
+        QG_REQUIRE(sh.typ > 0, "the projection tile does not fit this grid");
+        const RowsGeo* rgp = rows_geometry(nl, sh.typ, sh.tzp, false);
+        QG_REQUIRE(rgp && rgp->ok, "the projector geometry of this launch shape is not prepared (qg_nonlocal_create "
+                                   "and qg_nonlocal_update prepare the default shapes)");
+        const size_t per_band = (size_t)std::max(1, rgp->nslot) * rgp->njs + (size_t)nl->geo.ncoef;
+        QG_REQUIRE(per_band <= kScratchDoubles,
+                   "one band needs " + std::to_string(per_band) + " doubles of projector scratch, more than the fixed " +
+                       std::to_string(kScratchDoubles));
+        chunk = (int)std::min<size_t>((size_t)nbands, kScratchDoubles / per_band);
+    }
+    const size_t es = dtype == QG_F32 ? 4 : 8;
+    for (int b0 = 0; b0 < nbands; b0 += chunk) {
+        const int nb = std::min(chunk, nbands - b0);
+        const char* pin = (const char*)psi + (size_t)b0 * ld * es;
+        char* pout = (char*)hpsi + (size_t)b0 * ld_out * es;
+        bool done;
+        if (dtype == QG_F32)
+            done = rows_apply_t<float>(h, nb, (const float*)pin, ld, (float*)pout, ld_out, s, &ws);
+        else
+            done = rows_apply_t<double>(h, nb, (const double*)pin, ld, (double*)pout, ld_out, s, &ws);
+        QG_REQUIRE(done, "the row kernels do not serve this grid and projector set (tile does not fit in LDS)");
+    }
+}
+
 }  // namespace qg
