@@ -167,26 +167,34 @@ struct TileNL {
 void build_tile_nl(const qg_grid& g, const NlGeometry& geo, int ty, int tz, bool segments, TileNL& out);
 
 // Geometry of the row kernels (qg_rows.hip) for y-tiles of ty rows and projection slabs of
-// tzp planes: the periodic sphere images of every tile column, the sphere points of every
-// (tile, plane) as items (row | x start << 5 | (count - 1) << 15 | column image << 17) grouped
-// by image, and the projection's per-thread image ownership with its partial slots.
+// tzp planes: the periodic sphere images of every tile column and the sphere points of every
+// (tile, plane) as items (row | x start << 5 | (count - 1) << 15 | column image << 17).
+// Projection order: items of up to 4 consecutive points grouped by image, with the
+// projection's per-thread image ownership and its partial slots.  Group order (the apply
+// kernel): items cut at multiples of 4 in x, so each lies in one 4-point group of its row,
+// sorted by (row, group, image), with the offsets of every group's run of items.
 struct RowsGeo {
     int ty = 0, tzp = 0, ntile = 0, nslabp = 0;
+    bool by_group = false;
+    int ng4 = 0;         // 4-point groups per row (group order)
     int max_col = 0;     // most images of one tile column
     int max_items = 0;   // most items of one (tile, plane)
     int nslot = 0;       // projection partial slots (one per active (tile, slab, image))
     int njs = 4;         // coefficient slots per atom (4 or 8)
     bool sp4 = false;    // every image: one s channel (slot 0) and one p channel (slots 1..3)
+    int pform = 0;       // projector form of the kernels: 0 general, 1 s + p, 2 s + p with one exponent
     bool ok = false;     // the geometry fits the encodings
     std::vector<int> col_off, col_img, it_off, pl_base, pl_off, thr_slot, atom_slot_off, atom_slot;
+    std::vector<int> grp_off;   // group order: per (tile, plane) ty * ng4 + 1 run offsets (plane-relative)
     std::vector<uint32_t> items, thr_map;
     DevBuf<int> d_col_off, d_col_img, d_it_off, d_pl_base, d_pl_off, d_thr_slot, d_atom_slot_off, d_atom_slot;
+    DevBuf<int> d_grp_off;
     DevBuf<uint32_t> d_items, d_thr_map;
 };
 
 // Threads of a projection workgroup of the row kernels (the thread map is built for it).
 constexpr int kRowsProjThreads = 256;
-bool build_rows_geometry(const qg_grid& g, const NlGeometry& geo, int ty, int tzp, RowsGeo& o);
+bool build_rows_geometry(const qg_grid& g, const NlGeometry& geo, int ty, int tzp, bool by_group, RowsGeo& o);
 
 // Caller-provided FP64 scratch of one application (projector partials, then coefficients).
 struct RowsScratch {

@@ -79,10 +79,18 @@ __device__ __forceinline__ T gauss(T rr, T k) {
     if constexpr (sizeof(T) == 4) return exp2f(-(rr * k));
     else return exp(-(rr * k));
 }
+// 1/sqrt(rr): the hardware FP32 estimate; in FP64 that estimate refined by two Newton steps
+// y <- y (3 - rr y^2) / 2 (relative error ~1e-7 -> ~1e-14 -> rounding), rr in [1e-14, 1e30]
 template <class T>
 __device__ __forceinline__ T inv_sqrt(T rr) {
-    if constexpr (sizeof(T) == 4) return __frsqrt_rn(rr);
-    else return 1.0 / sqrt(rr);
+    if constexpr (sizeof(T) == 4) {
+        return __frsqrt_rn(rr);
+    } else {
+        double y = (double)__frsqrt_rn((float)rr);
+        y = y * (1.5 - 0.5 * rr * y * y);
+        y = y * (1.5 - 0.5 * rr * y * y);
+        return y;
+    }
 }
 
 }  // namespace
@@ -110,6 +118,9 @@ struct RowsHpsiArgs {
     const int* it_off;          // per (tile, plane): first item
     const uint32_t* items;
     int nitems;                 // total items (item loads are clamped to the array)
+    const int* grp_off;         // per (tile, plane): ty * ng4 + 1 run offsets of the 4-point groups
+    int ng4;                    // 4-point groups per row
+    int stg_cap;                // item capacity of the per-item value area (the most items of a plane)
     const DevImage* images;
     const DevProj* projs;
     int nproj;
@@ -180,18 +191,26 @@ __device__ __forceinline__ void stage_proj(const ProjLds<T>& L, const DevImage* 
 }
 
 // beta_j at point (ix, iy, iz) of column image li, for coefficient slots j < NJS (zero for
-// slots of projectors the image does not hold).  SP4: the image holds exactly an s channel
-// in slot 0 and a p channel in slots 1..3.
-template <class T, int NJS, bool SP4>
+// slots of projectors the image does not hold).  Projector form PF: 0 general; 1 the image
+// holds exactly an s channel in slot 0 and a p channel in slots 1..3; 2 as 1 with one Gaussian
+// exponent for both channels (one exponential per point).
+template <class T, int NJS, int PF>
 __device__ __forceinline__ void beta_at(const ProjLds<T>& L, int li, T dx, T dy, T dz, T (&beta)[NJS]) {
     const T rr = dx * dx + (dy * dy + dz * dz);
     T ir = T(0);
     if (!(T(1.0e-14) > rr)) ir = inv_sqrt<T>(rr);
     const int* ii = L.im_i + 5 * li;
-    if constexpr (SP4) {
+    if constexpr (PF != 0) {
         const int pb = ii[3];
-        const T es = L.pr_f[2 * pb] * gauss<T>(rr, L.pr_f[2 * pb + 1]);
-        const T ep = L.pr_f[2 * pb + 2] * gauss<T>(rr, L.pr_f[2 * pb + 3]) * ir;
+        T es, ep;
+        if constexpr (PF == 2) {
+            const T e = gauss<T>(rr, L.pr_f[2 * pb + 1]);
+            es = L.pr_f[2 * pb] * e;
+            ep = L.pr_f[2 * pb + 2] * e * ir;
+        } else {
+            es = L.pr_f[2 * pb] * gauss<T>(rr, L.pr_f[2 * pb + 1]);
+            ep = L.pr_f[2 * pb + 2] * gauss<T>(rr, L.pr_f[2 * pb + 3]) * ir;
+        }
         beta[0] = es;
         beta[1] = ep * dy;
         beta[2] = ep * dz;
@@ -223,11 +242,12 @@ __device__ __forceinline__ void beta_at(const ProjLds<T>& L, int li, T dx, T dy,
 constexpr int kProjThreads = kRowsProjThreads;
 constexpr int kProjPV = 8;      // staged psi vectors per thread per plane
 constexpr int kItemsPT = 4;     // staged items per thread per plane (both kernels)
+constexpr int kGroupsPT = 2;    // staged group run offsets per thread per plane (apply kernel)
 
 // LDS offset rounded up to 16 bytes
 __host__ __device__ constexpr size_t align16(size_t b) { return (b + 15) & ~size_t(15); }
 
-template <class T, int NBP, int NJS, bool SP4>
+template <class T, int NBP, int NJS, int PF>
 __global__ __launch_bounds__(kProjThreads) void rows_project_kernel(RowsProjArgs a, const T* __restrict__ psi) {
     constexpr int VN = 16 / sizeof(T);
     extern __shared__ __align__(16) unsigned char smem[];
@@ -358,7 +378,7 @@ __global__ __launch_bounds__(kProjThreads) void rows_project_kernel(RowsProjArgs
                     if (p < np) {
                         const T dx = hx * ((T)(xs + p - ii[0]) - ff[0]);
                         T beta[NJS];
-                        beta_at<T, NJS, SP4>(L, li, dx, dy, dz, beta);
+                        beta_at<T, NJS, PF>(L, li, dx, dy, dz, beta);
                         const Row<T>* pv = reinterpret_cast<const Row<T>*>(prow + p * NBP);
 #pragma unroll
                         for (int q = 0; q < NBP / VN; ++q) {
@@ -417,7 +437,7 @@ constexpr int kRowsMaxThreads = 1024;
 // the hardware returns zero for the load and drops the store
 constexpr int kOOB = 0x7ffffff0;
 
-template <class T, int NBW, int NJS, bool SP4, bool NL>
+template <class T, int NBW, int NJS, int PF, bool NL>
 __global__ __launch_bounds__(NBW >= 4 ? 1024 : 512) void rows_hpsi_kernel(RowsHpsiArgs a, Coef8<T> sc,
                                                                           const T* __restrict__ V,
                                                                           const T* __restrict__ psi,
@@ -430,14 +450,17 @@ __global__ __launch_bounds__(NBW >= 4 ? 1024 : 512) void rows_hpsi_kernel(RowsHp
     const int ty = a.ty;
     const int lband = (ty + 8) * W;
     const int nthr = blockDim.x;
-    // LDS layout (rows_hpsi_lds on the host): planes, projector term, E*c, items, the slab's
-    // per-plane item offsets, projector staging
+    // LDS layout (rows_hpsi_lds on the host): planes, per-item projector values, E*c, items,
+    // group run offsets, the slab's per-plane item offsets, projector staging
+    const int ngo = ty * a.ng4;                                // groups of a tile plane
     T* planes = reinterpret_cast<T*>(smem);                    // [2][NBW][ty + 8][W]
-    T* nlb = planes + (size_t)2 * NBW * lband;                 // [NBW][ty][nx]
-    T* ec = nlb + (size_t)NBW * ty * nx;                       // [max_col][NBW][NJS]
-    size_t off = sizeof(T) * ((size_t)2 * NBW * lband + (size_t)NBW * ty * nx + (size_t)a.max_col * NBW * NJS);
+    T* stg = planes + (size_t)2 * NBW * lband;                 // [stg_cap][NBW][4]
+    T* ec = stg + (size_t)a.stg_cap * NBW * 4;                 // [max_col][NBW][NJS]
+    size_t off = sizeof(T) * ((size_t)2 * NBW * lband + (size_t)a.stg_cap * NBW * 4 + (size_t)a.max_col * NBW * NJS);
     uint32_t* itl = reinterpret_cast<uint32_t*>(smem + off);   // [nthr * kItemsPT]
     off += sizeof(uint32_t) * (size_t)nthr * kItemsPT;
+    int* gol = reinterpret_cast<int*>(smem + off);             // [ngo + 1]
+    off += sizeof(int) * (size_t)(ngo + 1);
     int* itoffl = reinterpret_cast<int*>(smem + off);          // [tz + 1]
     off = align16(off + sizeof(int) * (a.tz + 1));
     unsigned char* lp = smem + off;
@@ -516,7 +539,6 @@ __global__ __launch_bounds__(NBW >= 4 ? 1024 : 512) void rows_hpsi_kernel(RowsHp
                 v = (T)(a.coef_energy[cb + j] * a.coef[(int64_t)(b0 + b) * a.ncoef + cb + j]);
             ec[i] = v;
         }
-        for (int i = tid; i < NBW * ty * nx; i += nthr) nlb[i] = T(0);
         for (int i = tid; i <= z1 - z0; i += nthr) itoffl[i] = a.it_off[(size_t)tile * nz + z0 + i];
         __syncthreads();
     }
@@ -534,6 +556,7 @@ __global__ __launch_bounds__(NBW >= 4 ? 1024 : 512) void rows_hpsi_kernel(RowsHp
     // the projector phase reads LDS only: these loads stay in flight across the plane's work.
     Row<T> qn{}, hn{}, vn{};
     uint32_t ipre[kItemsPT];
+    int gpre[kGroupsPT];
     const int lim = max(a.nitems - 1, 0);
     auto prefetch = [&](int z) {
         qn = vload<T>(rin, vo, zoff(z + 4));
@@ -543,6 +566,9 @@ __global__ __launch_bounds__(NBW >= 4 ? 1024 : 512) void rows_hpsi_kernel(RowsHp
             const int ib = itoffl[z - z0];
 #pragma unroll
             for (int k = 0; k < kItemsPT; ++k) ipre[k] = a.items[min(ib + tid + nthr * k, lim)];
+            const int* gp = a.grp_off + ((size_t)tile * nz + z) * (ngo + 1);
+#pragma unroll
+            for (int k = 0; k < kGroupsPT; ++k) gpre[k] = gp[min(tid + nthr * k, ngo)];
         }
     };
     prefetch(z0);
@@ -570,19 +596,28 @@ __global__ __launch_bounds__(NBW >= 4 ? 1024 : 512) void rows_hpsi_kernel(RowsHp
         }
         int nit = 0;
         if constexpr (NL) {
-            // this plane's items (the previous plane's readers passed its second barrier)
+            // this plane's items and group runs (the previous plane's readers passed its second
+            // barrier)
             nit = itoffl[z - z0 + 1] - itoffl[z - z0];
 #pragma unroll
             for (int k = 0; k < kItemsPT; ++k) {
                 const int i = tid + nthr * k;
                 if (i < nit) itl[i] = ipre[k];
             }
+#pragma unroll
+            for (int k = 0; k < kGroupsPT; ++k) {
+                const int i = tid + nthr * k;
+                if (i <= ngo) gol[i] = gpre[k];
+            }
         }
         if (z + 1 < z1) prefetch(z + 1);
         __syncthreads();
 
         if constexpr (NL) {
-            // projector term of this plane: items dealt over all threads of the workgroup
+            // projector values of this plane's items, dealt over all threads of the workgroup:
+            // each item (points of one image in one 4-point group) is evaluated once for the
+            // workgroup's bands and its values are written to the item's own slots (no two
+            // threads write one slot, no atomics)
             for (int i = tid; i < nit; i += nthr) {
                 const uint32_t it = itl[i];
                 const int ir = (int)(it & 31u), xs = (int)((it >> 5) & 1023u);
@@ -602,21 +637,34 @@ __global__ __launch_bounds__(NBW >= 4 ? 1024 : 512) void rows_hpsi_kernel(RowsHp
                         for (int u = 0; u < VN; ++u) e[b][q4 * VN + u] = w.v[u];
                     }
                 }
-                T* nrow = nlb + (size_t)ir * nx + xs;
+                const int gx = xs & ~3, u0 = xs & 3;
+                T val[NBW][4];
 #pragma unroll
-                for (int p = 0; p < 4; ++p) {
-                    if (p < np) {
-                        const T dx = hx * ((T)(xs + p - ii[0]) - ff[0]);
+                for (int u = 0; u < 4; ++u) {
+#pragma unroll
+                    for (int b = 0; b < NBW; ++b) val[b][u] = T(0);
+                    if (u >= u0 && u < u0 + np) {
+                        const T dx = hx * ((T)(gx + u - ii[0]) - ff[0]);
                         T beta[NJS];
-                        beta_at<T, NJS, SP4>(L, li, dx, dy, dz, beta);
+                        beta_at<T, NJS, PF>(L, li, dx, dy, dz, beta);
 #pragma unroll
                         for (int b = 0; b < NBW; ++b) {
                             T v = T(0);
 #pragma unroll
                             for (int j = 0; j < NJS; ++j) v += e[b][j] * beta[j];
-                            __hip_atomic_fetch_add(nrow + (size_t)b * ty * nx + p, v, __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                            val[b][u] = v;
                         }
+                    }
+                }
+#pragma unroll
+                for (int b = 0; b < NBW; ++b) {
+                    Row<T>* dst = reinterpret_cast<Row<T>*>(stg + ((size_t)i * NBW + b) * 4);
+#pragma unroll
+                    for (int h = 0; h < 4 / VN; ++h) {
+                        Row<T> w;
+#pragma unroll
+                        for (int u = 0; u < VN; ++u) w.v[u] = val[b][h * VN + u];
+                        dst[h] = w;
                     }
                 }
             }
@@ -628,11 +676,15 @@ __global__ __launch_bounds__(NBW >= 4 ? 1024 : 512) void rows_hpsi_kernel(RowsHp
 #pragma unroll
             for (int i = 0; i < VN; ++i) res.v[i] = a.add_v ? vcur.v[i] * q[4].v[i] : vcur.v[i];
             if constexpr (NL) {
-                Row<T>* np = reinterpret_cast<Row<T>*>(nlb + ((size_t)bl * ty + r) * nx + x);
-                const Row<T> nlv = *np;
-                *np = Row<T>{};
+                // projector term: the values of the items of this point's 4-point group
+                const int gi = r * a.ng4 + (x >> 2);
+                const int s1 = gol[gi + 1];
+                const int h = (VN == 4) ? 0 : ((x >> 1) & 1);   // FP64: which half of the group
+                for (int i = gol[gi]; i < s1; ++i) {
+                    const Row<T> w = reinterpret_cast<const Row<T>*>(stg + ((size_t)i * NBW + bl) * 4)[h];
 #pragma unroll
-                for (int i = 0; i < VN; ++i) res.v[i] += nlv.v[i];
+                    for (int u = 0; u < VN; ++u) res.v[u] += w.v[u];
+                }
             }
             // stencil: centre and z from the register queue, x and y from LDS, accumulated
             // term by term so few values are live at once
@@ -708,6 +760,9 @@ static int rows_pick_ty(int nq, int ny, int want) {
         while (ty > 8 && round64i(ty * nq) > kRowsBandThreads) --ty;
         return ty;
     }
+    // 8 rows (more workgroups, less LDS: measured best at 80^3 and 120^3) unless that leaves
+    // more than a quarter of the lanes idle
+    if (8 <= ny && 4 * 8 * nq >= 3 * round64i(8 * nq)) return 8;
     int best = 8, bests = -1;
     for (int ty : {8, 16, 12, 10, 6, 20, 24, 4, 32}) {
         if (ty > ny) continue;
@@ -753,16 +808,28 @@ struct RowsShape {
     int nbw, nbp;   // bands per workgroup of the apply and projection kernels
 };
 
-static RowsShape rows_shape(const qg_grid& g, int dtype) {
+// Launch shape for nbands bands (0: only the geometry-defining fields, ty and tzp, which do not
+// depend on the band count).  QG_ROWS_* environment variables override (measurement switches).
+static RowsShape rows_shape(const qg_grid& g, int dtype, int nbands) {
     const int VN = dtype == QG_F32 ? 4 : 2;
     const int nq = g.n[0] / VN;
     RowsShape sh{};
     sh.ty = rows_pick_ty(nq, g.n[1], env_int("QG_ROWS_TY", 0));
-    sh.nbw = env_int("QG_ROWS_NBW", dtype == QG_F32 ? 2 : 1);
-    if (sh.nbw != 1 && sh.nbw != 2 && sh.nbw != 4) sh.nbw = 2;
-    while (sh.nbw > 1 && sh.nbw * round64i(std::max(sh.ty, 8) * nq) > (sh.nbw >= 4 ? 1024 : 512)) sh.nbw /= 2;
-    sh.tz = std::max(1, std::min(g.n[2], env_int("QG_ROWS_TZ", g.n[2])));
-    sh.tzp = std::max(1, std::min(g.n[2], env_int("QG_ROWS_TZP", 16)));
+    const int ntile = std::max(1, (g.n[1] + std::max(1, sh.ty) - 1) / std::max(1, sh.ty));
+    // about 4096 workgroups per launch: 2 bands per workgroup (FP32) when the tiles and bands
+    // give that many, else 1, and z-slabs (at least 8 planes each) for the rest
+    constexpr int kWantWg = 4096;
+    int nbw = 1;
+    if (dtype == QG_F32 && (int64_t)ntile * ((nbands + 1) / 2) >= kWantWg / 2) nbw = 2;
+    nbw = env_int("QG_ROWS_NBW", nbw);
+    if (nbw != 1 && nbw != 2 && nbw != 4) nbw = 1;
+    while (nbw > 1 && nbw * round64i(std::max(sh.ty, 8) * nq) > (nbw >= 4 ? 1024 : 512)) nbw /= 2;
+    sh.nbw = nbw;
+    const int64_t wg = (int64_t)ntile * std::max(1, (nbands + nbw - 1) / nbw);
+    const int nslab = (int)std::max<int64_t>(1, std::min<int64_t>((kWantWg + wg - 1) / wg, std::max(1, g.n[2] / 8)));
+    sh.tz = env_int("QG_ROWS_TZ", (g.n[2] + nslab - 1) / nslab);
+    sh.tz = std::max(1, std::min(g.n[2], sh.tz));
+    sh.tzp = std::max(1, std::min(g.n[2], env_int("QG_ROWS_TZP", 8)));
     // FP64 projection with 4 bands per workgroup: its reduction buffer (256 threads x 4 slots
     // x bands) stays at 32 KB
     sh.nbp = env_int("QG_ROWS_NBP", dtype == QG_F32 ? 8 : 4) == 4 ? 4 : 8;
@@ -770,15 +837,16 @@ static RowsShape rows_shape(const qg_grid& g, int dtype) {
     return sh;
 }
 
-// The row geometry of nl for tile rows ty and projection slabs of tzp planes, built (and
-// uploaded) on first use; with build == false only a cached one is returned (nullptr if none).
-static RowsGeo* rows_geometry(qg_nonlocal* nl, int ty, int tzp, bool build = true) {
+// The row geometry of nl for tile rows ty, projection slabs of tzp planes and item order
+// (by 4-point group for the apply kernel, by image for the projection), built (and uploaded)
+// on first use; with build == false only a cached one is returned (nullptr if none).
+static RowsGeo* rows_geometry(qg_nonlocal* nl, int ty, int tzp, bool by_group, bool build = true) {
     std::lock_guard<std::mutex> lk(nl->rows_mu);
     for (auto& c : nl->rows)
-        if (c && c->ty == ty && c->tzp == tzp) return c.get();
+        if (c && c->ty == ty && c->tzp == tzp && c->by_group == by_group) return c.get();
     if (!build) return nullptr;
     auto c = std::make_unique<RowsGeo>();
-    if (!build_rows_geometry(nl->gcopy, nl->geo, ty, tzp, *c)) {
+    if (!build_rows_geometry(nl->gcopy, nl->geo, ty, tzp, by_group, *c)) {
         c->ok = false;
     } else {
         c->ok = true;
@@ -796,13 +864,32 @@ static RowsGeo* rows_geometry(qg_nonlocal* nl, int ty, int tzp, bool build = tru
         up(c->d_thr_slot, c->thr_slot);
         up(c->d_atom_slot_off, c->atom_slot_off);
         up(c->d_atom_slot, c->atom_slot);
+        up(c->d_grp_off, c->grp_off);
         std::vector<uint32_t>().swap(c->items);
         std::vector<int>().swap(c->pl_off);
         std::vector<uint32_t>().swap(c->thr_map);
         std::vector<int>().swap(c->thr_slot);
+        std::vector<int>().swap(c->grp_off);
     }
     nl->rows.push_back(std::move(c));
     return nl->rows.back().get();
+}
+
+// Most LDS one workgroup may allocate on the current device (160 KB on MI355X), looked up once
+// per device.
+static size_t lds_limit() {
+    static std::mutex mu;
+    static size_t cached[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 65536;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!cached[dev]) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+            v = 65536;
+        cached[dev] = (size_t)v;
+    }
+    return cached[dev];
 }
 
 // LDS bytes of a projection workgroup (layout of rows_project_kernel)
@@ -815,19 +902,22 @@ static size_t rows_proj_lds(const RowsGeo& rg, const qg_grid& g, int nbp, int np
     return off + sizeof(T) * (3 * rg.max_col + 2 * nproj) + sizeof(int) * (5 * rg.max_col + 2 * nproj);
 }
 
-// LDS bytes of an apply workgroup of nthr threads (layout of rows_hpsi_kernel)
+// LDS bytes of an apply workgroup of nthr threads (layout of rows_hpsi_kernel); stg_cap items
+// of per-item values, ng4 groups per row
 template <class T>
-static size_t rows_hpsi_lds(const qg_grid& g, int ty, int tz, int nbw, int njs, int max_col, int nproj, int nthr) {
+static size_t rows_hpsi_lds(const qg_grid& g, int ty, int tz, int nbw, int njs, int max_col, int nproj, int nthr,
+                            int stg_cap, int ng4) {
     const size_t W = g.n[0] + 8;
-    size_t off = sizeof(T) * (2 * (size_t)nbw * (ty + 8) * W + (size_t)nbw * ty * g.n[0] + (size_t)max_col * nbw * njs);
+    size_t off = sizeof(T) * (2 * (size_t)nbw * (ty + 8) * W + (size_t)stg_cap * nbw * 4 + (size_t)max_col * nbw * njs);
     off += sizeof(uint32_t) * (size_t)nthr * kItemsPT;
+    off += sizeof(int) * (size_t)(ty * ng4 + 1);
     off = align16(off + sizeof(int) * (tz + 1));
     return off + sizeof(T) * (3 * (size_t)max_col + 2 * (size_t)nproj) +
            sizeof(int) * (5 * (size_t)max_col + 2 * (size_t)nproj);
 }
 
 // Scratch of one application: projector partials and coefficients (grown, never shrunk).
-template <class T, int NJS, bool SP4>
+template <class T, int NJS, int PF>
 static void rows_launch_project(qg_nonlocal* nl, const RowsGeo& rg, int nbp, int nbands, const T* psi, int64_t ld,
                                 double* part, double* coef, hipStream_t s) {
     const qg_grid& g = nl->gcopy;
@@ -850,13 +940,13 @@ static void rows_launch_project(qg_nonlocal* nl, const RowsGeo& rg, int nbp, int
                "projection tile does not fit the staging registers");
     QG_REQUIRE(rg.max_items <= kProjThreads * kItemsPT, "a tile plane holds more projector items than staged");
     const size_t lds = rows_proj_lds<T>(rg, g, nbp, a.nproj);
-    QG_REQUIRE(lds <= 65536, "projection tile does not fit in LDS");
+    QG_REQUIRE(lds <= lds_limit(), "projection tile does not fit in LDS");
     const int64_t nwg = (int64_t)rg.ntile * rg.nslabp * ((nbands + nbp - 1) / nbp);
     QG_REQUIRE(nwg < (1LL << 31), "too many workgroups");
     if (nbp == 8)
-        hipLaunchKernelGGL((rows_project_kernel<T, 8, NJS, SP4>), dim3((unsigned)nwg), dim3(kProjThreads), lds, s, a, psi);
+        hipLaunchKernelGGL((rows_project_kernel<T, 8, NJS, PF>), dim3((unsigned)nwg), dim3(kProjThreads), lds, s, a, psi);
     else
-        hipLaunchKernelGGL((rows_project_kernel<T, 4, NJS, SP4>), dim3((unsigned)nwg), dim3(kProjThreads), lds, s, a, psi);
+        hipLaunchKernelGGL((rows_project_kernel<T, 4, NJS, PF>), dim3((unsigned)nwg), dim3(kProjThreads), lds, s, a, psi);
     QG_HIP(hipGetLastError());
     const int64_t nred = (int64_t)nl->geo.natoms * NJS * nbands;
     if (nred > 0) {
@@ -867,7 +957,7 @@ static void rows_launch_project(qg_nonlocal* nl, const RowsGeo& rg, int nbp, int
     }
 }
 
-template <class T, int NBW, int NJS, bool SP4>
+template <class T, int NBW, int NJS, int PF>
 static void rows_launch_hpsi_t(const qg_grid& g, const RowsShape& sh, const RowsGeo* rg, qg_nonlocal* nl, int nbands,
                                const T* psi, int64_t ld_in, T* out, int64_t ld_out, double lap_coef, int add_v,
                                int accumulate, const T* V, const double* coef, hipStream_t s) {
@@ -889,6 +979,9 @@ static void rows_launch_hpsi_t(const qg_grid& g, const RowsShape& sh, const Rows
         a.col_off = rg->d_col_off.p; a.col_img = rg->d_col_img.p;
         a.it_off = rg->d_it_off.p; a.items = rg->d_items.p;
         a.nitems = rg->it_off.empty() ? 0 : rg->it_off.back();
+        a.grp_off = rg->d_grp_off.p;
+        a.ng4 = rg->ng4;
+        a.stg_cap = rg->max_items;
         a.images = nl->d_images.p; a.projs = nl->d_projs.p; a.nproj = (int)nl->geo.projs.size();
         a.coef = coef; a.coef_energy = nl->d_coef_energy.p; a.atom_coef_off = nl->d_atom_coef_off.p;
         a.ncoef = nl->geo.ncoef;
@@ -898,34 +991,35 @@ static void rows_launch_hpsi_t(const qg_grid& g, const RowsShape& sh, const Rows
     const int nthr = NBW * a.tpb;
     QG_REQUIRE(nthr <= (NBW >= 4 ? 1024 : 512), "row tile too wide for one workgroup");
     QG_REQUIRE(!rg || rg->max_items <= nthr * kItemsPT, "a tile plane holds more projector items than staged");
-    const size_t lds = rows_hpsi_lds<T>(g, sh.ty, sh.tz, NBW, NJS, a.max_col, a.nproj, nthr);
-    QG_REQUIRE(lds <= 65536, "row tile does not fit in LDS");
+    QG_REQUIRE(!rg || rg->ty * rg->ng4 + 1 <= nthr * kGroupsPT, "a tile plane holds more groups than staged");
+    const size_t lds = rows_hpsi_lds<T>(g, sh.ty, sh.tz, NBW, NJS, a.max_col, a.nproj, nthr, a.stg_cap, a.ng4);
+    QG_REQUIRE(lds <= lds_limit(), "row tile does not fit in LDS");
     const int64_t nwg = (int64_t)a.ntile * a.nslab * ((nbands + NBW - 1) / NBW);
     QG_REQUIRE(nwg < (1LL << 31), "too many workgroups");
     if (rg)
-        hipLaunchKernelGGL((rows_hpsi_kernel<T, NBW, NJS, SP4, true>), dim3((unsigned)nwg), dim3(nthr), lds, s, a,
+        hipLaunchKernelGGL((rows_hpsi_kernel<T, NBW, NJS, PF, true>), dim3((unsigned)nwg), dim3(nthr), lds, s, a,
                            coef8<T>(g, lap_coef), V, psi, out);
     else
-        hipLaunchKernelGGL((rows_hpsi_kernel<T, NBW, 4, false, false>), dim3((unsigned)nwg), dim3(nthr), lds, s, a,
+        hipLaunchKernelGGL((rows_hpsi_kernel<T, NBW, 4, 0, false>), dim3((unsigned)nwg), dim3(nthr), lds, s, a,
                            coef8<T>(g, lap_coef), V, psi, out);
     QG_HIP(hipGetLastError());
 }
 
-template <class T, int NJS, bool SP4>
+template <class T, int NJS, int PF>
 static void rows_launch_hpsi(const qg_grid& g, const RowsShape& sh, const RowsGeo* rg, qg_nonlocal* nl, int nbands,
                              const T* psi, int64_t ld_in, T* out, int64_t ld_out, double lap_coef, int add_v,
                              int accumulate, const T* V, const double* coef, hipStream_t s) {
     switch (sh.nbw) {
         case 4:
-            rows_launch_hpsi_t<T, 4, NJS, SP4>(g, sh, rg, nl, nbands, psi, ld_in, out, ld_out, lap_coef, add_v,
+            rows_launch_hpsi_t<T, 4, NJS, PF>(g, sh, rg, nl, nbands, psi, ld_in, out, ld_out, lap_coef, add_v,
                                                accumulate, V, coef, s);
             break;
         case 2:
-            rows_launch_hpsi_t<T, 2, NJS, SP4>(g, sh, rg, nl, nbands, psi, ld_in, out, ld_out, lap_coef, add_v,
+            rows_launch_hpsi_t<T, 2, NJS, PF>(g, sh, rg, nl, nbands, psi, ld_in, out, ld_out, lap_coef, add_v,
                                                accumulate, V, coef, s);
             break;
         default:
-            rows_launch_hpsi_t<T, 1, NJS, SP4>(g, sh, rg, nl, nbands, psi, ld_in, out, ld_out, lap_coef, add_v,
+            rows_launch_hpsi_t<T, 1, NJS, PF>(g, sh, rg, nl, nbands, psi, ld_in, out, ld_out, lap_coef, add_v,
                                                accumulate, V, coef, s);
     }
 }
@@ -939,24 +1033,26 @@ static bool rows_apply_t(qg_ham* h, int nbands, const T* psi, int64_t ld, T* hps
     qg_nonlocal* nl = (h->nl && h->nl->geo.ncoef > 0) ? h->nl : nullptr;
     const int dtype = sizeof(T) == 4 ? QG_F32 : QG_F64;
     if (!rows_eligible(g, dtype, nl)) return false;
-    const RowsShape sh = rows_shape(g, dtype);
+    const RowsShape sh = rows_shape(g, dtype, nbands);
     const RowsGeo* rg = nullptr;    // apply kernel's geometry (tile rows ty)
     const RowsGeo* rgp = nullptr;   // projection kernel's geometry (tile rows typ)
     if (nl) {
         if (sh.typ == 0) return false;
         // with caller scratch nothing may be allocated: the geometry must exist already
-        rg = rows_geometry(nl, sh.ty, sh.tzp, ws == nullptr);
-        rgp = sh.typ == sh.ty ? rg : rows_geometry(nl, sh.typ, sh.tzp, ws == nullptr);
+        rg = rows_geometry(nl, sh.ty, sh.tzp, true, ws == nullptr);
+        rgp = rows_geometry(nl, sh.typ, sh.tzp, false, ws == nullptr);
         QG_REQUIRE(rg && rgp, "the projector geometry of this launch shape is not prepared (qg_nonlocal_create and "
                               "qg_nonlocal_update prepare the default shapes)");
         if (!rg->ok || !rgp->ok) return false;
         const int nthr = rows_nthr(g, dtype, sh.ty, sh.nbw);
         if (rgp->max_items > kProjThreads * kItemsPT || rg->max_items > nthr * kItemsPT) return false;
-        if (rows_proj_lds<T>(*rgp, g, sh.nbp, (int)nl->geo.projs.size()) > 65536) return false;
+        if (rg->ty * rg->ng4 + 1 > nthr * kGroupsPT) return false;
+        if (rows_proj_lds<T>(*rgp, g, sh.nbp, (int)nl->geo.projs.size()) > lds_limit()) return false;
         if (rows_hpsi_lds<T>(g, sh.ty, sh.tz, sh.nbw, rg->njs, std::max(1, rg->max_col), (int)nl->geo.projs.size(),
-                             nthr) > 65536)
+                             nthr, rg->max_items, rg->ng4) > lds_limit())
             return false;
-    } else if (rows_hpsi_lds<T>(g, sh.ty, sh.tz, sh.nbw, 4, 1, 0, rows_nthr(g, dtype, sh.ty, sh.nbw)) > 65536) {
+    } else if (rows_hpsi_lds<T>(g, sh.ty, sh.tz, sh.nbw, 4, 1, 0, rows_nthr(g, dtype, sh.ty, sh.nbw), 0, 0) >
+               lds_limit()) {
         return false;
     }
     const T* V = sizeof(T) == 4 ? (const T*)(const void*)h->V32.p : (const T*)(const void*)h->V64.p;
@@ -980,20 +1076,23 @@ static bool rows_apply_t(qg_ham* h, int nbands, const T* psi, int64_t ld, T* hps
     if (timed) QG_HIP(hipEventRecord(h->tev[0], s));
     if (nl) {
         if (rgp->njs == 4) {
-            if (rgp->sp4) rows_launch_project<T, 4, true>(nl, *rgp, sh.nbp, nbands, psi, ld, part, coef, s);
-            else rows_launch_project<T, 4, false>(nl, *rgp, sh.nbp, nbands, psi, ld, part, coef, s);
+            if (rgp->pform == 2) rows_launch_project<T, 4, 2>(nl, *rgp, sh.nbp, nbands, psi, ld, part, coef, s);
+            else if (rgp->pform == 1) rows_launch_project<T, 4, 1>(nl, *rgp, sh.nbp, nbands, psi, ld, part, coef, s);
+            else rows_launch_project<T, 4, 0>(nl, *rgp, sh.nbp, nbands, psi, ld, part, coef, s);
         } else {
-            rows_launch_project<T, 8, false>(nl, *rgp, sh.nbp, nbands, psi, ld, part, coef, s);
+            rows_launch_project<T, 8, 0>(nl, *rgp, sh.nbp, nbands, psi, ld, part, coef, s);
         }
     }
     if (timed) QG_HIP(hipEventRecord(h->tev[1], s));
     if (!nl || rg->njs == 4) {
-        if (nl && rg->sp4)
-            rows_launch_hpsi<T, 4, true>(g, sh, rg, nl, nbands, psi, ld, hpsi, ld_out, -0.5, 1, 0, V, coef, s);
+        if (nl && rg->pform == 2)
+            rows_launch_hpsi<T, 4, 2>(g, sh, rg, nl, nbands, psi, ld, hpsi, ld_out, -0.5, 1, 0, V, coef, s);
+        else if (nl && rg->pform == 1)
+            rows_launch_hpsi<T, 4, 1>(g, sh, rg, nl, nbands, psi, ld, hpsi, ld_out, -0.5, 1, 0, V, coef, s);
         else
-            rows_launch_hpsi<T, 4, false>(g, sh, rg, nl, nbands, psi, ld, hpsi, ld_out, -0.5, 1, 0, V, coef, s);
+            rows_launch_hpsi<T, 4, 0>(g, sh, rg, nl, nbands, psi, ld, hpsi, ld_out, -0.5, 1, 0, V, coef, s);
     } else {
-        rows_launch_hpsi<T, 8, false>(g, sh, rg, nl, nbands, psi, ld, hpsi, ld_out, -0.5, 1, 0, V, coef, s);
+        rows_launch_hpsi<T, 8, 0>(g, sh, rg, nl, nbands, psi, ld, hpsi, ld_out, -0.5, 1, 0, V, coef, s);
     }
     if (timed) QG_HIP(hipEventRecord(h->tev[2], s));
     return true;
@@ -1026,22 +1125,23 @@ bool rows_nonlocal_project(qg_nonlocal* nl, int dtype, int nbands, const void* p
                            DevBuf<double>& part_buf, hipStream_t s) {
     const qg_grid& g = nl->gcopy;
     if (nl->geo.ncoef == 0 || !rows_eligible(g, dtype, nl)) return false;
-    const RowsShape sh = rows_shape(g, dtype);
+    const RowsShape sh = rows_shape(g, dtype, nbands);
     if (sh.typ == 0) return false;
-    const RowsGeo* rg = rows_geometry(nl, sh.typ, sh.tzp);
+    const RowsGeo* rg = rows_geometry(nl, sh.typ, sh.tzp, false);
     if (!rg->ok || rg->max_items > kProjThreads * kItemsPT) return false;
     const size_t lds = dtype == QG_F32 ? rows_proj_lds<float>(*rg, g, sh.nbp, (int)nl->geo.projs.size())
                                        : rows_proj_lds<double>(*rg, g, sh.nbp, (int)nl->geo.projs.size());
-    if (lds > 65536) return false;
+    if (lds > lds_limit()) return false;
     part_buf.alloc((size_t)std::max(1, rg->nslot) * rg->njs * nbands);
     auto go = [&](auto tag) {
         using T = decltype(tag);
         const T* p = (const T*)psi;
         if (rg->njs == 4) {
-            if (rg->sp4) rows_launch_project<T, 4, true>(nl, *rg, sh.nbp, nbands, p, ld, part_buf.p, coef, s);
-            else rows_launch_project<T, 4, false>(nl, *rg, sh.nbp, nbands, p, ld, part_buf.p, coef, s);
+            if (rg->pform == 2) rows_launch_project<T, 4, 2>(nl, *rg, sh.nbp, nbands, p, ld, part_buf.p, coef, s);
+            else if (rg->pform == 1) rows_launch_project<T, 4, 1>(nl, *rg, sh.nbp, nbands, p, ld, part_buf.p, coef, s);
+            else rows_launch_project<T, 4, 0>(nl, *rg, sh.nbp, nbands, p, ld, part_buf.p, coef, s);
         } else {
-            rows_launch_project<T, 8, false>(nl, *rg, sh.nbp, nbands, p, ld, part_buf.p, coef, s);
+            rows_launch_project<T, 8, 0>(nl, *rg, sh.nbp, nbands, p, ld, part_buf.p, coef, s);
         }
     };
     if (dtype == QG_F32) go(float{});
@@ -1054,16 +1154,18 @@ bool rows_nonlocal_project(qg_nonlocal* nl, int dtype, int nbands, const void* p
 bool rows_laplacian_apply(const qg_grid& g, int dtype, int nbands, const void* psi, int64_t ld, void* out,
                           int64_t ld_out, double coef, int accumulate, hipStream_t s) {
     if (!rows_eligible(g, dtype, nullptr)) return false;
-    const RowsShape sh = rows_shape(g, dtype);
+    const RowsShape sh = rows_shape(g, dtype, nbands);
     if (dtype == QG_F32) {
-        if (rows_hpsi_lds<float>(g, sh.ty, sh.tz, sh.nbw, 4, 1, 0, rows_nthr(g, dtype, sh.ty, sh.nbw)) > 65536)
+        if (rows_hpsi_lds<float>(g, sh.ty, sh.tz, sh.nbw, 4, 1, 0, rows_nthr(g, dtype, sh.ty, sh.nbw), 0, 0) >
+            lds_limit())
             return false;
-        rows_launch_hpsi<float, 4, false>(g, sh, nullptr, nullptr, nbands, (const float*)psi, ld, (float*)out, ld_out,
+        rows_launch_hpsi<float, 4, 0>(g, sh, nullptr, nullptr, nbands, (const float*)psi, ld, (float*)out, ld_out,
                                           coef, 0, accumulate, nullptr, nullptr, s);
     } else {
-        if (rows_hpsi_lds<double>(g, sh.ty, sh.tz, sh.nbw, 4, 1, 0, rows_nthr(g, dtype, sh.ty, sh.nbw)) > 65536)
+        if (rows_hpsi_lds<double>(g, sh.ty, sh.tz, sh.nbw, 4, 1, 0, rows_nthr(g, dtype, sh.ty, sh.nbw), 0, 0) >
+            lds_limit())
             return false;
-        rows_launch_hpsi<double, 4, false>(g, sh, nullptr, nullptr, nbands, (const double*)psi, ld, (double*)out,
+        rows_launch_hpsi<double, 4, 0>(g, sh, nullptr, nullptr, nbands, (const double*)psi, ld, (double*)out,
                                            ld_out, coef, 0, accumulate, nullptr, nullptr, s);
     }
     return true;
@@ -1075,10 +1177,10 @@ void rows_prepare(qg_nonlocal* nl) {
     if (nl->geo.ncoef == 0) return;
     for (int dtype : {QG_F32, QG_F64}) {
         if (!rows_eligible(nl->gcopy, dtype, nl)) continue;
-        const RowsShape sh = rows_shape(nl->gcopy, dtype);
+        const RowsShape sh = rows_shape(nl->gcopy, dtype, 0);
         if (sh.typ == 0) continue;
-        rows_geometry(nl, sh.ty, sh.tzp);
-        rows_geometry(nl, sh.typ, sh.tzp);
+        rows_geometry(nl, sh.ty, sh.tzp, true);
+        rows_geometry(nl, sh.typ, sh.tzp, false);
     }
 }
 
@@ -1125,9 +1227,9 @@ void rows_apply_noalloc(qg_ham* h, int dtype, int nbands, const void* psi, int64
     RowsScratch ws{scratch_address(), kScratchDoubles};
     int chunk = nbands;
     if (nl) {
-        const RowsShape sh = rows_shape(g, dtype);
+        const RowsShape sh = rows_shape(g, dtype, nbands);
         QG_REQUIRE(sh.typ > 0, "the projection tile does not fit this grid");
-        const RowsGeo* rgp = rows_geometry(nl, sh.typ, sh.tzp, false);
+        const RowsGeo* rgp = rows_geometry(nl, sh.typ, sh.tzp, false, false);
         QG_REQUIRE(rgp && rgp->ok, "the projector geometry of this launch shape is not prepared (qg_nonlocal_create "
                                    "and qg_nonlocal_update prepare the default shapes)");
         const size_t per_band = (size_t)std::max(1, rgp->nslot) * rgp->njs + (size_t)nl->geo.ncoef;

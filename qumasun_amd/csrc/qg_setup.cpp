@@ -396,12 +396,14 @@ void build_tile_nl(const qg_grid& g, const NlGeometry& geo, int ty, int tz, bool
 // Build the tile-column images, the (tile, plane) items and the projection's thread map for
 // tiles of ty rows and projection slabs of tzp planes.  Returns false when the geometry does
 // not fit the encodings (then the caller uses the table kernels).
-bool build_rows_geometry(const qg_grid& g, const NlGeometry& geo, int ty, int tzp, RowsGeo& o) {
+bool build_rows_geometry(const qg_grid& g, const NlGeometry& geo, int ty, int tzp, bool by_group, RowsGeo& o) {
     // o is freshly constructed by the caller
     const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
     if (nx > 1024 || ty > 32) return false;
     o.ty = ty;
     o.tzp = tzp;
+    o.by_group = by_group;
+    o.ng4 = (nx + 3) / 4;
     o.ntile = (ny + ty - 1) / ty;
     o.nslabp = (nz + tzp - 1) / tzp;
     // images of each tile column
@@ -417,101 +419,148 @@ bool build_rows_geometry(const qg_grid& g, const NlGeometry& geo, int ty, int tz
         o.col_off[t + 1] = (int)o.col_img.size();
         o.max_col = std::max(o.max_col, (int)col[t].size());
     }
-    // items per (tile, plane), grouped by column image, with per (plane, image) offsets
-    o.it_off.assign((size_t)o.ntile * nz + 1, 0);
-    o.pl_base.assign(o.ntile + 1, 0);
-    for (int t = 0; t < o.ntile; ++t) o.pl_base[t + 1] = o.pl_base[t] + nz * ((int)col[t].size() + 1);
-    o.pl_off.assign((size_t)o.pl_base[o.ntile], 0);
-    // per (tile, projection slab, image): number of items
-    std::vector<std::vector<int>> slab_cnt((size_t)o.ntile * o.nslabp);
-    for (int t = 0; t < o.ntile; ++t) {
-        const int y0 = t * ty, rows = std::min(ty, ny - y0);
-        const int nc = (int)col[t].size();
-        for (int s = 0; s < o.nslabp; ++s) slab_cnt[(size_t)t * o.nslabp + s].assign(nc, 0);
-        for (int z = 0; z < nz; ++z) {
-            const size_t tp = (size_t)t * nz + z;
-            int* po = &o.pl_off[o.pl_base[t] + (size_t)z * (nc + 1)];
-            int n = 0;
-            for (int li = 0; li < nc; ++li) {
-                po[li] = n;
-                const DevImage& im = geo.images[col[t][li]];
-                if (z < im.b[2] || z >= im.e[2]) continue;
-                const int wy = im.e[1] - im.b[1];
-                for (int y = std::max(y0, im.b[1]); y < std::min(y0 + rows, im.e[1]); ++y) {
-                    const int2 iv = geo.intervals[im.row_base + (y - im.b[1]) + wy * (z - im.b[2])];
-                    for (int xs = iv.x; xs < iv.y; xs += 4) {
-                        const int c = std::min(4, iv.y - xs);
-                        o.items.push_back((uint32_t)(y - y0) | ((uint32_t)xs << 5) | ((uint32_t)(c - 1) << 15) |
-                                          ((uint32_t)li << 17));
-                        ++n;
-                        ++slab_cnt[(size_t)t * o.nslabp + z / tzp][li];
+    if (by_group) {
+        // items cut at multiples of 4 in x, sorted by (row, group, image); per (tile, plane)
+        // the run offsets of every (row, group)
+        o.it_off.assign((size_t)o.ntile * nz + 1, 0);
+        const int ng4 = o.ng4;
+        o.grp_off.assign((size_t)o.ntile * nz * (ty * ng4 + 1), 0);
+        std::vector<std::pair<uint64_t, uint32_t>> pl;   // (sort key, item) of one (tile, plane)
+        for (int t = 0; t < o.ntile; ++t) {
+            const int y0 = t * ty, rows = std::min(ty, ny - y0);
+            const int nc = (int)col[t].size();
+            for (int z = 0; z < nz; ++z) {
+                pl.clear();
+                for (int li = 0; li < nc; ++li) {
+                    const DevImage& im = geo.images[col[t][li]];
+                    if (z < im.b[2] || z >= im.e[2]) continue;
+                    const int wy = im.e[1] - im.b[1];
+                    for (int y = std::max(y0, im.b[1]); y < std::min(y0 + rows, im.e[1]); ++y) {
+                        const int2 iv = geo.intervals[im.row_base + (y - im.b[1]) + wy * (z - im.b[2])];
+                        for (int xs = iv.x; xs < iv.y;) {
+                            const int xe = std::min(iv.y, (xs / 4 + 1) * 4);
+                            const uint32_t it = (uint32_t)(y - y0) | ((uint32_t)xs << 5) |
+                                                ((uint32_t)(xe - xs - 1) << 15) | ((uint32_t)li << 17);
+                            const uint64_t key = ((uint64_t)((y - y0) * ng4 + xs / 4) << 32) | (uint32_t)li;
+                            pl.push_back({key, it});
+                            xs = xe;
+                        }
                     }
                 }
-            }
-            po[nc] = n;
-            o.max_items = std::max(o.max_items, n);
-            o.it_off[tp + 1] = o.it_off[tp] + n;
-            if ((int64_t)o.it_off[tp] + n >= (1LL << 31) - 1) return false;
-        }
-    }
-    // projection thread map: every active image of a (tile, slab) owns a contiguous range
-    // of threads in proportion to its items; each range gets one partial slot
-    constexpr int T = kRowsProjThreads;
-    o.thr_map.assign((size_t)o.ntile * o.nslabp * T, 0xffffffffu);
-    o.thr_slot.assign((size_t)o.ntile * o.nslabp * T, 0);
-    std::vector<std::vector<int>> atom_slots(geo.natoms);
-    int nslot = 0;
-    for (int t = 0; t < o.ntile; ++t)
-        for (int s = 0; s < o.nslabp; ++s) {
-            const std::vector<int>& cnt = slab_cnt[(size_t)t * o.nslabp + s];
-            std::vector<int> act;
-            int64_t tot = 0;
-            for (int li = 0; li < (int)cnt.size(); ++li)
-                if (cnt[li] > 0) {
-                    act.push_back(li);
-                    tot += cnt[li];
+                std::sort(pl.begin(), pl.end());
+                const size_t tp = (size_t)t * nz + z;
+                int* go = &o.grp_off[tp * (ty * ng4 + 1)];
+                size_t k = 0;
+                for (int gi = 0; gi < ty * ng4; ++gi) {
+                    go[gi] = (int)k;
+                    while (k < pl.size() && (int)(pl[k].first >> 32) == gi) ++k;
                 }
-            if (act.empty()) continue;
-            if ((int)act.size() > T) return false;
-            std::vector<int> nthr(act.size());
-            int sum = 0;
-            for (size_t k = 0; k < act.size(); ++k) {
-                nthr[k] = std::max(1, (int)((int64_t)T * cnt[act[k]] / tot));
-                sum += nthr[k];
-            }
-            while (sum > T) {   // take from the image with the most threads
-                size_t m = 0;
-                for (size_t k = 1; k < act.size(); ++k)
-                    if (nthr[k] > nthr[m]) m = k;
-                --nthr[m];
-                --sum;
-            }
-            while (sum < T) {   // give to the image with the most items per thread
-                size_t m = 0;
-                for (size_t k = 1; k < act.size(); ++k)
-                    if ((int64_t)cnt[act[k]] * nthr[m] > (int64_t)cnt[act[m]] * nthr[k]) m = k;
-                ++nthr[m];
-                ++sum;
-            }
-            int tb = 0;
-            for (size_t k = 0; k < act.size(); ++k) {
-                const int n = std::min(nthr[k], 256);
-                const int slot = nslot++;
-                atom_slots[geo.images[col[t][act[k]]].atom].push_back(slot);
-                for (int u = 0; u < n; ++u) {
-                    const size_t at = ((size_t)t * o.nslabp + s) * T + tb + u;
-                    o.thr_map[at] = (uint32_t)act[k] | ((uint32_t)u << 10) | ((uint32_t)(n - 1) << 18);
-                    o.thr_slot[at] = slot;
-                }
-                tb += nthr[k];
+                go[ty * ng4] = (int)k;
+                for (auto& e : pl) o.items.push_back(e.second);
+                const int n = (int)pl.size();
+                o.max_items = std::max(o.max_items, n);
+                o.it_off[tp + 1] = o.it_off[tp] + n;
+                if ((int64_t)o.it_off[tp] + n >= (1LL << 31) - 1) return false;
             }
         }
-    o.nslot = nslot;
-    o.atom_slot_off.assign(geo.natoms + 1, 0);
-    for (int a = 0; a < geo.natoms; ++a) {
-        o.atom_slot.insert(o.atom_slot.end(), atom_slots[a].begin(), atom_slots[a].end());
-        o.atom_slot_off[a + 1] = (int)o.atom_slot.size();
-    }
+        o.atom_slot_off.assign(geo.natoms + 1, 0);
+    } else {
+        // items per (tile, plane), grouped by column image, with per (plane, image) offsets
+        o.it_off.assign((size_t)o.ntile * nz + 1, 0);
+        o.pl_base.assign(o.ntile + 1, 0);
+        for (int t = 0; t < o.ntile; ++t) o.pl_base[t + 1] = o.pl_base[t] + nz * ((int)col[t].size() + 1);
+        o.pl_off.assign((size_t)o.pl_base[o.ntile], 0);
+        // per (tile, projection slab, image): number of items
+        std::vector<std::vector<int>> slab_cnt((size_t)o.ntile * o.nslabp);
+        for (int t = 0; t < o.ntile; ++t) {
+            const int y0 = t * ty, rows = std::min(ty, ny - y0);
+            const int nc = (int)col[t].size();
+            for (int s = 0; s < o.nslabp; ++s) slab_cnt[(size_t)t * o.nslabp + s].assign(nc, 0);
+            for (int z = 0; z < nz; ++z) {
+                const size_t tp = (size_t)t * nz + z;
+                int* po = &o.pl_off[o.pl_base[t] + (size_t)z * (nc + 1)];
+                int n = 0;
+                for (int li = 0; li < nc; ++li) {
+                    po[li] = n;
+                    const DevImage& im = geo.images[col[t][li]];
+                    if (z < im.b[2] || z >= im.e[2]) continue;
+                    const int wy = im.e[1] - im.b[1];
+                    for (int y = std::max(y0, im.b[1]); y < std::min(y0 + rows, im.e[1]); ++y) {
+                        const int2 iv = geo.intervals[im.row_base + (y - im.b[1]) + wy * (z - im.b[2])];
+                        for (int xs = iv.x; xs < iv.y; xs += 4) {
+                            const int c = std::min(4, iv.y - xs);
+                            o.items.push_back((uint32_t)(y - y0) | ((uint32_t)xs << 5) | ((uint32_t)(c - 1) << 15) |
+                                              ((uint32_t)li << 17));
+                            ++n;
+                            ++slab_cnt[(size_t)t * o.nslabp + z / tzp][li];
+                        }
+                    }
+                }
+                po[nc] = n;
+                o.max_items = std::max(o.max_items, n);
+                o.it_off[tp + 1] = o.it_off[tp] + n;
+                if ((int64_t)o.it_off[tp] + n >= (1LL << 31) - 1) return false;
+            }
+        }
+        // projection thread map: every active image of a (tile, slab) owns a contiguous range
+        // of threads in proportion to its items; each range gets one partial slot
+        constexpr int T = kRowsProjThreads;
+        o.thr_map.assign((size_t)o.ntile * o.nslabp * T, 0xffffffffu);
+        o.thr_slot.assign((size_t)o.ntile * o.nslabp * T, 0);
+        std::vector<std::vector<int>> atom_slots(geo.natoms);
+        int nslot = 0;
+        for (int t = 0; t < o.ntile; ++t)
+            for (int s = 0; s < o.nslabp; ++s) {
+                const std::vector<int>& cnt = slab_cnt[(size_t)t * o.nslabp + s];
+                std::vector<int> act;
+                int64_t tot = 0;
+                for (int li = 0; li < (int)cnt.size(); ++li)
+                    if (cnt[li] > 0) {
+                        act.push_back(li);
+                        tot += cnt[li];
+                    }
+                if (act.empty()) continue;
+                if ((int)act.size() > T) return false;
+                std::vector<int> nthr(act.size());
+                int sum = 0;
+                for (size_t k = 0; k < act.size(); ++k) {
+                    nthr[k] = std::max(1, (int)((int64_t)T * cnt[act[k]] / tot));
+                    sum += nthr[k];
+                }
+                while (sum > T) {   // take from the image with the most threads
+                    size_t m = 0;
+                    for (size_t k = 1; k < act.size(); ++k)
+                        if (nthr[k] > nthr[m]) m = k;
+                    --nthr[m];
+                    --sum;
+                }
+                while (sum < T) {   // give to the image with the most items per thread
+                    size_t m = 0;
+                    for (size_t k = 1; k < act.size(); ++k)
+                        if ((int64_t)cnt[act[k]] * nthr[m] > (int64_t)cnt[act[m]] * nthr[k]) m = k;
+                    ++nthr[m];
+                    ++sum;
+                }
+                int tb = 0;
+                for (size_t k = 0; k < act.size(); ++k) {
+                    const int n = std::min(nthr[k], 256);
+                    const int slot = nslot++;
+                    atom_slots[geo.images[col[t][act[k]]].atom].push_back(slot);
+                    for (int u = 0; u < n; ++u) {
+                        const size_t at = ((size_t)t * o.nslabp + s) * T + tb + u;
+                        o.thr_map[at] = (uint32_t)act[k] | ((uint32_t)u << 10) | ((uint32_t)(n - 1) << 18);
+                        o.thr_slot[at] = slot;
+                    }
+                    tb += nthr[k];
+                }
+            }
+        o.nslot = nslot;
+        o.atom_slot_off.assign(geo.natoms + 1, 0);
+        for (int a = 0; a < geo.natoms; ++a) {
+            o.atom_slot.insert(o.atom_slot.end(), atom_slots[a].begin(), atom_slots[a].end());
+            o.atom_slot_off[a + 1] = (int)o.atom_slot.size();
+        }
+    }   // projection order
     // s + p layout everywhere: every image holds an s channel (slot 0) and a p channel
     // (slots 1..3), both Gaussian
     o.sp4 = geo.max_coef_per_atom == 4;
@@ -522,6 +571,13 @@ bool build_rows_geometry(const qg_grid& g, const NlGeometry& geo, int ty, int tz
         if (!(s.l == 0 && s.coef_rel == 0 && p.l == 1 && p.coef_rel == 1)) { o.sp4 = false; break; }
     }
     o.njs = geo.max_coef_per_atom <= 4 ? 4 : 8;
+    // s + p with one Gaussian exponent for both channels of every image
+    bool same = o.sp4;
+    for (const DevImage& im : geo.images) {
+        if (!same) break;
+        same = geo.projs[im.proj_begin].inv_2s2 == geo.projs[im.proj_begin + 1].inv_2s2;
+    }
+    o.pform = o.sp4 ? (same ? 2 : 1) : 0;
     return true;
 }
 

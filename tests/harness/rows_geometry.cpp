@@ -26,6 +26,7 @@ int main(int argc, char** argv) {
     const int ty = argc > 4 ? atoi(argv[4]) : 8;
     const int tzp = argc > 5 ? atoi(argv[5]) : 16;
     const int two_species = argc > 6 ? atoi(argv[6]) : 0;
+    const int by_group = argc > 7 ? atoi(argv[7]) : 0;   // apply-kernel order (items by 4-point group)
     const double A = 6.74;
     qg_grid g;
     g.n[0] = nx; g.n[1] = ny; g.n[2] = nz;
@@ -52,7 +53,7 @@ int main(int argc, char** argv) {
     NlGeometry geo;
     build_nl_geometry(g, 8, xyz, species, nspecies, nproj, sp, geo);
     RowsGeo rg;
-    if (!build_rows_geometry(g, geo, ty, tzp, rg)) {
+    if (!build_rows_geometry(g, geo, ty, tzp, by_group != 0, rg)) {
         printf("geometry rejected\n");
         return 1;
     }
@@ -69,16 +70,25 @@ int main(int argc, char** argv) {
     }
     int64_t items = 0, dup = 0, badgroup = 0;
     std::set<std::tuple<int, int, int, int>> got;
+    const int ngo = ty * rg.ng4;
     for (int t = 0; t < rg.ntile; ++t) {
         const int nc = rg.col_off[t + 1] - rg.col_off[t];
         for (int z = 0; z < nz; ++z) {
             const size_t tp = (size_t)t * nz + z;
-            const int* po = &rg.pl_off[rg.pl_base[t] + (size_t)z * (nc + 1)];
+            const int* po = by_group ? nullptr : &rg.pl_off[rg.pl_base[t] + (size_t)z * (nc + 1)];
+            const int* go = by_group ? &rg.grp_off[tp * (ngo + 1)] : nullptr;
+            if (go && (go[0] != 0 || go[ngo] != rg.it_off[tp + 1] - rg.it_off[tp])) ++badgroup;
             for (int k = rg.it_off[tp]; k < rg.it_off[tp + 1]; ++k) {
                 const uint32_t it = rg.items[k];
                 const int r = it & 31, xs = (it >> 5) & 1023, c = ((it >> 15) & 3) + 1, li = it >> 17;
                 const int rel = k - rg.it_off[tp];
-                if (li >= nc || rel < po[li] || rel >= po[li + 1]) ++badgroup;
+                if (by_group) {
+                    // inside one 4-point group, in that group's run
+                    const int gi = r * rg.ng4 + xs / 4;
+                    if (li >= nc || (xs + c - 1) / 4 != xs / 4 || rel < go[gi] || rel >= go[gi + 1]) ++badgroup;
+                } else if (li >= nc || rel < po[li] || rel >= po[li + 1]) {
+                    ++badgroup;
+                }
                 const int gi = rg.col_img[rg.col_off[t] + li];
                 for (int p = 0; p < c; ++p) {
                     ++items;
@@ -92,6 +102,18 @@ int main(int argc, char** argv) {
         if (!got.count(k)) ++missing;
     for (auto& k : got)
         if (!ref.count(k)) ++extra;
+    if (by_group) {
+        int64_t missing = 0, extra = 0;
+        for (auto& k : ref)
+            if (!got.count(k)) ++missing;
+        for (auto& k : got)
+            if (!ref.count(k)) ++extra;
+        printf("items %lld refpts %zu missing %lld extra %lld dup %lld badgroup %lld badmap 0 maxrel 0 "
+               "slots 0 maxcol %d sp4 %d njs %d\n",
+               (long long)items, ref.size(), (long long)missing, (long long)extra, (long long)dup, (long long)badgroup,
+               rg.max_col, (int)rg.sp4, rg.njs);
+        return 0;
+    }
     // thread map sanity and the replayed projection
     std::mt19937 mt(7);
     std::uniform_real_distribution<double> U(-1.0, 1.0);

@@ -37,10 +37,13 @@ CASES = [(40, 40, 40, 8, 16, 0), (40, 40, 40, 16, 40, 0), (24, 28, 20, 6, 7, 0),
          (40, 40, 40, 4, 3, 1), (16, 16, 16, 16, 16, 0)]
 
 
+@pytest.mark.parametrize("order", ["image", "group"])
 @pytest.mark.parametrize("case", CASES)
-def test_rows_items_and_projection_bookkeeping(harness, case):
-    r = subprocess.run([harness] + [str(v) for v in case], stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
-                       text=True, timeout=300)
+def test_rows_items_and_projection_bookkeeping(harness, case, order):
+    """Projection order (by image, with the thread map and partial slots) and the apply
+    kernel's order (items cut at 4-point groups, sorted by group, with the group runs)."""
+    r = subprocess.run([harness] + [str(v) for v in case] + [str(int(order == "group"))], stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True, timeout=300)
     assert r.returncode == 0, r.stdout
     m = re.search(r"items (\d+) refpts (\d+) missing (\d+) extra (\d+) dup (\d+) badgroup (\d+) badmap (\d+) "
                   r"maxrel (\S+) slots (\d+) maxcol (\d+) sp4 (\d) njs (\d)", r.stdout)
