@@ -1,28 +1,29 @@
 #!/usr/bin/env python3
-"""Benchmark of the H·psi hot path (and, optionally, time to eigenpairs) on one MI355X.
+"""Benchmark of the H·psi hot path and of the time to 512 eigenpairs on one MI355X.
 
-H·psi workload (the configuration the H·psi bandwidth metric is quoted on): the 64-atom
-2x2x2 diamond supercell, L = 13.48 bohr, periodic 80^3 grid, Vloc = sum -4 erf(r/0.8)/r +
-U(+-0.05), Gaussian s+p KB projectors (sigma 0.6, cutoff 2.4, E = +1.0 / -0.5), and a block of
-128 seeded random real bands in FP32.  A step is one application of H to the whole 128-band
-block with inputs resident in HBM: per band chunk, the projector-dot kernel, then (FP32 default)
-a stencil + V launch and a projector-term launch with in-kernel projector values, or one fused
-stencil + V + projector launch reading per-point tables.
+Default workload (the configuration the headline metric is quoted on): the 216-atom 3x3x3
+diamond supercell, L = 20.22 bohr, periodic 120^3 grid, Vloc = sum -4 erf(r/0.8)/r + U(+-0.05),
+Gaussian s+p KB projectors (sigma 0.6, cutoff 2.4, E = +1.0 / -0.5), a block of 512 seeded
+random real bands in FP32, inputs resident in HBM.  A step is one application of H to the
+whole block: the tile-centric projector-dot launch (and its per-atom reduction), then one
+stencil + V + projector-apply launch that writes H psi once.
 
-value     = algorithmic bytes of a step / step time, algorithmic bytes = 2*B*Ng*s + Ng*s (psi
-            read once, H psi written once, V read once; the projector tables are reported
-            separately as table_bytes).
-roofline  = the H psi launches of one band chunk: its algorithmic bytes / their average
-            duration (hipEvents around them, on the launch stream); traffic = HBM bytes per
-            chunk from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this
-            workload, scaled by the copy-kernel calibration factor recorded there.
-cpu_baseline = the FP64 oracle (a C++ restatement of the reference's CPU path) on 1 host
-            thread, one pass over a bounded sample of the same block.
---eigen cfg0|cfg2|none (default cfg0, 8 atoms / 40^3 / 16 bands, ~15 s): also time the
-            LOBPCG solve to eigenvalues within 1e-6 Ha of the FP64 reference: plain FP64 (the
-            reference's algorithm, and on the orthonormal basis) vs FP32 + learned
-            preconditioner.  cfg2 (216 atoms / 120^3 / 512 bands) takes ~15 minutes with
-            --eigen-budget 650.
+value     = algorithmic bytes of a step / step time; algorithmic bytes = 2*B*Ng*s + Ng*s (psi
+            read once, H psi written once, V read once; projector values are evaluated in the
+            kernels, so no table bytes).
+roofline  = the same bytes over the summed duration of all H psi launches of a step (projection
+            included), measured with hipEvents inside the library on the launch stream;
+            traffic = HBM bytes per step from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+            summary of this workload, scaled by its copy-kernel calibration.
+secondary = the same measurement on the 64-atom / 80^3 / 128-band cell.
+cpu_baseline = the FP64 oracle (a C++ restatement of the reference's CPU loops) on all host
+            cores (bands dealt over threads; one pass over a bounded sample of the same block),
+            with the single-thread figure beside it.
+eigen     = seconds until the 512 eigenvalues lie within 1e-6 Ha of the FP64 fixture
+            (tests/golden/cfg2_eigenvalues.json): FP32 LOBPCG with the learned preconditioner
+            (training time reported and counted) followed by FP64 sweeps, against plain FP64
+            LOBPCG (no preconditioner, orthonormal basis) stopped at a budget of a few times the
+            learned run.  --eigen cfg0 runs the 8-atom / 40^3 / 16-band comparison instead.
 """
 import argparse
 import json
@@ -31,6 +32,7 @@ import platform
 import subprocess
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -38,23 +40,31 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "fused_hpsi_80cube_128bands_f32_pmc.json")
+WORKLOADS = {
+    "cfg2": dict(ncell=3, bands=512, desc="H psi 216-atom diamond 3x3x3, L=20.22 bohr, 120^3 grid, 512 bands"),
+    "cfg1": dict(ncell=2, bands=128, desc="H psi 64-atom diamond 2x2x2, L=13.48 bohr, 80^3 grid, 128 bands"),
+}
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "hpsi_216atoms_120cube_512bands_f32_pmc.json")
+FIXTURE_CFG2 = os.path.join(ROOT, "tests", "golden", "cfg2_eigenvalues.json")
+FIXTURE_CFG0 = os.path.join(ROOT, "tests", "golden", "cfg0_eigenvalues.json")
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="cfg2")
     ap.add_argument("--dtype", choices=["f32", "f64"], default="f32")
-    ap.add_argument("--bands", type=int, default=128)
+    ap.add_argument("--no-secondary", action="store_true", help="skip the 80^3 / 128-band secondary H psi line")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-bands", type=int, default=128,
-                    help="bands of the CPU baseline sample (FP64 oracle, 1 thread)")
-    ap.add_argument("--eigen", choices=["none", "cfg0", "cfg2"], default="cfg0",
-                    help="also time LOBPCG to 1e-6 Ha: plain FP64 vs FP32 + learned preconditioner")
-    ap.add_argument("--eigen-budget", type=float, default=600.0,
-                    help="seconds after which the plain FP64 eigen runs stop (reported as unconverged)")
+    ap.add_argument("--eigen", choices=["none", "cfg0", "cfg2"], default="cfg2",
+                    help="also time LOBPCG to 1e-6 Ha: FP32 + learned preconditioner vs plain FP64")
+    ap.add_argument("--eigen-budget", type=float, default=0.0,
+                    help="seconds after which the plain FP64 run stops (0: 3.2 x the learned run, training "
+                         "included, at least 60 s and at most 300 s)")
+    ap.add_argument("--eigen-compare", action="store_true",
+                    help="also time FP32 + kinetic and FP64 + kinetic (the same preconditioner in both precisions)")
     ap.add_argument("--lobpcg-iters", type=int, default=0,
                     help="only run this many LOBPCG iterations per precision (FP64, FP32, mixed) on the --eigen "
                          "workload and print the per-sector times: the command the solver's GEMM profile is taken on")
@@ -62,6 +72,11 @@ def parse():
                     help="only run 5 device copies of 256 MiB (FP32), the known-size kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters to bytes")
     return ap.parse_args()
+
+
+def log(msg):
+    # progress on stderr (stdout carries the one JSON line)
+    print(msg, file=sys.stderr, flush=True)
 
 
 def copy_calibration():
@@ -86,63 +101,155 @@ def cpu_model():
     return platform.processor()
 
 
-def cpu_baseline(sysd, X, nbands):
-    """The FP64 oracle on 1 host thread, one pass over nbands bands of the 80^3 block."""
-    import oracle as O
-    O.olib().or_set_threads(1)
-    n = (sysd["n"],) * 3
-    h = (sysd["h"],) * 3
-    nat = len(sysd["positions"])
-    onl = O.OracleNL(n, h, sysd["positions"], np.zeros(nat, np.int32), sysd["projectors"])
-    sample = np.ascontiguousarray(X[:nbands])
-    t0 = time.perf_counter()
-    O.hamiltonian_block(n, h, sysd["V"], onl, sample)
-    dt = time.perf_counter() - t0
-    ng = n[0] * n[1] * n[2]
-    bytes_ = 2 * nbands * ng * 8 + ng * 8
-    return dict(value=round(bytes_ / dt / 1e9, 4), unit="GB/s", cores=1, kind="port",
-                sample=f"FP64 oracle H·psi (C++ restatement of the reference's loops), {nbands} bands of "
-                       f"the 80^3/64-atom block, one pass, {dt:.2f} s on 1 thread of {cpu_model()}; "
-                       f"bytes counted as 2*B*Ng*8+Ng*8")
-
-
-def traffic_per_launch(dtype, chunk_bands):
-    """HBM bytes per fused launch from the committed rocprofv3 counter summary, if it matches."""
+def host_threads():
+    """Host cores this process may use (the box's CPU share for one GPU is 16)."""
     try:
-        d = json.load(open(PMC_SUMMARY))
+        n = len(os.sched_getaffinity(0))
     except Exception:
-        return None
-    if d.get("dtype") != dtype or d.get("chunk_bands") != chunk_bands:
-        return None
-    return int(d["hbm_bytes_per_launch"])
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
 
 
-def eigen_leg(cfg, budget):
-    """Seconds to eigenvalues within 1e-6 Ha of the FP64 reference: plain FP64 LOBPCG (no
-    preconditioner; the reference's [X R P] iteration, and the same on the orthonormal basis)
-    vs FP32 with the learned preconditioner (and with the kinetic one). The FP32 runs are single
-    solver calls with a residual tolerance; a run that has not reached 1e-6 Ha continues with
-    FP64 sweeps at the next tolerance of the ladder. Plain runs stop at `budget` seconds."""
-    import torch
+def build_system(ncell):
     import qumasun_amd as Q
-    from qumasun_amd import _native as N
-    ncell, nb = (1, 16) if cfg == "cfg0" else (3, 512)
     s = Q.diamond_system(ncell, 40)
     n, h = s["n"], s["h"]
     g = Q.Grid(n, n, n, h, h, h)
     integ = Q.PseudoPotIntegrator(g, s["projectors"])
     integ.UpdatePosition(s["positions"])
-    H = Q.Hamiltonian(g, s["V"], integ)
+    return s, g, integ, Q.Hamiltonian(g, s["V"], integ)
+
+
+def traffic_per_step(dtype, workload):
+    """HBM bytes per step from the committed rocprofv3 counter summary, if it matches."""
+    try:
+        d = json.load(open(PMC_SUMMARY))
+    except Exception:
+        return None
+    if d.get("dtype") != dtype or d.get("workload_key") != workload:
+        return None
+    return int(d["hbm_bytes_per_step"])
+
+
+def hpsi_leg(key, dtype, steps, warmup):
+    """Time H psi on workload key: (result dict, system, psi tensor)."""
+    import torch
+    w = WORKLOADS[key]
+    s, g, integ, H = build_system(w["ncell"])
+    nb = w["bands"]
+    ng = g.size
+    tdt = torch.float32 if dtype == "f32" else torch.float64
+    esz = 4 if dtype == "f32" else 8
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(20241203)
+    psi = (torch.rand((nb, ng), generator=gen, device="cuda", dtype=torch.float32) - 0.5).to(tdt).contiguous()
+    out = torch.empty_like(psi)
+    for _ in range(warmup):
+        H.apply(psi, out)
+    torch.cuda.synchronize()
+    # timed region: K steps, HIP events on torch's current stream (the launch stream)
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        H.apply(psi, out)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = e0.elapsed_time(e1) / steps
+    # per-launch durations of the same kernels: hipEvents around the projection and the apply
+    # launch inside the library, on the launch stream
+    H.set_timing(True)
+    tp, ta = [], []
+    for _ in range(steps):
+        H.apply(psi, out)
+        a, b = H.last_times()
+        tp.append(a)
+        ta.append(b)
+    H.set_timing(False)
+    t_proj, t_apply = float(np.mean(tp)), float(np.mean(ta))
+    bytes_step = 2 * nb * ng * esz + ng * esz
+    achieved = bytes_step / ((t_proj + t_apply) * 1e-3) / 1e9
+    res = {
+        "value": round(bytes_step / (ms_step * 1e-3) / 1e9, 2),
+        "ms_per_step": round(ms_step, 5),
+        "wall_s_timed": round(wall, 4),
+        "workload": w["desc"],
+        "grid": [s["n"]] * 3, "atoms": len(s["positions"]), "bands": nb,
+        "roofline": {"bound": "hbm", "kernel": "rows_project_kernel + rows_coef_reduce_kernel (projection), "
+                                               "rows_hpsi_kernel (stencil + V + projector apply)",
+                     "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "traffic": traffic_per_step(dtype, key),
+                     "algorithmic_bytes_per_step": bytes_step,
+                     "launch_ms": round(t_proj + t_apply, 5), "project_ms": round(t_proj, 5),
+                     "apply_ms": round(t_apply, 5)},
+    }
+    del out
+    return res, (s, g, integ, H), psi
+
+
+def cpu_baseline(s, psi, nb_one, nb_all):
+    """The FP64 oracle on one host thread (nb_one bands) and on all host cores (nb_all bands
+    dealt over threads), one pass over a bounded sample of the same block."""
+    import oracle as O
+    lib = O.olib()
+    lib.or_set_threads(1)
+    n = (s["n"],) * 3
+    h = (s["h"],) * 3
+    nat = len(s["positions"])
+    onl = O.OracleNL(n, h, s["positions"], np.zeros(nat, np.int32), s["projectors"])
+    ng = n[0] * n[1] * n[2]
+    X = psi[:max(nb_one, nb_all)].double().cpu().numpy()
+
+    def run(lo, hi):
+        O.hamiltonian_block(n, h, s["V"], onl, X[lo:hi])
+
+    t0 = time.perf_counter()
+    run(0, nb_one)
+    dt1 = time.perf_counter() - t0
+    th = host_threads()
+    edges = np.linspace(0, nb_all, th + 1).astype(int)
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=th) as ex:
+        list(ex.map(lambda k: run(edges[k], edges[k + 1]), range(th)))
+    dtn = time.perf_counter() - t0
+    by = lambda nb: 2 * nb * ng * 8 + ng * 8
+    model = cpu_model()
+    return {"value": round(by(nb_all) / dtn / 1e9, 4), "unit": "GB/s", "cores": th, "kind": "port",
+            "sample": f"FP64 oracle H psi (C++ restatement of the reference's loops), {nb_all} bands of the "
+                      f"{s['n']}^3 / {nat}-atom block dealt over {th} threads, one pass, {dtn:.2f} s on {model}; "
+                      f"bytes counted as 2*B*Ng*8+Ng*8",
+            "one_core": {"value": round(by(nb_one) / dt1 / 1e9, 4), "unit": "GB/s", "cores": 1,
+                         "sample": f"{nb_one} bands on 1 thread, {dt1:.2f} s"}}
+
+
+def eigen_leg(cfg, budget, compare):
+    """Seconds to eigenvalues within 1e-6 Ha of the FP64 reference."""
+    import torch
+    import qumasun_amd as Q
+    from qumasun_amd import _native as N
+    ncell, nb = (1, 16) if cfg == "cfg0" else (3, 512)
+    s, g, integ, H = build_system(ncell)
+    fixture = FIXTURE_CFG0 if cfg == "cfg0" else FIXTURE_CFG2
     X0 = Q.random_states(g.size, nb, g.dvol)
 
     def progress(label, it, el, st):
-        # progress on stderr (stdout carries the one JSON line)
-        print(f"eigen {cfg} {label}: {it} iterations, {el:.1f} s, max residual {st['max_residual']:.2e}",
-              file=sys.stderr, flush=True)
+        log(f"eigen {cfg} {label}: {it} iterations, {el:.1f} s, max residual {st['max_residual']:.2e}")
 
-    def solve(label, chunk, ref=None, budget=600.0, **kw):
+    def fresh():
         X = torch.from_numpy(X0.copy()).cuda()
-        P = torch.zeros_like(X)
+        return X, torch.zeros_like(X)
+
+    ref = np.array(json.load(open(fixture))["eigenvalues"])
+
+    def until(label, ref, budget, chunk, **kw):
+        """Repeated calls (a rollback or chunk end continues with the kept P) until the eigenvalues
+        are within 1e-6 Ha of ref or the budget is spent."""
+        X, P = fresh()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         it, step = 0, 0
@@ -152,21 +259,18 @@ def eigen_leg(cfg, budget):
             it += st["iterations"]
             el = time.perf_counter() - t0
             progress(label, it, el, st)
-            if ref is None and st["max_residual"] <= kw.get("tol", 0.0):
-                return e, el, it, True
-            if ref is not None and float(np.abs(e - ref).max()) <= 1e-6:
-                return e, el, it, True
+            if float(np.abs(e - ref).max()) <= 1e-6:
+                return el, it, True
             if el > budget or st["iterations"] == 0:
-                return e, el, it, False
+                return el, it, False
 
     def ladder(label, ref, tols, **kw):
-        X = torch.from_numpy(X0.copy()).cuda()
-        P = torch.zeros_like(X)
+        X, P = fresh()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         it = 0
+        st = {}
         for step, tol in enumerate(tols):
-            # at most 100 iterations per call, so progress lines keep coming
             e, st = Q.Eigen_d_multi(H, X, P, step, max_iter=100, tol=tol, **kw)
             while st["max_residual"] > tol and st["iterations"] == 100:
                 it += st["iterations"]
@@ -176,52 +280,49 @@ def eigen_leg(cfg, budget):
             el = time.perf_counter() - t0
             progress(label, it, el, st)
             if float(np.abs(e - ref).max()) <= 1e-6:
-                return el, it, True
+                return el, it, True, st
             kw["precision"] = N.QG_PREC_F64   # later sweeps refine in FP64
-        return el, it, False
+        return el, it, False, st
 
-    ref, t_ref, it_ref, _ = solve("reference", 400 if cfg == "cfg0" else 50, precision=N.QG_PREC_F64,
-                                  precond=N.QG_PRECOND_KINETIC, precond_shift=0.5,
-                                  tol=1e-9 if cfg == "cfg0" else 1e-6, budget=1e9)
-    chunk = 10 if cfg == "cfg0" else 50
-    _, t_plain, it_plain, ok_plain = solve("plain", chunk, ref=ref, budget=min(budget, 60.0),
-                                           precision=N.QG_PREC_F64)
-    _, t_plo, it_plo, ok_plo = solve("plain-ortho", chunk, ref=ref, budget=budget, precision=N.QG_PREC_F64,
-                                     basis=N.QG_BASIS_ORTHO)
+    tols = (1e-5, 1e-6) if cfg == "cfg0" else (2e-3, 1.5e-3, 1e-3, 7e-4, 5e-4, 3e-4, 1e-4)
     t1 = time.perf_counter()
     sigma = float(ref[0]) - 0.7
     net, info = Q.fit_learned_preconditioner(H, min(nb, 32), sigma, 2.0, snapshots=(2, 5, 10), steps=300)
     t_train = time.perf_counter() - t1
+    log(f"eigen {cfg}: learned preconditioner trained in {t_train:.1f} s: {info}")
     net.attach()
-    tols = (1e-5, 1e-6) if cfg == "cfg0" else (2e-3, 1.5e-3, 1e-3, 7e-4, 5e-4, 3e-4, 1e-4)
-    t_l, it_l, ok_l = ladder("fp32-learned", ref, tols, precision=N.QG_PREC_F32, precond=N.QG_PRECOND_LEARNED,
-                             precond_shift=2.0, switch_tol=1e-3)
-    t_k, it_k, ok_k = ladder("fp32-kinetic", ref, tols, precision=N.QG_PREC_F32, precond=N.QG_PRECOND_KINETIC,
-                             precond_shift=0.5, switch_tol=1e-3)
+    t_l, it_l, ok_l, st_l = ladder("fp32-learned", ref, tols, precision=N.QG_PREC_F32, precond=N.QG_PRECOND_LEARNED,
+                                   precond_shift=2.0, switch_tol=1e-3)
     net.detach()
+    if budget <= 0:
+        budget = min(300.0, max(60.0, 3.2 * (t_l + t_train)))
+    chunk = 10 if cfg == "cfg0" else 50
+    t_p, it_p, ok_p = until("plain-fp64", ref, budget, chunk, precision=N.QG_PREC_F64, basis=N.QG_BASIS_ORTHO)
+    res = {"workload": f"{len(s['positions'])} atoms, {s['n']}^3 grid, {nb} bands",
+           "criterion": "max |e - e_FP64| <= 1e-6 Ha (FP64 fixture " + os.path.relpath(fixture, ROOT) + ")",
+           "fp32_learned": {"seconds": round(t_l, 3), "iterations": it_l, "reached": ok_l,
+                            "train_seconds": round(t_train, 3), "fallback_iter": st_l.get("fallback_iter"),
+                            "train": info},
+           "plain_fp64": {"seconds": round(t_p, 3), "iterations": it_p, "reached": ok_p, "budget_s": round(budget, 1),
+                          "basis": "orthonormal (SVQB), no preconditioner"}}
 
-    def ratio(t_base, ok_base, base_budget, t, ok):
-        if not ok:
+    def ratio(t_base, ok_base, t):
+        if not ok_l:
             return None
         if ok_base:
             return {"value": round(t_base / t, 3), "bound": "exact"}
-        if t_base >= base_budget:
-            return {"value": round(t_base / t, 3), "bound": "lower (plain run stopped unconverged at its budget)"}
-        return None   # the plain run stalled before its budget: no time to compare with
+        return {"value": round(t_base / t, 3), "bound": "lower (plain run stopped unconverged at its budget)"}
 
-    return {"workload": f"{len(s['positions'])} atoms, {n}^3 grid, {nb} bands",
-            "criterion": "max |e - e_FP64| <= 1e-6 Ha",
-            "plain_fp64": {"seconds": round(t_plain, 3), "iterations": it_plain, "reached": ok_plain,
-                           "basis": "reference ([X R P] generalized, rollback exit)"},
-            "plain_fp64_ortho": {"seconds": round(t_plo, 3), "iterations": it_plo, "reached": ok_plo,
-                                 "basis": "orthonormal (SVQB)", "budget_s": budget},
-            "fp32_learned": {"seconds": round(t_l, 3), "iterations": it_l, "reached": ok_l,
-                             "train_seconds": round(t_train, 3), "train": info},
-            "fp32_kinetic": {"seconds": round(t_k, 3), "iterations": it_k, "reached": ok_k},
-            "speedup_learned_vs_plain_ortho": ratio(t_plo, ok_plo, budget, t_l, ok_l),
-            "speedup_learned_with_training_vs_plain_ortho": ratio(t_plo, ok_plo, budget, t_l + t_train, ok_l),
-            "speedup_learned_vs_plain": ratio(t_plain, ok_plain, min(budget, 60.0), t_l, ok_l),
-            "reference_fp64_kinetic": {"seconds": round(t_ref, 3), "iterations": it_ref}}
+    res["speedup_learned_vs_plain_fp64"] = ratio(t_p, ok_p, t_l)
+    res["speedup_learned_with_training_vs_plain_fp64"] = ratio(t_p, ok_p, t_l + t_train)
+    if compare:
+        t_k, it_k, ok_k, _ = ladder("fp32-kinetic", ref, tols, precision=N.QG_PREC_F32,
+                                    precond=N.QG_PRECOND_KINETIC, precond_shift=0.5, switch_tol=1e-3)
+        t_d, it_d, ok_d = until("fp64-kinetic", ref, 900.0, chunk, precision=N.QG_PREC_F64,
+                                precond=N.QG_PRECOND_KINETIC, precond_shift=0.5)
+        res["fp32_kinetic"] = {"seconds": round(t_k, 3), "iterations": it_k, "reached": ok_k}
+        res["fp64_kinetic"] = {"seconds": round(t_d, 3), "iterations": it_d, "reached": ok_d}
+    return res
 
 
 def lobpcg_iterations(cfg, iters):
@@ -232,16 +333,11 @@ def lobpcg_iterations(cfg, iters):
     import qumasun_amd as Q
     from qumasun_amd import _native as N
     ncell, nb = (1, 16) if cfg == "cfg0" else (3, 512)
-    s = Q.diamond_system(ncell, 40)
-    n, h = s["n"], s["h"]
-    g = Q.Grid(n, n, n, h, h, h)
-    integ = Q.PseudoPotIntegrator(g, s["projectors"])
-    integ.UpdatePosition(s["positions"])
-    H = Q.Hamiltonian(g, s["V"], integ)
+    s, g, integ, H = build_system(ncell)
     X0 = Q.random_states(g.size, nb, g.dvol)
     K = g.size
     gemm = 2.0 * nb * nb * K   # flops of one N x N x Ng product
-    out = {"workload": f"{len(s['positions'])} atoms, {n}^3 grid, {nb} bands", "iterations": iters,
+    out = {"workload": f"{len(s['positions'])} atoms, {s['n']}^3 grid, {nb} bands", "iterations": iters,
            "flops_per_NxNxNg_product": gemm}
     for name, prec in (("f64", N.QG_PREC_F64), ("f32", N.QG_PREC_F32), ("mixed", N.QG_PREC_MIXED)):
         X = torch.from_numpy(X0.copy()).cuda()
@@ -261,7 +357,7 @@ def lobpcg_iterations(cfg, iters):
         out[name] = {"seconds": round(el, 3), "iterations": st["iterations"],
                      "per_iteration_ms": {k[2:]: round(st[k] / it * 1e3, 2)
                                           for k in ("t_gram", "t_update", "t_eig", "t_hpsi", "t_precond")}}
-        print(f"lobpcg {name}: {st['iterations']} iterations in {el:.2f} s", file=sys.stderr, flush=True)
+        log(f"lobpcg {name}: {st['iterations']} iterations in {el:.2f} s")
     print(json.dumps(out))
 
 
@@ -278,96 +374,45 @@ def main():
         lobpcg_iterations(args.eigen if args.eigen != "none" else "cfg2", args.lobpcg_iters)
         return
     import torch
-    import qumasun_amd as Q
-
-    dev = torch.device("cuda:0")
-    sysd = Q.diamond_system(2, 40)
-    n = sysd["n"]
-    ng = n ** 3
-    h = sysd["h"]
-    g = Q.Grid(n, n, n, h, h, h)
-    integ = Q.PseudoPotIntegrator(g, sysd["projectors"])
-    integ.UpdatePosition(sysd["positions"])
-    H = Q.Hamiltonian(g, sysd["V"], integ)
-    X = Q.random_states(ng, args.bands, h ** 3)
-    dt = torch.float32 if args.dtype == "f32" else torch.float64
-    s = 4 if args.dtype == "f32" else 8
-    psi = torch.from_numpy(X).to(dev).to(dt).contiguous()
-    out = torch.empty_like(psi)
-
-    for _ in range(args.warmup):
-        H.apply(psi, out)
-    torch.cuda.synchronize()
-
-    # timed region: K steps, HIP events on torch's current stream (the launch stream)
-    stream = torch.cuda.current_stream()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(args.steps):
-        H.apply(psi, out)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_step = e0.elapsed_time(e1) / args.steps
-
-    # per-launch durations of the same kernels (hipEvents around each launch inside the library)
-    H.set_timing(True)
-    tp, tf = [], []
-    for _ in range(args.steps):
-        H.apply(psi, out)
-        a, b = H.last_times()
-        tp.append(a)
-        tf.append(b)
-    nchunks = H.last_chunks()
-    H.set_timing(False)
-    t_proj = float(np.mean(tp))
-    t_fused = float(np.mean(tf))
-    chunk_bands = (args.bands + nchunks - 1) // nchunks
-
-    bytes_step = 2 * args.bands * ng * s + ng * s
-    bytes_launch = 2 * chunk_bands * ng * s + ng * s
-    value = bytes_step / (ms_step * 1e-3) / 1e9
-    fused_launch_ms = t_fused / nchunks
-    achieved = bytes_launch / (fused_launch_ms * 1e-3) / 1e9
-    table_bytes = int(round(1.51 * ng)) * 4 * s   # beta tables, 4 slots per sphere point
-    split = (args.dtype == "f32" and integ.projector_form == "kernel" and os.environ.get("QG_SPLIT_NL", "") != "0") \
-        or os.environ.get("QG_SPLIT_NL", "") == "1"
-    kernel_desc = ("fused_hpsi_kernel: stencil+V pass + projector pass (in-kernel projector values) per band chunk"
-                   if split else "fused_hpsi_kernel: one fused stencil+V+projector pass per band chunk")
-
+    log(f"H psi {args.workload} {args.dtype}: {args.warmup} warmup + {args.steps} timed steps")
+    r, system, psi = hpsi_leg(args.workload, args.dtype, args.steps, args.warmup)
     res = {
-        "metric": "Hpsi HBM GB/s (algorithmic bytes / step time)",
-        "value": round(value, 2),
+        "metric": "Hpsi HBM GB/s (algorithmic bytes / step time); s to 512 eigenpairs @1e-6 Ha, FP32+learned vs "
+                  "plain FP64 in 'eigen'",
+        "value": r["value"],
         "unit": "GB/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 5),
+        "ms_per_step": r["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": args.dtype,
         "data": "synthetic (seeded diamond cell, seeded random bands)",
-        "config": {"workload": "Hpsi 64-atom diamond 2x2x2, L=13.48 bohr, 80^3 grid, 128 bands",
-                   "grid": [n, n, n], "atoms": len(sysd["positions"]), "bands": args.bands,
-                   "projectors": "gaussian s+p, sigma 0.6, rc 2.4", "band_chunks": nchunks},
-        "roofline": {"bound": "hbm", "kernel": kernel_desc,
-                     "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4),
-                     "traffic": traffic_per_launch(args.dtype, chunk_bands),
-                     "algorithmic_bytes_per_launch": bytes_launch, "launch_ms": round(fused_launch_ms, 5),
-                     "project_ms_per_step": round(t_proj, 5), "fused_ms_per_step": round(t_fused, 5),
-                     "table_bytes": table_bytes},
-        "wall_s_timed": round(wall, 4),
+        "config": {"workload": r["workload"], "grid": r["grid"], "atoms": r["atoms"], "bands": r["bands"],
+                   "projectors": "gaussian s+p, sigma 0.6, rc 2.4"},
+        "roofline": r["roofline"],
+        "wall_s_timed": r["wall_s_timed"],
     }
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline(sysd, X, min(args.cpu_bands, args.bands))
+        log("CPU baseline (FP64 oracle)")
+        nb = r["bands"]
+        res["cpu_baseline"] = cpu_baseline(system[0], psi, min(nb, 32 if args.workload == "cfg2" else 64),
+                                           min(nb, 256 if args.workload == "cfg2" else 128))
+    del psi, system
+    torch.cuda.empty_cache()
+    if not args.no_secondary and args.workload != "cfg1":
+        log("H psi cfg1 (secondary)")
+        r1, sys1, psi1 = hpsi_leg("cfg1", args.dtype, args.steps, args.warmup)
+        res["secondary"] = {"workload": r1["workload"], "value": r1["value"], "unit": "GB/s",
+                            "ms_per_step": r1["ms_per_step"], "roofline_frac": r1["roofline"]["frac"],
+                            "project_ms": r1["roofline"]["project_ms"], "apply_ms": r1["roofline"]["apply_ms"]}
+        del psi1, sys1
+        torch.cuda.empty_cache()
     if args.eigen != "none":
         try:
-            res["eigen"] = eigen_leg(args.eigen, args.eigen_budget)
+            res["eigen"] = eigen_leg(args.eigen, args.eigen_budget, args.eigen_compare)
         except Exception as exc:   # the H psi line stands on its own
             res["eigen"] = {"error": f"{type(exc).__name__}: {exc}"}
     print(json.dumps(res))

@@ -171,6 +171,12 @@ int qg_hamiltonian_last_chunks(qg_ham* h, int* nchunks);
 #define QG_BASIS_REFERENCE 0
 #define QG_BASIS_ORTHO 1
 
+/* Orthonormalisation of the W and P blocks in the orthonormal basis: QG_ORTH_CHOLQR (default)
+ * Cholesky-QR of the diagonally scaled Gram matrix, with SVQB where the Cholesky factor fails or
+ * its pivots show near-dependent columns; QG_ORTH_SVQB SVQB always (an eigensolve per block). */
+#define QG_ORTH_CHOLQR 0
+#define QG_ORTH_SVQB 1
+
 typedef struct qg_lobpcg_opts {
     int precision;           /* QG_PREC_* */
     int max_iter;            /* iter_max of Eigen_d_multi_mpi */
@@ -203,6 +209,7 @@ typedef struct qg_lobpcg_opts {
      * products (GEMM1).  NULL: no calls. */
     void (*watch)(void* user, int sector, int64_t count);
     void* watch_user;
+    int orth;                /* QG_ORTH_* (orthonormal basis only) */
 } qg_lobpcg_opts;
 
 typedef struct qg_lobpcg_stats {
@@ -244,6 +251,15 @@ int qg_gram_schmidt(int dtype, int64_t ng, int nbands, void* X, int64_t ld, doub
  * residual, lobpcg_d_multi_mpi.h:982-990); it is the classic preconditioner stage. */
 int qg_precond_apply(qg_ham* h, int kind, double shift, int dtype, int nbands, void* r, int64_t ld,
                      void* stream);
+
+/* Gram product C = A^T B of two tall band blocks in their own precision (dtype QG_F64 or
+ * QG_F32): A (K x M, column m at A + m*lda) and B (K x N) device blocks, C FP64 device (column
+ * stride ldc).  Split-K library GEMMs in the block precision with the chunks summed in FP64:
+ * the arithmetic of the solver's Rayleigh-Ritz Gram blocks, which replace blas_DGEMM_t
+ * (w_dgemm.h:32-37) inside MakeMatrix_d_blas (lobpcg_d_multi_mpi.h:192-250).  Allocates its
+ * workspace and waits for the stream. */
+int qg_gram(int dtype, const void* A, const void* B, int64_t K, int M, int N, int64_t lda, int64_t ldb,
+            double* C, int ldc, void* stream);
 
 /* Gram product C = A^T B on the matrix cores: A (K x M) and B (K x N) FP32 device blocks
  * (column m at A + m*lda), rounded to BF16 (split != 0: x = hi + lo in two BF16 terms,

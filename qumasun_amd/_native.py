@@ -22,6 +22,8 @@ QG_PRECOND_KINETIC = 1
 QG_PRECOND_LEARNED = 2
 QG_BASIS_REFERENCE = 0
 QG_BASIS_ORTHO = 1
+QG_ORTH_CHOLQR = 0
+QG_ORTH_SVQB = 1
 
 c_int, c_double, c_void_p, c_int64, c_uint32, c_uint64 = (ctypes.c_int, ctypes.c_double, ctypes.c_void_p,
                                                            ctypes.c_int64, ctypes.c_uint32, ctypes.c_uint64)
@@ -48,7 +50,8 @@ class LobpcgOpts(ctypes.Structure):
                 ("refresh_period", c_int), ("rollback_limit", c_double), ("precond", c_int),
                 ("precond_shift", c_double), ("verbose", c_int), ("switch_tol", c_double),
                 ("stall_window", c_int), ("op", OP_CALLBACK), ("op_user", c_void_p), ("basis", c_int),
-                ("fallback_shift", c_double), ("watch", WATCH_CALLBACK), ("watch_user", c_void_p)]
+                ("fallback_shift", c_double), ("watch", WATCH_CALLBACK), ("watch_user", c_void_p),
+                ("orth", c_int)]
 
 
 class LobpcgStats(ctypes.Structure):
@@ -92,6 +95,8 @@ SIGNATURES = {
     "qg_lobpcg_d_multi": (c_int, [c_void_p, P(LobpcgOpts), c_int, P(c_double), c_void_p, c_void_p,
                                   c_int, P(LobpcgStats)]),
     "qg_precond_apply": (c_int, [c_void_p, c_int, c_double, c_int, c_int, c_void_p, c_int64, c_void_p]),
+    "qg_gram": (c_int, [c_int, c_void_p, c_void_p, c_int64, c_int, c_int, c_int64, c_int64, c_void_p, c_int,
+                        c_void_p]),
     "qg_gram_bf16": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_int, c_int64, c_int64, c_void_p, c_int,
                              c_int, c_void_p]),
     "qg_cnn_num_params": (c_int, []),

@@ -218,6 +218,54 @@ __global__ void direct_xr_rp_kernel(int N, double* XtR, double* RtP, int ld, con
 
 static int grid_for(int64_t n) { return (int)std::min<int64_t>(4096, (n + 255) / 256); }
 
+// K chunks of the split-K Gram (each chunk summed by the library in the block precision, the
+// chunks in FP64).  FP32: chunks of at most ~6,750 terms (256 at 120^3) keep the Gram rounding
+// low enough for the FP32 phase to converge to ~2e-3 at 512 bands; 64 chunks of 27,000 terms
+// were 9% faster but stalled it near 7e-2.  FP64: 64 chunks (the fastest measured).
+// QG_GRAM_NCH overrides (measurement switch).
+template <class T>
+static int64_t gram_chunks(int64_t K) {
+    const int64_t want = env_int("QG_GRAM_NCH", 0);
+    if (want > 0) return std::max<int64_t>(1, std::min<int64_t>(want, K / 64));
+    return std::max<int64_t>(1, std::min<int64_t>(sizeof(T) == 4 ? 256 : 64, K / 2048));
+}
+template <class T>
+static size_t gram_workspace(int64_t K, int M, int N) {
+    return (size_t)(gram_chunks<T>(K) + 1) * M * N * sizeof(T);
+}
+
+// C (M x N, FP64, column stride ldc) = A^T B over K rows; A: M columns of length K (column m
+// at A + m lda), B: N columns (ld ldb).  The Gram blocks of the Rayleigh-Ritz matrices
+// (MakeMatrix_d_blas, lobpcg_d_multi_mpi.h:192-250): split-K over the rows as one
+// strided-batched library GEMM in the block precision (enough parallel work for small M, N),
+// then the chunks summed in FP64.
+template <class T>
+static void gram_splitk(rocblas_handle hb, hipStream_t s, const T* A, int64_t lda, const T* B, int64_t ldb, int64_t K,
+                        int M, int N, double* C, int ldc, T* ws, size_t ws_bytes) {
+    const int64_t nch = gram_chunks<T>(K);
+    const int64_t kc = K / nch;
+    const int64_t rem = K - nch * kc;
+    const int64_t nn = (int64_t)M * N;
+    QG_REQUIRE(ws_bytes >= (size_t)(nch + 1) * nn * sizeof(T), "split-K workspace too small");
+    if constexpr (sizeof(T) == 8) {
+        const double one = 1.0, zero = 0.0;
+        QG_BLAS(rocblas_dgemm_strided_batched(hb, rocblas_operation_transpose, rocblas_operation_none, M, N, (int)kc,
+                                              &one, A, (int)lda, kc, B, (int)ldb, kc, &zero, ws, M, nn, (int)nch));
+        if (rem)
+            QG_BLAS(rocblas_dgemm(hb, rocblas_operation_transpose, rocblas_operation_none, M, N, (int)rem, &one,
+                                  A + nch * kc, (int)lda, B + nch * kc, (int)ldb, &zero, ws + nch * nn, M));
+    } else {
+        const float one = 1.f, zero = 0.f;
+        QG_BLAS(rocblas_sgemm_strided_batched(hb, rocblas_operation_transpose, rocblas_operation_none, M, N, (int)kc,
+                                              &one, A, (int)lda, kc, B, (int)ldb, kc, &zero, ws, M, nn, (int)nch));
+        if (rem)
+            QG_BLAS(rocblas_sgemm(hb, rocblas_operation_transpose, rocblas_operation_none, M, N, (int)rem, &one,
+                                  A + nch * kc, (int)lda, B + nch * kc, (int)ldb, &zero, ws + nch * nn, M));
+    }
+    hipLaunchKernelGGL((split_reduce_kernel<T>), dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, ws,
+                       (int)(nch + (rem ? 1 : 0)), M, N, C, ldc);
+}
+
 // ------------------------------------------------------------------ solver state ----
 struct LobpcgState {
     int N = 0;
@@ -272,45 +320,10 @@ struct Ops {
                 return;
             }
         }
-        const int64_t nch = split_chunks(K, N);
-        const int64_t kc = K / nch;
-        const int64_t rem = K - nch * kc;
-        const int64_t nn = (int64_t)N * nb;
-        T* W = reinterpret_cast<T*>(st.split_ws.p);
-        QG_REQUIRE(st.split_ws.n >= (size_t)(nch + 1) * nn * sizeof(T), "split-K workspace too small");
-        if constexpr (sizeof(T) == 8) {
-            const double one = 1.0, zero = 0.0;
-            QG_BLAS(rocblas_dgemm_strided_batched(st.blas, rocblas_operation_transpose, rocblas_operation_none, N, nb,
-                                                  (int)kc, &one, A, (int)K, kc, B, (int)K, kc, &zero, W, N, nn,
-                                                  (int)nch));
-            if (rem)
-                QG_BLAS(rocblas_dgemm(st.blas, rocblas_operation_transpose, rocblas_operation_none, N, nb, (int)rem,
-                                      &one, A + nch * kc, (int)K, B + nch * kc, (int)K, &zero, W + nch * nn, N));
-        } else {
-            const float one = 1.f, zero = 0.f;
-            QG_BLAS(rocblas_sgemm_strided_batched(st.blas, rocblas_operation_transpose, rocblas_operation_none, N, nb,
-                                                  (int)kc, &one, A, (int)K, kc, B, (int)K, kc, &zero, W, N, nn,
-                                                  (int)nch));
-            if (rem)
-                QG_BLAS(rocblas_sgemm(st.blas, rocblas_operation_transpose, rocblas_operation_none, N, nb, (int)rem,
-                                      &one, A + nch * kc, (int)K, B + nch * kc, (int)K, &zero, W + nch * nn, N));
-        }
-        hipLaunchKernelGGL((split_reduce_kernel<T>), dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, W,
-                           (int)(nch + (rem ? 1 : 0)), N, nb, C, ldc);
-    }
-    // K chunks of the split-K Gram (each chunk summed by the library in the block precision,
-    // the chunks in FP64). FP32: chunks of at most ~6,750 terms (256 at 120^3) keep the Gram
-    // rounding low enough for the FP32 phase to converge to ~2e-3 at 512 bands; 64 chunks of
-    // 27,000 terms were 9% faster but stalled it near 7e-2. FP64: 64 chunks (the fastest
-    // measured). QG_GRAM_NCH overrides.
-    static int64_t split_chunks(int64_t K, int N) {
-        const int64_t want = env_int("QG_GRAM_NCH", 0);
-        if (want > 0) return std::max<int64_t>(1, std::min<int64_t>(want, K / 64));
-        return std::max<int64_t>(1, std::min<int64_t>(sizeof(T) == 4 ? 256 : 64, K / 2048));
+        gram_splitk<T>(st.blas, s, A, K, B, K, K, N, nb, C, ldc, reinterpret_cast<T*>(st.split_ws.p), st.split_ws.n);
     }
     static size_t split_workspace(int64_t K, int N, int nb_max = 0) {
-        const int64_t nch = split_chunks(K, N);
-        return (size_t)(nch + 1) * N * (nb_max > 0 ? nb_max : N) * sizeof(T);
+        return gram_workspace<T>(K, N, nb_max > 0 ? nb_max : N);
     }
     // Out = A * Cm + beta * Out, A: K x N (ld K), Cm: N x N FP64 (ld ldc) / FP32 copy for T = float
     void update(const T* A, const double* Cm, const float* Cm32, int ldc, double beta, T* Out) {
@@ -736,6 +749,25 @@ __global__ void svqb_transform_kernel(const double* U, int N, int ld, const doub
     if (i == 0) keep[j] = k ? 1 : 0;
 }
 
+// Cholesky-QR transform T = D R^-1: T_ij = d_i (R^-1)_ij for i <= j and 0 below the diagonal
+// (R^-1 is in the upper triangle of Rinv, whose strict lower part holds stale values); keep_j = 1
+__global__ void chol_transform_kernel(const double* Rinv, int N, int ld, const double* d, double* Tm, int ldt,
+                                      int* keep) {
+    const int j = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    Tm[i + (int64_t)ldt * j] = (i <= j) ? d[i] * Rinv[i + (int64_t)ld * j] : 0.0;
+    if (i == 0) keep[j] = 1;
+}
+
+// pivots of a Cholesky factor: p_j = R_jj^2
+__global__ void chol_pivots_kernel(const double* R, int N, int ld, double* p) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= N) return;
+    const double r = R[j + (int64_t)ld * j];
+    p[j] = r * r;
+}
+
 // dropped basis directions: large diagonal in the Rayleigh-Ritz matrix, so that none of the
 // lowest N eigenpairs uses them
 __global__ void mask_diag_kernel(double* A, int ld, const int* keep, int off, int N, double big) {
@@ -784,8 +816,8 @@ static void run_lobpcg_ortho(qg_ham* h, const qg_lobpcg_opts& o, int N, double* 
     st.C32.alloc(nn9);
     if (o.precision == QG_PREC_MIXED) st.gram_ws.alloc(gram_bf16_workspace(K, N, n3));
     st.split_ws.alloc(Ops<T>::split_workspace(K, N, n3));
-    DevBuf<double> Gm, Tr, dsc;
-    Gm.alloc(nn); Tr.alloc(nn); dsc.alloc(N);
+    DevBuf<double> Gm, Tr, dsc, Rc;
+    Gm.alloc(nn); Tr.alloc(nn); dsc.alloc(N); Rc.alloc(nn);
     DevBuf<int> keep;
     keep.alloc(n3);
 
@@ -855,6 +887,7 @@ static void run_lobpcg_ortho(qg_ham* h, const qg_lobpcg_opts& o, int N, double* 
     // largest (1: already orthonormal; small: the transform amplified rounding, so callers
     // orthogonalise once more).
     std::vector<double> theta(N);
+    int info2[2] = {0, 0};
     auto svqb = [&](T*& V, int* keep_out, bool drop) -> double {
         auto t0 = sector_begin();
         op.gram(V, V, Gm.p, N, true);
@@ -863,6 +896,43 @@ static void run_lobpcg_ortho(qg_ham* h, const qg_lobpcg_opts& o, int N, double* 
                            (const double*)dsc.p);
         sync();
         stats.t_gram += sector_end(t0, 17, 1);
+        if (o.orth != QG_ORTH_SVQB) {
+            // Cholesky-QR: D G D = R^T R, V <- V D R^-1 (a triangular factor and its inverse
+            // instead of an eigensolve).  A failed factor or a pivot ratio p_min / p_max at or
+            // below the drop threshold (near-dependent columns) goes to SVQB, which drops the
+            // dependent directions.  Returns the pivot ratio (1 for orthonormal columns).
+            t0 = sector_begin();
+            QG_HIP(hipMemcpyAsync(Rc.p, Gm.p, nn * sizeof(double), hipMemcpyDeviceToDevice, s));
+            QG_BLAS(rocsolver_dpotrf(st.blas, rocblas_fill_upper, N, Rc.p, N, st.info.p));
+            hipLaunchKernelGGL(chol_pivots_kernel, dim3((N + 255) / 256), dim3(256), 0, s, Rc.p, N, N, st.D.p);
+            QG_HIP(hipMemcpyAsync(&info2[0], st.info.p, sizeof(int), hipMemcpyDeviceToHost, s));
+            QG_HIP(hipMemcpyAsync(theta.data(), st.D.p, N * 8, hipMemcpyDeviceToHost, s));
+            sync();
+            double pmax = 0.0, pmin = 0.0;
+            if (info2[0] == 0) {
+                pmax = *std::max_element(theta.begin(), theta.end());
+                pmin = *std::min_element(theta.begin(), theta.end());
+            }
+            const double ratio = pmax > 0.0 ? pmin / pmax : 0.0;
+            if (info2[0] == 0 && pmin > 0.0 && ratio > (drop ? tau : 0.0)) {
+                QG_BLAS(rocsolver_dtrtri(st.blas, rocblas_fill_upper, rocblas_diagonal_non_unit, N, Rc.p, N,
+                                         st.info.p));
+                QG_HIP(hipMemcpyAsync(&info2[1], st.info.p, sizeof(int), hipMemcpyDeviceToHost, s));
+                sync();
+                if (info2[1] == 0) {
+                    hipLaunchKernelGGL(chol_transform_kernel, dim3((N + 255) / 256, N), dim3(256), 0, s, Rc.p, N, N,
+                                       (const double*)dsc.p, Tr.p, N, keep_out);
+                    stats.t_eig += sector_end(t0, 15, 1);
+                    t0 = sector_begin();
+                    upd(V, Tr.p, N, 1.0, 0.0, Tm);
+                    std::swap(V, Tm);
+                    sync();
+                    stats.t_update += sector_end(t0, 16, 1);
+                    return ratio;
+                }
+            }
+            stats.t_eig += sector_end(t0, 15, 1);
+        }
         const int info = eigh(Gm.p, N, N, st.D.p);
         QG_REQUIRE(info == 0, "SVQB eigensolve failed");
         hipLaunchKernelGGL(svqb_transform_kernel, dim3((N + 255) / 256, N), dim3(256), 0, s, Gm.p, N, N, st.D.p,
@@ -1076,6 +1146,36 @@ __global__ void strided_convert_kernel(const Ts* x, int64_t ld_x, Td* y, int64_t
 
 extern "C" {
 
+int qg_gram(int dtype, const void* A, const void* B, int64_t K, int M, int N, int64_t lda, int64_t ldb, double* C,
+            int ldc, void* stream) {
+    return guarded([&] {
+        QG_REQUIRE(dtype == QG_F64 || dtype == QG_F32, "bad dtype");
+        QG_REQUIRE(K >= 1 && K < (1LL << 31) && M >= 0 && N >= 0 && lda >= K && ldb >= K && ldc >= M, "bad sizes");
+        if (M == 0 || N == 0) return;
+        QG_REQUIRE(A && B && C, "null pointer");
+        hipStream_t s = as_stream(stream);
+        rocblas_handle hb = nullptr;
+        QG_BLAS(rocblas_create_handle(&hb));
+        struct HandleGuard {
+            rocblas_handle h;
+            ~HandleGuard() { rocblas_destroy_handle(h); }
+        } guard{hb};
+        QG_BLAS(rocblas_set_stream(hb, s));
+        DevBuf<unsigned char> ws;
+        if (dtype == QG_F64) {
+            ws.alloc(gram_workspace<double>(K, M, N));
+            gram_splitk<double>(hb, s, (const double*)A, lda, (const double*)B, ldb, K, M, N, C, ldc,
+                                reinterpret_cast<double*>(ws.p), ws.n);
+        } else {
+            ws.alloc(gram_workspace<float>(K, M, N));
+            gram_splitk<float>(hb, s, (const float*)A, lda, (const float*)B, ldb, K, M, N, C, ldc,
+                               reinterpret_cast<float*>(ws.p), ws.n);
+        }
+        // the workspace is freed on return
+        QG_HIP(hipStreamSynchronize(s));
+    });
+}
+
 int qg_gram_schmidt(int dtype, int64_t ng, int nbands, void* X, int64_t ld, double dvol, void* stream) {
     return guarded([&] {
         QG_REQUIRE(dtype == QG_F64 || dtype == QG_F32, "bad dtype");
@@ -1154,6 +1254,7 @@ void qg_lobpcg_default_opts(qg_lobpcg_opts* o) {
     o->fallback_shift = 0.0;
     o->watch = nullptr;
     o->watch_user = nullptr;
+    o->orth = QG_ORTH_CHOLQR;
 }
 
 int qg_lobpcg_d_multi(qg_ham* h, const qg_lobpcg_opts* opts, int nbands, double* eig, double* X, double* P,
