@@ -340,6 +340,7 @@ def lobpcg_iterations(cfg, iters):
     out = {"workload": f"{len(s['positions'])} atoms, {s['n']}^3 grid, {nb} bands", "iterations": iters,
            "flops_per_NxNxNg_product": gemm}
     for name, prec in (("f64", N.QG_PREC_F64), ("f32", N.QG_PREC_F32), ("mixed", N.QG_PREC_MIXED)):
+        log(f"lobpcg {name}: {iters} iterations on {out['workload']}")
         X = torch.from_numpy(X0.copy()).cuda()
         P = torch.zeros_like(X)
         # one warm-up iteration (library handles, workspaces), then the timed ones from the same start

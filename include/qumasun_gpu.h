@@ -140,6 +140,27 @@ int qg_hamiltonian_apply(qg_ham* h, int dtype, int nbands, const void* psi, int6
  * share that array: calls that overlap on one GPU (other streams, other threads) race. */
 int qg_hamiltonian_apply_noalloc(qg_ham* h, int dtype, int nbands, const void* psi, int64_t ld,
                                  void* hpsi, int64_t ld_out, void* stream);
+/* Kernel plan of a Hamiltonian's H psi (no counterpart in the reference).  The row kernels
+ * (a tile-centric projection launch and one stencil + V + projector launch) serve grids with
+ * Nx a multiple of 4 (FP32) or 2 (FP64), 8 <= Nx <= 1024, and Gaussian projectors with l <= 1;
+ * the table kernels serve everything.  Zero fields are chosen automatically from the grid and
+ * the band count.  The QG_ROWS_* environment variables override the shape fields (measurement
+ * switches). */
+#define QG_KERNELS_AUTO 0     /* row kernels where they serve, table kernels otherwise */
+#define QG_KERNELS_ROWS 1     /* row kernels only (QG_ERR_ARG where they do not serve) */
+#define QG_KERNELS_TABLES 2   /* table kernels */
+typedef struct qg_hpsi_plan {
+    int kernels;      /* QG_KERNELS_* */
+    int rows_ty;      /* apply kernel: rows per y-tile */
+    int rows_bands;   /* apply kernel: bands per workgroup (1, 2 or 4) */
+    int rows_tz;      /* apply kernel: z-planes per slab */
+    int rows_tzp;     /* projection: z-planes per slab (default 8; the projector geometry of other
+                         values is built at the first application) */
+    int rows_nbp;     /* projection: bands per workgroup (4 or 8) */
+} qg_hpsi_plan;
+void qg_default_hpsi_plan(qg_hpsi_plan* plan);
+int qg_hamiltonian_set_plan(qg_ham* h, const qg_hpsi_plan* plan);
+
 /* Sector timers of qg_hamiltonian_apply on this handle (hipEvents recorded on the apply's
  * stream), the counterpart of the reference's StopWatch sectors 13 (PP nonlocal) and
  * 11/12 (kinetic, V) (qumasun_mpi.matrix.h:30-41): t_project_ms is the total time of the
@@ -210,6 +231,13 @@ typedef struct qg_lobpcg_opts {
     void (*watch)(void* user, int sector, int64_t count);
     void* watch_user;
     int orth;                /* QG_ORTH_* (orthonormal basis only) */
+    int gram_chunks;         /* K chunks of the split-K library Gram products (each chunk summed
+                                in the block precision, the chunks in FP64); 0: 64 in FP64,
+                                chunks of at most ~6,750 terms in FP32 (256 at 120^3: longer FP32
+                                partial sums stalled the FP32 phase at 512 bands) */
+    double reortho;          /* orthonormal basis: a second orthonormalisation pass runs when the
+                                first one's pivot (or SVQB) ratio is at or below this; 0: 1e-3 in
+                                FP64, always in FP32 */
 } qg_lobpcg_opts;
 
 typedef struct qg_lobpcg_stats {

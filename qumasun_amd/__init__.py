@@ -217,6 +217,15 @@ class Hamiltonian:
         check(lib().qg_hamiltonian_apply(self._h, _dtype_code(psi), nb, pp, ld, op, ldo, _stream()))
         return out
 
+    def set_plan(self, kernels=N.QG_KERNELS_AUTO, rows_ty=0, rows_bands=0, rows_tz=0, rows_tzp=0, rows_nbp=0):
+        """Kernel plan of H psi (qg_hamiltonian_set_plan): which kernel family, and the row
+        kernels' tile shape (0: automatic)."""
+        p = N.HpsiPlan()
+        lib().qg_default_hpsi_plan(ctypes.byref(p))
+        p.kernels, p.rows_ty, p.rows_bands, p.rows_tz, p.rows_tzp, p.rows_nbp = (kernels, rows_ty, rows_bands, rows_tz,
+                                                                               rows_tzp, rows_nbp)
+        check(lib().qg_hamiltonian_set_plan(self._h, ctypes.byref(p)))
+
     def set_timing(self, enable=True):
         check(lib().qg_hamiltonian_set_timing(self._h, int(bool(enable))))
 

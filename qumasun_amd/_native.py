@@ -24,6 +24,9 @@ QG_BASIS_REFERENCE = 0
 QG_BASIS_ORTHO = 1
 QG_ORTH_CHOLQR = 0
 QG_ORTH_SVQB = 1
+QG_KERNELS_AUTO = 0
+QG_KERNELS_ROWS = 1
+QG_KERNELS_TABLES = 2
 
 c_int, c_double, c_void_p, c_int64, c_uint32, c_uint64 = (ctypes.c_int, ctypes.c_double, ctypes.c_void_p,
                                                            ctypes.c_int64, ctypes.c_uint32, ctypes.c_uint64)
@@ -51,7 +54,13 @@ class LobpcgOpts(ctypes.Structure):
                 ("precond_shift", c_double), ("verbose", c_int), ("switch_tol", c_double),
                 ("stall_window", c_int), ("op", OP_CALLBACK), ("op_user", c_void_p), ("basis", c_int),
                 ("fallback_shift", c_double), ("watch", WATCH_CALLBACK), ("watch_user", c_void_p),
-                ("orth", c_int)]
+                ("orth", c_int), ("gram_chunks", c_int), ("reortho", c_double)]
+
+
+class HpsiPlan(ctypes.Structure):
+    """qg_hpsi_plan"""
+    _fields_ = [("kernels", c_int), ("rows_ty", c_int), ("rows_bands", c_int), ("rows_tz", c_int),
+                ("rows_tzp", c_int), ("rows_nbp", c_int)]
 
 
 class LobpcgStats(ctypes.Structure):
@@ -88,6 +97,8 @@ SIGNATURES = {
                                      c_void_p]),
     "qg_hamiltonian_apply_noalloc": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, c_void_p, c_int64,
                                              c_void_p]),
+    "qg_default_hpsi_plan": (None, [P(HpsiPlan)]),
+    "qg_hamiltonian_set_plan": (c_int, [c_void_p, P(HpsiPlan)]),
     "qg_hamiltonian_set_timing": (c_int, [c_void_p, c_int]),
     "qg_hamiltonian_last_times": (c_int, [c_void_p, P(c_double), P(c_double)]),
     "qg_hamiltonian_last_chunks": (c_int, [c_void_p, P(c_int)]),

@@ -1643,6 +1643,25 @@ int qg_hamiltonian_apply(qg_ham* h, int dtype, int nbands, const void* psi, int6
     });
 }
 
+void qg_default_hpsi_plan(qg_hpsi_plan* plan) {
+    if (plan) *plan = qg_hpsi_plan{QG_KERNELS_AUTO, 0, 0, 0, 0, 0};
+}
+
+int qg_hamiltonian_set_plan(qg_ham* h, const qg_hpsi_plan* plan) {
+    return guarded([&] {
+        QG_REQUIRE(h, "null hamiltonian");
+        qg_hpsi_plan p{};
+        if (plan) p = *plan;
+        QG_REQUIRE(p.kernels == QG_KERNELS_AUTO || p.kernels == QG_KERNELS_ROWS || p.kernels == QG_KERNELS_TABLES,
+                   "unknown kernel selection");
+        QG_REQUIRE(p.rows_ty >= 0 && p.rows_ty <= 32 && p.rows_tz >= 0 && p.rows_tzp >= 0, "bad tile sizes");
+        QG_REQUIRE(p.rows_bands == 0 || p.rows_bands == 1 || p.rows_bands == 2 || p.rows_bands == 4,
+                   "bands per workgroup must be 1, 2 or 4");
+        QG_REQUIRE(p.rows_nbp == 0 || p.rows_nbp == 4 || p.rows_nbp == 8, "projection bands must be 4 or 8");
+        h->plan = p;
+    });
+}
+
 int qg_hamiltonian_apply_noalloc(qg_ham* h, int dtype, int nbands, const void* psi, int64_t ld, void* hpsi,
                                  int64_t ld_out, void* stream) {
     return guarded([&] {

@@ -269,6 +269,7 @@ void rows_apply_noalloc(qg_ham* h, int dtype, int nbands, const void* psi, int64
 
 struct qg_ham {
     qg_grid grid;
+    qg_hpsi_plan plan{};     // kernel plan of H psi (zeros: automatic)
     qg_cnn* cnn = nullptr;   // learned preconditioner (not owned)
     qg::BatchService* batch = nullptr;   // set while a batched solve runs this cell
     qg::LobpcgState* lobpcg = nullptr;

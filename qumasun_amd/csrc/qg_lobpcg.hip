@@ -107,7 +107,7 @@ static void batch_service_leave(BatchService* b, hipStream_t s) {
     if (b->active > 0 && (int)b->pending.size() == b->active) b->execute(s);
 }
 void gram_bf16(const float* A, const float* B, int64_t K, int M, int N, int64_t lda, int64_t ldb,
-               double* C, int ldc, float* partial, hipStream_t s, bool split);
+               double* C, int ldc, float* partial, size_t partial_floats, hipStream_t s, bool split);
 size_t gram_bf16_workspace(int64_t K, int M, int N);
 
 #define QG_BLAS(call)                                                                        \
@@ -221,17 +221,16 @@ static int grid_for(int64_t n) { return (int)std::min<int64_t>(4096, (n + 255) /
 // K chunks of the split-K Gram (each chunk summed by the library in the block precision, the
 // chunks in FP64).  FP32: chunks of at most ~6,750 terms (256 at 120^3) keep the Gram rounding
 // low enough for the FP32 phase to converge to ~2e-3 at 512 bands; 64 chunks of 27,000 terms
-// were 9% faster but stalled it near 7e-2.  FP64: 64 chunks (the fastest measured).
-// QG_GRAM_NCH overrides (measurement switch).
+// were 9% faster but stalled it near 7e-2.  FP64: 64 chunks (the fastest measured).  want > 0
+// (the solver's gram_chunks option) chooses the count.
 template <class T>
-static int64_t gram_chunks(int64_t K) {
-    const int64_t want = env_int("QG_GRAM_NCH", 0);
+static int64_t gram_chunks(int64_t K, int want = 0) {
     if (want > 0) return std::max<int64_t>(1, std::min<int64_t>(want, K / 64));
     return std::max<int64_t>(1, std::min<int64_t>(sizeof(T) == 4 ? 256 : 64, K / 2048));
 }
 template <class T>
-static size_t gram_workspace(int64_t K, int M, int N) {
-    return (size_t)(gram_chunks<T>(K) + 1) * M * N * sizeof(T);
+static size_t gram_workspace(int64_t K, int M, int N, int want = 0) {
+    return (size_t)(gram_chunks<T>(K, want) + 1) * M * N * sizeof(T);
 }
 
 // C (M x N, FP64, column stride ldc) = A^T B over K rows; A: M columns of length K (column m
@@ -241,8 +240,8 @@ static size_t gram_workspace(int64_t K, int M, int N) {
 // then the chunks summed in FP64.
 template <class T>
 static void gram_splitk(rocblas_handle hb, hipStream_t s, const T* A, int64_t lda, const T* B, int64_t ldb, int64_t K,
-                        int M, int N, double* C, int ldc, T* ws, size_t ws_bytes) {
-    const int64_t nch = gram_chunks<T>(K);
+                        int M, int N, double* C, int ldc, T* ws, size_t ws_bytes, int want_chunks = 0) {
+    const int64_t nch = gram_chunks<T>(K, want_chunks);
     const int64_t kc = K / nch;
     const int64_t rem = K - nch * kc;
     const int64_t nn = (int64_t)M * N;
@@ -304,6 +303,7 @@ struct Ops {
     int N;
     int64_t K;
     int prec;
+    int nch = 0;    // split-K Gram chunks (0: the block precision's default)
     T* blk(DevBuf<unsigned char>& b) { return reinterpret_cast<T*>(b.p); }
 
     // C (ld ldc, FP64) = A^T B over K rows, A and B are N columns of length K (ld K).
@@ -316,14 +316,15 @@ struct Ops {
     void gram_wide(const T* A, const T* B, int nb, double* C, int ldc, bool s_matrix = false) {
         if constexpr (sizeof(T) == 4) {
             if (prec == QG_PREC_MIXED && !s_matrix) {
-                gram_bf16(A, B, K, N, nb, K, K, C, ldc, st.gram_ws.p, s, true);
+                gram_bf16(A, B, K, N, nb, K, K, C, ldc, st.gram_ws.p, st.gram_ws.n, s, true);
                 return;
             }
         }
-        gram_splitk<T>(st.blas, s, A, K, B, K, K, N, nb, C, ldc, reinterpret_cast<T*>(st.split_ws.p), st.split_ws.n);
+        gram_splitk<T>(st.blas, s, A, K, B, K, K, N, nb, C, ldc, reinterpret_cast<T*>(st.split_ws.p), st.split_ws.n,
+                       nch);
     }
-    static size_t split_workspace(int64_t K, int N, int nb_max = 0) {
-        return gram_workspace<T>(K, N, nb_max > 0 ? nb_max : N);
+    static size_t split_workspace(int64_t K, int N, int nb_max = 0, int want = 0) {
+        return gram_workspace<T>(K, N, nb_max > 0 ? nb_max : N, want);
     }
     // Out = A * Cm + beta * Out, A: K x N (ld K), Cm: N x N FP64 (ld ldc) / FP32 copy for T = float
     void update(const T* A, const double* Cm, const float* Cm32, int ldc, double beta, T* Out) {
@@ -416,11 +417,11 @@ static void run_lobpcg(qg_ham* h, const qg_lobpcg_opts& o, int N, double* eig_ho
     st.eig.alloc(N); st.norm_a.alloc(N); st.norm_b.alloc(N); st.res2.alloc(N);
     if (sizeof(T) != 8) { st.C32.alloc(nn9); st.G32.alloc((size_t)N * N); }
     if (o.precision == QG_PREC_MIXED) st.gram_ws.alloc(gram_bf16_workspace(K, N, N));
-    st.split_ws.alloc(Ops<T>::split_workspace(K, N));
+    st.split_ws.alloc(Ops<T>::split_workspace(K, N, 0, o.gram_chunks));
     QG_HIP(hipMemsetAsync(st.Sa.p, 0, (nn9 + 4 * (int64_t)N * N) * 8, s));
     QG_HIP(hipMemsetAsync(st.Sb.p, 0, (nn9 + 4 * (int64_t)N * N) * 8, s));
 
-    Ops<T> op{st, s, N, K, o.precision};
+    Ops<T> op{st, s, N, K, o.precision, o.gram_chunks};
     T* X = (sizeof(T) == 8) ? reinterpret_cast<T*>(X64) : op.blk(st.Xw);
     T* P = (sizeof(T) == 8) ? reinterpret_cast<T*>(P64) : op.blk(st.Pw);
     T* R = op.blk(st.R);
@@ -815,13 +816,13 @@ static void run_lobpcg_ortho(qg_ham* h, const qg_lobpcg_opts& o, int N, double* 
     st.eig.alloc(N); st.norm_a.alloc(N); st.norm_b.alloc(N); st.res2.alloc(N);
     st.C32.alloc(nn9);
     if (o.precision == QG_PREC_MIXED) st.gram_ws.alloc(gram_bf16_workspace(K, N, n3));
-    st.split_ws.alloc(Ops<T>::split_workspace(K, N, n3));
+    st.split_ws.alloc(Ops<T>::split_workspace(K, N, n3, o.gram_chunks));
     DevBuf<double> Gm, Tr, dsc, Rc;
     Gm.alloc(nn); Tr.alloc(nn); dsc.alloc(N); Rc.alloc(nn);
     DevBuf<int> keep;
     keep.alloc(n3);
 
-    Ops<T> op{st, s, N, K, o.precision};
+    Ops<T> op{st, s, N, K, o.precision, o.gram_chunks};
     // X and P live in the caller's FP64 arrays for T = double; blocks are swapped by pointer
     T* X = (sizeof(T) == 8) ? reinterpret_cast<T*>(X64) : op.blk(st.Xw);
     T* P = (sizeof(T) == 8) ? reinterpret_cast<T*>(P64) : op.blk(st.Pw);
@@ -949,10 +950,9 @@ static void run_lobpcg_ortho(qg_ham* h, const qg_lobpcg_opts& o, int N, double* 
             if (theta[j] > (drop ? tau : 0.0) * tmax && theta[j] > 0.0) tmin = std::min(tmin, theta[j]);
         return tmax > 0.0 ? tmin / tmax : 1.0;
     };
-    // second pass when the first SVQB amplified by more than ~30 (FP64), always in FP32
-    // (QG_REORTHO32_MILLI = 1000 x the FP32 threshold overrides, for measurements)
-    const int r32 = env_int("QG_REORTHO32_MILLI", 0);
-    const double reortho = sizeof(T) == 8 ? 1e-3 : (r32 > 0 ? r32 * 1e-3 : 2.0);
+    // second pass when the first one amplified rounding (ratio at or below 1e-3) in FP64, always
+    // in FP32 (a ratio never exceeds 1); the reortho option overrides
+    const double reortho = o.reortho > 0.0 ? o.reortho : (sizeof(T) == 8 ? 1e-3 : 2.0);
     // Rayleigh-Ritz on the (orthonormal) columns of V with AV: V, AV rotated to Ritz vectors
     auto rr_single = [&](T*& V, T* AV) {
         op.gram(V, AV, st.A3.p, N);
@@ -1255,6 +1255,8 @@ void qg_lobpcg_default_opts(qg_lobpcg_opts* o) {
     o->watch = nullptr;
     o->watch_user = nullptr;
     o->orth = QG_ORTH_CHOLQR;
+    o->gram_chunks = 0;
+    o->reortho = 0.0;
 }
 
 int qg_lobpcg_d_multi(qg_ham* h, const qg_lobpcg_opts* opts, int nbands, double* eig, double* X, double* P,

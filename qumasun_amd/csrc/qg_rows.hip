@@ -810,29 +810,30 @@ struct RowsShape {
 
 // Launch shape for nbands bands (0: only the geometry-defining fields, ty and tzp, which do not
 // depend on the band count).  QG_ROWS_* environment variables override (measurement switches).
-static RowsShape rows_shape(const qg_grid& g, int dtype, int nbands) {
+static RowsShape rows_shape(const qg_grid& g, int dtype, int nbands, const qg_hpsi_plan* plan = nullptr) {
     const int VN = dtype == QG_F32 ? 4 : 2;
     const int nq = g.n[0] / VN;
+    const qg_hpsi_plan pl = plan ? *plan : qg_hpsi_plan{};
     RowsShape sh{};
-    sh.ty = rows_pick_ty(nq, g.n[1], env_int("QG_ROWS_TY", 0));
+    sh.ty = rows_pick_ty(nq, g.n[1], env_int("QG_ROWS_TY", pl.rows_ty));
     const int ntile = std::max(1, (g.n[1] + std::max(1, sh.ty) - 1) / std::max(1, sh.ty));
     // about 4096 workgroups per launch: 2 bands per workgroup (FP32) when the tiles and bands
     // give that many, else 1, and z-slabs (at least 8 planes each) for the rest
     constexpr int kWantWg = 4096;
     int nbw = 1;
     if (dtype == QG_F32 && (int64_t)ntile * ((nbands + 1) / 2) >= kWantWg / 2) nbw = 2;
-    nbw = env_int("QG_ROWS_NBW", nbw);
+    nbw = env_int("QG_ROWS_NBW", pl.rows_bands > 0 ? pl.rows_bands : nbw);
     if (nbw != 1 && nbw != 2 && nbw != 4) nbw = 1;
     while (nbw > 1 && nbw * round64i(std::max(sh.ty, 8) * nq) > (nbw >= 4 ? 1024 : 512)) nbw /= 2;
     sh.nbw = nbw;
     const int64_t wg = (int64_t)ntile * std::max(1, (nbands + nbw - 1) / nbw);
     const int nslab = (int)std::max<int64_t>(1, std::min<int64_t>((kWantWg + wg - 1) / wg, std::max(1, g.n[2] / 8)));
-    sh.tz = env_int("QG_ROWS_TZ", (g.n[2] + nslab - 1) / nslab);
+    sh.tz = env_int("QG_ROWS_TZ", pl.rows_tz > 0 ? pl.rows_tz : (g.n[2] + nslab - 1) / nslab);
     sh.tz = std::max(1, std::min(g.n[2], sh.tz));
-    sh.tzp = std::max(1, std::min(g.n[2], env_int("QG_ROWS_TZP", 8)));
+    sh.tzp = std::max(1, std::min(g.n[2], env_int("QG_ROWS_TZP", pl.rows_tzp > 0 ? pl.rows_tzp : 8)));
     // FP64 projection with 4 bands per workgroup: its reduction buffer (256 threads x 4 slots
     // x bands) stays at 32 KB
-    sh.nbp = env_int("QG_ROWS_NBP", dtype == QG_F32 ? 8 : 4) == 4 ? 4 : 8;
+    sh.nbp = env_int("QG_ROWS_NBP", pl.rows_nbp > 0 ? pl.rows_nbp : (dtype == QG_F32 ? 8 : 4)) == 4 ? 4 : 8;
     sh.typ = rows_pick_typ(nq, g.n[1], sh.nbp);
     return sh;
 }
@@ -1033,7 +1034,7 @@ static bool rows_apply_t(qg_ham* h, int nbands, const T* psi, int64_t ld, T* hps
     qg_nonlocal* nl = (h->nl && h->nl->geo.ncoef > 0) ? h->nl : nullptr;
     const int dtype = sizeof(T) == 4 ? QG_F32 : QG_F64;
     if (!rows_eligible(g, dtype, nl)) return false;
-    const RowsShape sh = rows_shape(g, dtype, nbands);
+    const RowsShape sh = rows_shape(g, dtype, nbands, &h->plan);
     const RowsGeo* rg = nullptr;    // apply kernel's geometry (tile rows ty)
     const RowsGeo* rgp = nullptr;   // projection kernel's geometry (tile rows typ)
     if (nl) {
@@ -1100,6 +1101,7 @@ static bool rows_apply_t(qg_ham* h, int nbands, const T* psi, int64_t ld, T* hps
 
 bool rows_hamiltonian_apply(qg_ham* h, int dtype, int nbands, const void* psi, int64_t ld, void* hpsi,
                             int64_t ld_out, hipStream_t s, RowsScratch* ws) {
+    if (h->plan.kernels == QG_KERNELS_TABLES) return false;
     if (h->timing) {
         while (h->tev.size() < 3) {
             hipEvent_t e;
@@ -1112,6 +1114,8 @@ bool rows_hamiltonian_apply(qg_ham* h, int dtype, int nbands, const void* psi, i
         done = rows_apply_t<float>(h, nbands, (const float*)psi, ld, (float*)hpsi, ld_out, s, ws);
     else
         done = rows_apply_t<double>(h, nbands, (const double*)psi, ld, (double*)hpsi, ld_out, s, ws);
+    QG_REQUIRE(done || h->plan.kernels != QG_KERNELS_ROWS,
+               "the plan asks for the row kernels, which do not serve this grid and projector set");
     if (done && h->timing) {
         h->tchunks = 1;
         h->timed_valid = true;
@@ -1227,7 +1231,7 @@ void rows_apply_noalloc(qg_ham* h, int dtype, int nbands, const void* psi, int64
     RowsScratch ws{scratch_address(), kScratchDoubles};
     int chunk = nbands;
     if (nl) {
-        const RowsShape sh = rows_shape(g, dtype, nbands);
+        const RowsShape sh = rows_shape(g, dtype, nbands, &h->plan);
         QG_REQUIRE(sh.typ > 0, "the projection tile does not fit this grid");
         const RowsGeo* rgp = rows_geometry(nl, sh.typ, sh.tzp, false, false);
         QG_REQUIRE(rgp && rgp->ok, "the projector geometry of this launch shape is not prepared (qg_nonlocal_create "

@@ -376,3 +376,43 @@ def test_rows_general_projector_sets(gpu, oracle, prec, nb):
     assert rel(coef, coef_ref) <= (TOL64 if prec == "f64" else 2e-6)
     loc = oracle.hamiltonian_block(n, h, V, None, X)
     assert rel(_np(out) - loc, ref - loc) <= (1e-11 if prec == "f64" else 2e-5)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_hpsi_plans(gpu, oracle, prec):
+    """Both kernel families and explicit row-kernel shapes (through qg_hamiltonian_set_plan)
+    give the oracle's H psi; asking for the row kernels where they do not serve is refused."""
+    import torch
+    from qumasun_amd import _native as N
+    print(f"hpsi plans {prec}", flush=True)
+    s = Q.diamond_system(1, 24)
+    n, h = s["n"], s["h"]
+    g = Q.Grid(n, n, n, h, h, h)
+    integ = Q.PseudoPotIntegrator(g, s["projectors"])
+    integ.UpdatePosition(s["positions"])
+    H = Q.Hamiltonian(g, s["V"], integ)
+    X = Q.random_states(g.size, 6, g.dvol)
+    onl = oracle.OracleNL((n,) * 3, (h,) * 3, s["positions"], np.zeros(len(s["positions"]), np.int32),
+                          s["projectors"])
+    ref = oracle.hamiltonian_block((n,) * 3, (h,) * 3, s["V"], onl, X)
+    dt = torch.float64 if prec == "f64" else torch.float32
+    x = _t(X, dt, gpu)
+    tol = TOL64 if prec == "f64" else TOL32
+    plans = [dict(), dict(kernels=N.QG_KERNELS_TABLES), dict(kernels=N.QG_KERNELS_ROWS, rows_ty=4, rows_tz=8),
+             dict(kernels=N.QG_KERNELS_ROWS, rows_bands=1, rows_tzp=5, rows_nbp=4)]
+    for plan in plans:
+        H.set_plan(**plan)
+        out = torch.empty_like(x)
+        H.apply(x, out)
+        torch.cuda.synchronize()
+        assert rel(_np(out), ref) <= tol, (plan, rel(_np(out), ref))
+    # Nx = 42 FP32: rows of 16-byte vectors do not fit, so the row kernels refuse
+    s2 = Q.diamond_system(1, 42)
+    g2 = Q.Grid(42, 42, 42, s2["h"], s2["h"], s2["h"])
+    i2 = Q.PseudoPotIntegrator(g2, s2["projectors"])
+    i2.UpdatePosition(s2["positions"])
+    H2 = Q.Hamiltonian(g2, s2["V"], i2)
+    H2.set_plan(kernels=N.QG_KERNELS_ROWS)
+    x2 = torch.zeros((1, g2.size), device=gpu, dtype=torch.float32)
+    with pytest.raises(Q.QGError):
+        H2.apply(x2, torch.empty_like(x2))
