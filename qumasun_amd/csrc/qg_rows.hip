@@ -73,10 +73,11 @@ struct Coef8 {
     T cx[4], cy[4], cz[4];
 };
 
-// exp(-rr k') with k' = k log2(e) in FP32 (hardware exp2), exp(-rr k) in FP64
+// exp(-rr k') with k' = k log2(e) in FP32 (the hardware exp2, v_exp_f32: results below the
+// normal range flush to zero, ~1e-38 of the amplitude), exp(-rr k) in FP64
 template <class T>
 __device__ __forceinline__ T gauss(T rr, T k) {
-    if constexpr (sizeof(T) == 4) return exp2f(-(rr * k));
+    if constexpr (sizeof(T) == 4) return __builtin_amdgcn_exp2f(-(rr * k));
     else return exp(-(rr * k));
 }
 // 1/sqrt(rr): the hardware FP32 estimate; in FP64 that estimate refined by two Newton steps
@@ -158,11 +159,48 @@ struct RowsProjArgs {
 // projector l, coefficient slot, amplitude with its Y_lm constant, and exponent factor.
 template <class T>
 struct ProjLds {
+    T* im_q;       // [max_col][8]: f.x f.y f.z, s amplitude * Y00, s exponent, p amplitude * sqrt(3/4pi),
+                   // p exponent, 0 (the s + p forms; 16-byte rows)
+    int* im_n;     // [max_col][4]: n.x n.y n.z pb (16-byte rows)
     int* im_i;     // [max_col][5]: n.x n.y n.z pb pe
     T* im_f;       // [max_col][3]: f
     int* pr_i;     // [nproj][2]: l, coef_rel
     T* pr_f;       // [nproj][2]: amp * (Y00 or sqrt(3/4pi)), exponent factor
 };
+
+// LDS bytes of the projector staging (ProjLds) for max_col images and nproj projectors
+template <class T>
+__host__ __device__ constexpr size_t proj_lds_bytes(size_t max_col, size_t nproj) {
+    return (8 * sizeof(T) + 16) * max_col + sizeof(T) * (3 * max_col + 2 * nproj) +
+           sizeof(int) * (5 * max_col + 2 * nproj);
+}
+
+// carve the projector staging out of LDS at lp (16-byte aligned)
+template <class T>
+__device__ __forceinline__ ProjLds<T> carve_proj(unsigned char* lp, int max_col, int nproj) {
+    ProjLds<T> L;
+    L.im_q = reinterpret_cast<T*>(lp);
+    lp += sizeof(T) * 8 * max_col;
+    L.im_n = reinterpret_cast<int*>(lp);
+    lp += sizeof(int) * 4 * max_col;
+    L.im_f = reinterpret_cast<T*>(lp);
+    lp += sizeof(T) * 3 * max_col;
+    L.pr_f = reinterpret_cast<T*>(lp);
+    lp += sizeof(T) * 2 * nproj;
+    L.im_i = reinterpret_cast<int*>(lp);
+    lp += sizeof(int) * 5 * max_col;
+    L.pr_i = reinterpret_cast<int*>(lp);
+    return L;
+}
+
+// amplitude with its Y_lm constant, and exponent factor of projector d (log2(e) folded in for
+// the FP32 exp2)
+template <class T>
+__device__ __forceinline__ void proj_consts(const DevProj& d, T& amp, T& k) {
+    const double ylm = d.l == 0 ? 0.28209479177387814 : 0.48860251190291992;   // 1/sqrt(4pi), sqrt(3/4pi)
+    amp = (T)(d.amp * ylm);
+    k = (sizeof(T) == 4) ? (T)(d.inv_2s2 * 1.4426950408889634) : (T)d.inv_2s2;
+}
 
 template <class T>
 __device__ __forceinline__ void stage_proj(const ProjLds<T>& L, const DevImage* images, const DevProj* projs,
@@ -175,18 +213,67 @@ __device__ __forceinline__ void stage_proj(const ProjLds<T>& L, const DevImage* 
             const double u = im.R[d] / h[d];
             const double nn = rint(u);
             L.im_f[3 * li + d] = (T)(u - nn);
+            L.im_q[8 * li + d] = (T)(u - nn);
             L.im_i[5 * li + d] = im.shift[d] + (int)nn;
+            L.im_n[4 * li + d] = im.shift[d] + (int)nn;
         }
         L.im_i[5 * li + 3] = im.proj_begin;
         L.im_i[5 * li + 4] = im.proj_end;
+        L.im_n[4 * li + 3] = im.proj_begin;
+        T as = T(0), ks = T(0), ap = T(0), kp = T(0);
+        if (im.proj_end > im.proj_begin) proj_consts<T>(projs[im.proj_begin], as, ks);
+        if (im.proj_end > im.proj_begin + 1) proj_consts<T>(projs[im.proj_begin + 1], ap, kp);
+        L.im_q[8 * li + 3] = as;
+        L.im_q[8 * li + 4] = ks;
+        L.im_q[8 * li + 5] = ap;
+        L.im_q[8 * li + 6] = kp;
+        L.im_q[8 * li + 7] = T(0);
     }
     for (int p = tid; p < nproj; p += nthr) {
         const DevProj& d = projs[p];
         L.pr_i[2 * p] = d.l;
         L.pr_i[2 * p + 1] = d.coef_rel;
-        const double ylm = d.l == 0 ? 0.28209479177387814 : 0.48860251190291992;   // 1/sqrt(4pi), sqrt(3/4pi)
-        L.pr_f[2 * p] = (T)(d.amp * ylm);
-        L.pr_f[2 * p + 1] = (sizeof(T) == 4) ? (T)(d.inv_2s2 * 1.4426950408889634) : (T)d.inv_2s2;
+        proj_consts<T>(d, L.pr_f[2 * p], L.pr_f[2 * p + 1]);
+    }
+}
+
+// s + p projector values at 4 consecutive points (x offsets u = 0..3 from gx) of one image
+// row, with the image's constants q (im_q row) and centre n: es (s channel, Y00 folded in) and
+// ep (p channel over r, sqrt(3/4pi) folded in; beta_1m = ep * (dy, dz, dx)).  Points outside
+// [u0, u0 + np) get zeros.  Branch-free, so the four points' exponentials and square roots
+// overlap.  PF 2: one exponent for both channels.
+template <class T, int PF>
+__device__ __forceinline__ void sp_values4(const T (&q)[8], int nx0, int gx, int u0, int np, T hx, T ryz, T (&dx)[4],
+                                           T (&es)[4], T (&ep)[4]) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        dx[u] = hx * ((T)(gx + u - nx0) - q[0]);
+        const T rr = dx[u] * dx[u] + ryz;
+        const T ir = (T(1.0e-14) > rr) ? T(0) : inv_sqrt<T>(rr);
+        T s, p;
+        if constexpr (PF == 2) {
+            const T e = gauss<T>(rr, q[4]);
+            s = q[3] * e;
+            p = q[5] * e * ir;
+        } else {
+            s = q[3] * gauss<T>(rr, q[4]);
+            p = q[5] * gauss<T>(rr, q[6]) * ir;
+        }
+        const bool on = (unsigned)(u - u0) < (unsigned)np;
+        es[u] = on ? s : T(0);
+        ep[u] = on ? p : T(0);
+    }
+}
+
+template <class T>
+__device__ __forceinline__ void load_q(const ProjLds<T>& L, int li, T (&q)[8]) {
+    const Row<T>* src = reinterpret_cast<const Row<T>*>(L.im_q + 8 * li);
+    constexpr int VN = 16 / sizeof(T);
+#pragma unroll
+    for (int k = 0; k < 8 / VN; ++k) {
+        const Row<T> w = src[k];
+#pragma unroll
+        for (int u = 0; u < VN; ++u) q[k * VN + u] = w.v[u];
     }
 }
 
@@ -273,15 +360,7 @@ __global__ __launch_bounds__(kProjThreads) void rows_project_kernel(RowsProjArgs
     off += sizeof(uint32_t) * kProjThreads * kItemsPT;
     int* itoffl = reinterpret_cast<int*>(smem + off);
     off = align16(off + sizeof(int) * (a.tz + 1));
-    unsigned char* lp = smem + off;
-    ProjLds<T> L;
-    L.im_f = reinterpret_cast<T*>(lp);
-    lp += sizeof(T) * 3 * a.max_col;
-    L.pr_f = reinterpret_cast<T*>(lp);
-    lp += sizeof(T) * 2 * a.nproj;
-    L.im_i = reinterpret_cast<int*>(lp);
-    lp += sizeof(int) * 5 * a.max_col;
-    L.pr_i = reinterpret_cast<int*>(lp);
+    const ProjLds<T> L = carve_proj<T>(smem + off, a.max_col, a.nproj);
 
     const int c0 = a.col_off[tile];
     const int ncol = a.col_off[tile + 1] - c0;
@@ -344,6 +423,15 @@ __global__ __launch_bounds__(kProjThreads) void rows_project_kernel(RowsProjArgs
 
     const T hx = (T)a.h[0], hy = (T)a.h[1], hz = (T)a.h[2];
     __syncthreads();   // itoffl and the projector staging
+    // s + p forms: this thread's image (centre and projector constants) in registers for the slab
+    T iq[8];
+    int in0 = 0, in1 = 0, in2 = 0;
+    if constexpr (PF != 0) {
+        load_q<T>(L, act ? li : 0, iq);
+        in0 = L.im_n[4 * (act ? li : 0)];
+        in1 = L.im_n[4 * (act ? li : 0) + 1];
+        in2 = L.im_n[4 * (act ? li : 0) + 2];
+    }
     load(z0);
     for (int z = z0; z < z1; ++z) {
         __syncthreads();   // the previous plane's readers are done
@@ -364,7 +452,32 @@ __global__ __launch_bounds__(kProjThreads) void rows_project_kernel(RowsProjArgs
         const int cs = nxt_s, ce = nxt_e;
         if (z + 1 < z1) load(z + 1);
         __syncthreads();
-        if (act) {
+        if (PF != 0 && act) {
+            const T dz = hz * ((T)(z - in2) - iq[2]);
+            const T dzz = dz * dz;
+            for (int i = cs + rk; i < ce; i += cnt) {
+                const uint32_t it = itl[i];
+                const int r = (int)(it & 31u), xs = (int)((it >> 5) & 1023u), np = (int)((it >> 15) & 3u) + 1;
+                const T dy = hy * ((T)(y0 + r - in1) - iq[1]);
+                const T* prow = pl + (size_t)(r * nx + xs) * NBP;
+                T dx[4], es[4], ep[4];
+                sp_values4<T, PF>(iq, in0, xs, 0, np, hx, dy * dy + dzz, dx, es, ep);
+#pragma unroll
+                for (int p = 0; p < 4; ++p) {
+                    const T beta[4] = {es[p], ep[p] * dy, ep[p] * dz, ep[p] * dx[p]};
+                    // points past the item read its first point (in the row) with zero weights
+                    const Row<T>* pv = reinterpret_cast<const Row<T>*>(prow + (p < np ? p : 0) * NBP);
+#pragma unroll
+                    for (int q = 0; q < NBP / VN; ++q) {
+                        const Row<T> w = pv[q];
+#pragma unroll
+                        for (int u = 0; u < VN; ++u)
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) acc[j][q * VN + u] += beta[j] * w.v[u];
+                    }
+                }
+            }
+        } else if (act) {
             const int* ii = L.im_i + 5 * li;
             const T* ff = L.im_f + 3 * li;
             const T dz = hz * ((T)(z - ii[2]) - ff[2]);
@@ -463,15 +576,7 @@ __global__ __launch_bounds__(NBW >= 4 ? 1024 : 512) void rows_hpsi_kernel(RowsHp
     off += sizeof(int) * (size_t)(ngo + 1);
     int* itoffl = reinterpret_cast<int*>(smem + off);          // [tz + 1]
     off = align16(off + sizeof(int) * (a.tz + 1));
-    unsigned char* lp = smem + off;
-    ProjLds<T> L;
-    L.im_f = reinterpret_cast<T*>(lp);
-    lp += sizeof(T) * 3 * a.max_col;
-    L.pr_f = reinterpret_cast<T*>(lp);
-    lp += sizeof(T) * 2 * a.nproj;
-    L.im_i = reinterpret_cast<int*>(lp);
-    lp += sizeof(int) * 5 * a.max_col;
-    L.pr_i = reinterpret_cast<int*>(lp);
+    const ProjLds<T> L = carve_proj<T>(smem + off, a.max_col, a.nproj);
 
     const int wg = xcd_order(blockIdx.x, gridDim.x);
     const int tile = wg % a.ntile;
@@ -618,6 +723,47 @@ __global__ __launch_bounds__(NBW >= 4 ? 1024 : 512) void rows_hpsi_kernel(RowsHp
             // each item (points of one image in one 4-point group) is evaluated once for the
             // workgroup's bands and its values are written to the item's own slots (no two
             // threads write one slot, no atomics)
+            if constexpr (PF != 0) {
+                for (int i = tid; i < nit; i += nthr) {
+                    const uint32_t it = itl[i];
+                    const int ir = (int)(it & 31u), xs = (int)((it >> 5) & 1023u);
+                    const int np = (int)((it >> 15) & 3u) + 1, li = (int)(it >> 17);
+                    // the item's image constants and the E*c of its atom: independent LDS reads
+                    const int4 cn = *reinterpret_cast<const int4*>(L.im_n + 4 * li);
+                    T cq[8];
+                    load_q<T>(L, li, cq);
+                    T e[NBW][4];
+#pragma unroll
+                    for (int b = 0; b < NBW; ++b) {
+                        const Row<T>* ep = reinterpret_cast<const Row<T>*>(ec + ((size_t)li * NBW + b) * NJS);
+#pragma unroll
+                        for (int q4 = 0; q4 < 4 / VN; ++q4) {
+                            const Row<T> w = ep[q4];
+#pragma unroll
+                            for (int u = 0; u < VN; ++u) e[b][q4 * VN + u] = w.v[u];
+                        }
+                    }
+                    const T dy = hy * ((T)(y0 + ir - cn.y) - cq[1]);
+                    const T dz = hz * ((T)(z - cn.z) - cq[2]);
+                    T dx[4], es[4], ep[4];
+                    sp_values4<T, PF>(cq, cn.x, xs & ~3, xs & 3, np, hx, dy * dy + dz * dz, dx, es, ep);
+#pragma unroll
+                    for (int b = 0; b < NBW; ++b) {
+                        T val[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                            val[u] = e[b][0] * es[u] + ep[u] * (e[b][1] * dy + e[b][2] * dz + e[b][3] * dx[u]);
+                        Row<T>* dst = reinterpret_cast<Row<T>*>(stg + ((size_t)i * NBW + b) * 4);
+#pragma unroll
+                        for (int h = 0; h < 4 / VN; ++h) {
+                            Row<T> w;
+#pragma unroll
+                            for (int u = 0; u < VN; ++u) w.v[u] = val[h * VN + u];
+                            dst[h] = w;
+                        }
+                    }
+                }
+            } else
             for (int i = tid; i < nit; i += nthr) {
                 const uint32_t it = itl[i];
                 const int ir = (int)(it & 31u), xs = (int)((it >> 5) & 1023u);
@@ -900,7 +1046,7 @@ static size_t rows_proj_lds(const RowsGeo& rg, const qg_grid& g, int nbp, int np
     size_t off = align16(sizeof(T) * std::max(pl, red));
     off += sizeof(uint32_t) * kProjThreads * kItemsPT;
     off = align16(off + sizeof(int) * (rg.tzp + 1));
-    return off + sizeof(T) * (3 * rg.max_col + 2 * nproj) + sizeof(int) * (5 * rg.max_col + 2 * nproj);
+    return off + proj_lds_bytes<T>(std::max(1, rg.max_col), nproj);
 }
 
 // LDS bytes of an apply workgroup of nthr threads (layout of rows_hpsi_kernel); stg_cap items
@@ -913,8 +1059,7 @@ static size_t rows_hpsi_lds(const qg_grid& g, int ty, int tz, int nbw, int njs, 
     off += sizeof(uint32_t) * (size_t)nthr * kItemsPT;
     off += sizeof(int) * (size_t)(ty * ng4 + 1);
     off = align16(off + sizeof(int) * (tz + 1));
-    return off + sizeof(T) * (3 * (size_t)max_col + 2 * (size_t)nproj) +
-           sizeof(int) * (5 * (size_t)max_col + 2 * (size_t)nproj);
+    return off + proj_lds_bytes<T>(max_col, nproj);
 }
 
 // Scratch of one application: projector partials and coefficients (grown, never shrunk).
