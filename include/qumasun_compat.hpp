@@ -24,8 +24,10 @@
 // GridRangeMPI both qualify); one process holds the whole periodic grid.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -157,7 +159,7 @@ void Laplasian8thHost(const G& g, T* out, const T* p, double coef, double dx, do
 
 // Nonlocal KB projectors for one process holding the whole grid (PseudoPotIntegrator_mpi).
 // Load() takes the projector channels of an element directly (qg_proj_spec, Gaussian or log-grid
-// table radial part); LoadVps() reads them from an MBK .vps file (qg_vps_*).
+// table radial part); Load(Z, filepath) reads them from an MBK .vps file (qg_vps_*).
 class PseudoPotIntegratorGPU {
 public:
     ~PseudoPotIntegratorGPU() {
@@ -175,6 +177,38 @@ public:
         for (auto& e : elements_)
             if (e.first == Z) { e.second = projectors; return; }
         elements_.emplace_back(Z, projectors);
+    }
+    // Load(Z, filepath, comm) (PseudoPotOperator_vlocal.h:80-91): the element's projector
+    // channels from an MBK .vps file; a second Load of the same Z keeps the first, as the
+    // reference's m_pp_list.emplace does.  The communicator is not used (one process).
+    void Load(int Z, const char* filepath) {
+        if (nl_) throw std::runtime_error("Load before the first UpdatePosition");
+        for (auto& e : elements_)
+            if (e.first == Z) return;
+        qg_vps* v = nullptr;
+        check(qg_vps_load(filepath, &v), "qg_vps_load");
+        qg_vps_info info{};
+        std::vector<qg_proj_spec> specs;
+        int rc = qg_vps_get_info(v, &info);
+        if (rc == QG_OK) {
+            specs.resize(std::max(1, info.num_projectors));
+            rc = qg_vps_projector_specs(v, specs.data(), (int)specs.size());
+            specs.resize(info.num_projectors);
+        }
+        if (rc == QG_OK) {
+            // the tables live in v: keep copies owned here
+            for (auto& s : specs) {
+                tables_.emplace_back(s.table, s.table + s.n_table);
+                s.table = tables_.back().data();
+            }
+        }
+        qg_vps_destroy(v);
+        check(rc, "qg_vps_projector_specs");
+        elements_.emplace_back(Z, specs);
+    }
+    template <class Comm>
+    void Load(int Z, const char* filepath, Comm&) {
+        Load(Z, filepath);
     }
     // UpdatePosition(nuclei, num_nuclei, l_grid) (PseudoPotOperator_mpi_v2.h:231-248): the first
     // call builds the projector geometry, later calls move the nuclei of the same handle (every
@@ -245,6 +279,7 @@ private:
     double d_[3] = {0, 0, 0};
     std::unique_ptr<Grid> grid_;
     std::vector<std::pair<int, std::vector<qg_proj_spec>>> elements_;
+    std::deque<std::vector<double>> tables_;   // radial tables of the .vps channels
     qg_nonlocal* nl_ = nullptr;
 };
 

@@ -79,6 +79,43 @@ typedef struct qg_proj_spec {
     const double* table; /* host pointer, n_table values */
 } qg_proj_spec;
 
+/* ----------------------------------------------------- MBK pseudopotential files ---- */
+/* An MBK .vps file read as LoadVPS reads it (vps_loader.h:87-247, file model of
+ * JobParam.h:246-384): pseudo.NandL cutoffs per l, project.energies, the Pseudo.Potentials
+ * table (xi, r, V_local and two j-components per projector), the optional density.PCC
+ * block, valence.electron and local.cutoff; PseudoPot_MBK::SortByL applied as the reference
+ * applies it.  Errors (no file, vps.type other than MBK, missing or inconsistent blocks) return
+ * QG_ERR_ARG with the message in qg_last_error(). */
+typedef struct qg_vps qg_vps;
+typedef struct qg_vps_info {
+    int valence_electron;      /* (int) valence.electron */
+    int num_projectors;
+    int num_radial_grids;
+    int has_pcc_charge;
+    int total_projector_lm;    /* PseudoPot_MBK::TotalProjectorLM */
+    int max_projector_l;       /* PseudoPot_MBK::MaxProjectorL */
+    double cutoff_vlocal;      /* local.cutoff */
+    double cutoff_r[4];        /* largest pseudo.NandL cutoff per l */
+    double xi_min, xi_delta;   /* log-grid of the tables */
+    double pcc_charge_integral;/* 4 pi xi_delta sum pcc r^3 (the figure LoadVPS prints) */
+} qg_vps_info;
+int qg_vps_load(const char* path, qg_vps** out);
+int qg_vps_destroy(qg_vps* v);
+int qg_vps_get_info(const qg_vps* v, qg_vps_info* info);
+/* radial grid r, V_local and the PCC charge (zeros without a density.PCC block); each
+ * num_radial_grids values, any pointer may be NULL */
+int qg_vps_get_radial(const qg_vps* v, double* radius, double* v_local, double* pcc_charge);
+/* projector k: l, the two energies and the two j-components (num_radial_grids values each;
+ * any pointer may be NULL) */
+int qg_vps_get_projector(const qg_vps* v, int k, int* l, double* energy_up, double* energy_down,
+                         double* up_values, double* down_values);
+/* The projector channels of the file for qg_nonlocal_create, as the reference builds them
+ * (PseudoPotOperator_mpi_v2.h:64-84, :108-166): l, E = energy_up, cutoff = cutoff_r[l], radial
+ * kind QG_RADIAL_TABLE over the file's log grid with r_min = r[0] and the mean of the two
+ * j-components.  specs[k].table points into v (valid until qg_vps_destroy).  cap >=
+ * num_projectors. */
+int qg_vps_projector_specs(const qg_vps* v, qg_proj_spec* specs, int cap);
+
 typedef struct qg_nonlocal qg_nonlocal;
 
 /* Build the projector geometry for natoms nuclei at xyz_host[3*a..3*a+2] (bohr), each of

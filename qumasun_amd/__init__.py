@@ -10,9 +10,11 @@ Reference interfaces mirrored (QUMASUN source/):
   PseudoPotIntegrator.UpdatePosition / ProjectionPP PseudoPotOperator_mpi_v2.h:231, :465
   Hamiltonian.apply                                qumasun_mpi.matrix.h:21-43
   Eigen_d_multi(...)                               lobpcg_d_multi_mpi.h:882-1572
+  LoadVPS(path) -> PseudoPot_MBK                   vps_loader.h:87-247
 """
 import ctypes
 import math
+import os
 
 import numpy as np
 
@@ -21,7 +23,8 @@ from ._native import (QG_F32, QG_F64, QG_PREC_F32, QG_PREC_F64, QG_PREC_MIXED, Q
                       QG_PRECOND_LEARNED, QG_PRECOND_NONE, QGError, check, dptr, lib)
 
 __all__ = ["Grid", "Laplasian8th", "PseudoPotIntegrator", "Hamiltonian", "Eigen_d_multi",
-           "LearnedPreconditioner", "gaussian_projector", "diamond_system", "QGError"]
+           "LearnedPreconditioner", "gaussian_projector", "diamond_system", "QGError", "LoadVPS",
+           "PseudoPot_MBK"]
 
 
 def _dtype_code(t):
@@ -101,6 +104,77 @@ def table_projector(l, energy, cutoff, xi_min, xi_delta, values, r_min=None):
         r_min = math.exp(xi_min)
     return dict(l=l, energy=energy, cutoff=cutoff, kind="table", xi_min=xi_min,
                 xi_delta=xi_delta, values=values, r_min=r_min)
+
+
+class PseudoPot_MBK:
+    """An MBK .vps pseudopotential as LoadVPS reads it (vps_loader.h:9-247): the fields of
+    PseudoPot_MBK (valence_electron, num_projectors, num_radial_grids, cutoff_vlocal, cutoff_r,
+    projector_quantum_l, projector_energy_up / _down, radius, V_local, projector (two
+    j-components per projector, in the reference's order), pcc_charge, has_pcc_charge, xi_min,
+    xi_delta), read by the engine's C++ reader (qg_vps_load)."""
+
+    def __init__(self, path):
+        h = ctypes.c_void_p()
+        check(lib().qg_vps_load(os.fsencode(path), ctypes.byref(h)))
+        try:
+            info = N.VpsInfo()
+            check(lib().qg_vps_get_info(h, ctypes.byref(info)))
+            n, npj = info.num_radial_grids, info.num_projectors
+            self.valence_electron = info.valence_electron
+            self.num_projectors = npj
+            self.num_radial_grids = n
+            self.cutoff_vlocal = info.cutoff_vlocal
+            self.cutoff_r = list(info.cutoff_r)
+            self.xi_min, self.xi_delta = info.xi_min, info.xi_delta
+            self.has_pcc_charge = info.has_pcc_charge
+            self.pcc_charge_integral = info.pcc_charge_integral
+            self._total_lm, self._max_l = info.total_projector_lm, info.max_projector_l
+            self.radius = np.zeros(n)
+            self.V_local = np.zeros(n)
+            pcc = np.zeros(n)
+            check(lib().qg_vps_get_radial(h, dptr(self.radius), dptr(self.V_local), dptr(pcc)))
+            self.pcc_charge = pcc if self.has_pcc_charge else None
+            self.projector_quantum_l = []
+            self.projector_energy_up = []
+            self.projector_energy_down = []
+            self.projector = []
+            for k in range(npj):
+                l, eu, ed = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+                up, dn = np.zeros(n), np.zeros(n)
+                check(lib().qg_vps_get_projector(h, k, ctypes.byref(l), ctypes.byref(eu), ctypes.byref(ed),
+                                                 dptr(up), dptr(dn)))
+                self.projector_quantum_l.append(l.value)
+                self.projector_energy_up.append(eu.value)
+                self.projector_energy_down.append(ed.value)
+                self.projector += [up, dn]
+            specs = (N.ProjSpec * max(1, npj))()
+            check(lib().qg_vps_projector_specs(h, specs, max(1, npj)))
+            self._specs = []
+            for k in range(npj):
+                s = specs[k]
+                vals = np.ctypeslib.as_array(s.table, shape=(s.n_table,)).copy()
+                self._specs.append(table_projector(s.l, s.energy, s.cutoff, s.xi_min, s.xi_delta, vals,
+                                                   r_min=s.r_min))
+        finally:
+            lib().qg_vps_destroy(h)
+
+    def TotalProjectorLM(self):
+        return self._total_lm
+
+    def MaxProjectorL(self):
+        return self._max_l
+
+    def projector_specs(self):
+        """The projector channels for PseudoPotIntegrator, as the reference builds them from the
+        file (PseudoPotOperator_mpi_v2.h:64-84, :108-166): l, energy_up, cutoff_r[l], the mean of
+        the two j-components on the file's log grid, r below radius[0] taking the first value."""
+        return [dict(p) for p in self._specs]
+
+
+def LoadVPS(path):
+    """LoadVPS(filepath) (vps_loader.h:87): the file's PseudoPot_MBK; raises QGError where the
+    reference prints an error (not MBK, missing blocks) or would read out of bounds."""
+    return PseudoPot_MBK(path)
 
 
 def _specs_array(species_projectors):

@@ -41,6 +41,14 @@ class ProjSpec(ctypes.Structure):
                 ("table", P(c_double))]
 
 
+class VpsInfo(ctypes.Structure):
+    """qg_vps_info"""
+    _fields_ = [("valence_electron", c_int), ("num_projectors", c_int), ("num_radial_grids", c_int),
+                ("has_pcc_charge", c_int), ("total_projector_lm", c_int), ("max_projector_l", c_int),
+                ("cutoff_vlocal", c_double), ("cutoff_r", c_double * 4), ("xi_min", c_double),
+                ("xi_delta", c_double), ("pcc_charge_integral", c_double)]
+
+
 # void (*op)(void* user, int dtype, int nbands, const void* x, void* ax, void* stream)
 OP_CALLBACK = ctypes.CFUNCTYPE(None, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p)
 # void (*watch)(void* user, int sector, int64_t count)
@@ -83,6 +91,13 @@ SIGNATURES = {
     "qg_nonlocal_create": (c_int, [c_void_p, c_int, P(c_double), P(c_int), c_int, P(c_int),
                                    P(ProjSpec), P(c_void_p)]),
     "qg_nonlocal_update": (c_int, [c_void_p, c_int, P(c_double), P(c_int)]),
+    "qg_vps_load": (c_int, [ctypes.c_char_p, P(c_void_p)]),
+    "qg_vps_destroy": (c_int, [c_void_p]),
+    "qg_vps_get_info": (c_int, [c_void_p, P(VpsInfo)]),
+    "qg_vps_get_radial": (c_int, [c_void_p, P(c_double), P(c_double), P(c_double)]),
+    "qg_vps_get_projector": (c_int, [c_void_p, c_int, P(c_int), P(c_double), P(c_double), P(c_double),
+                                     P(c_double)]),
+    "qg_vps_projector_specs": (c_int, [c_void_p, P(ProjSpec), c_int]),
     "qg_nonlocal_destroy": (c_int, [c_void_p]),
     "qg_nonlocal_num_coef": (c_int, [c_void_p]),
     "qg_nonlocal_projector_form": (c_int, [c_void_p]),

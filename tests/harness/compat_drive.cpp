@@ -38,7 +38,7 @@ struct Watch {
 
 int main(int argc, char** argv) {
     if (argc < 2) {
-        std::fprintf(stderr, "usage: compat_drive OUT\n");
+        std::fprintf(stderr, "usage: compat_drive OUT [VPS]\n");
         return 2;
     }
     const int n = 24;
@@ -102,6 +102,18 @@ int main(int argc, char** argv) {
     Watch w3;
     EigenLOBPCG_d_multi((int)ng, dvol, nb, 40, ev3.data(), X3.data(), K3.data(), work.data(), true, &H, w3);
     put(ev3.data(), nb);
+
+    if (argc > 2) {
+        // projector channels from an MBK .vps file: Load(Z, filepath, comm), then ProjectionPP
+        int comm = 0;
+        PseudoPotIntegratorGPU pv;
+        pv.InitializeGrid(g, h, h, h);
+        pv.Load(6, argv[2], comm);
+        pv.UpdatePosition(nuc, 8, g);
+        std::fill(out.begin(), out.end(), 0.0);
+        pv.ProjectionPP(out.data(), X0.data(), g, nuc, 8);
+        put(out.data(), ng);
+    }
     std::fclose(f);
     std::printf("watch");
     for (int k = 10; k < 20; ++k) std::printf(" %d:%lld", k, (long long)watch.cnt[k]);

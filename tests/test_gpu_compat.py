@@ -4,7 +4,8 @@ l_grid, nuclei, n), UpdatePosition, the Hamiltonian, LOBPCG::Eigen_d_multi_mpi(l
 N, iter_max, ev, X, keep_P, work, keep_S_matrix, SCF_step, OpeA, watch) with the engine's
 Hamiltonian and with a per-band user operator, and EigenLOBPCG_d_multi, all on host arrays.
 Every result is compared with the FP64 oracle on the same seeded inputs; the watch must have
-been driven with the reference's sector ids."""
+been driven with the reference's sector ids.  A second integrator takes its channels from an
+MBK .vps file through Load(Z, filepath, comm)."""
 import os
 import re
 import subprocess
@@ -28,7 +29,12 @@ def test_compat_surface_against_oracle(gpu, oracle, tmp_path):
     print("compat driver", flush=True)
     exe = B.compat_driver()
     out = tmp_path / "compat.bin"
-    r = subprocess.run([exe, str(out)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=240)
+    import vps_synth as VS
+    vd = VS.synth_pp(n=160)
+    vps = tmp_path / "c.vps"
+    VS.write_vps(str(vps), vd)
+    r = subprocess.run([exe, str(out), str(vps)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=240)
     assert r.returncode == 0, r.stdout[-3000:]
     s = Q.diamond_system(1, 24)
     n = (s["n"],) * 3
@@ -36,10 +42,10 @@ def test_compat_surface_against_oracle(gpu, oracle, tmp_path):
     ng = n[0] ** 3
     nb = 8
     d = np.fromfile(str(out), dtype=np.float64)
-    sizes = [ng, ng, 4 * ng, nb, nb * ng, nb, ng, nb]
+    sizes = [ng, ng, 4 * ng, nb, nb * ng, nb, ng, nb, ng]
     assert d.size == sum(sizes), (d.size, sum(sizes))
     parts = np.split(d, np.cumsum(sizes)[:-1])
-    lap, ppv, hp, ev, X, ev_user, hp_moved, ev_single = parts
+    lap, ppv, hp, ev, X, ev_user, hp_moved, ev_single, ppv_vps = parts
     X0 = Q.random_states(ng, nb, h[0] ** 3)
     nat = len(s["positions"])
     onl = oracle.OracleNL(n, h, s["positions"], np.zeros(nat, np.int32), s["projectors"])
@@ -67,6 +73,17 @@ def test_compat_surface_against_oracle(gpu, oracle, tmp_path):
     Xs, Ps = X0.copy(), np.zeros_like(X0)
     es, ss = oracle.lobpcg_hamiltonian(n, h, s["V"], onl, Xs, Ps, 40, tol=np.sqrt(1e-10 * h[0] ** 3))
     assert np.abs(ev_single - es).max() < 1e-8, np.abs(ev_single - es).max()
+    # Load(Z, filepath, comm) from the .vps file: the channels the reference builds from the
+    # arrays written (energy_up, NandL cutoff per l, mean of the two j-components)
+    xi = vd["xi"]
+    dxi = (xi[-1] - xi[0]) / (len(xi) - 1)
+    cut = {l: c for (_, _, l, c) in vd["nandl"]}
+    vch = [Q.table_projector(l, vd["e_up"][k], cut[l], xi[0], dxi, (vd["comps"][2 * k] + vd["comps"][2 * k + 1]) / 2.0,
+                             r_min=vd["r"][0]) for k, l in enumerate(vd["l"])]
+    onl_v = oracle.OracleNL(n, h, s["positions"], np.zeros(nat, np.int32), [vch])
+    ref_v = np.zeros(ng)
+    onl_v.apply(X0[0].copy(), ref_v)
+    assert rel(ppv_vps, ref_v) <= 1e-12
     for tag in ("watch", "watch_user"):
         m = re.search(rf"^{tag} (.*)$", r.stdout, re.M)
         assert m, r.stdout

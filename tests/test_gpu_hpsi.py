@@ -379,6 +379,53 @@ def test_rows_general_projector_sets(gpu, oracle, prec, nb):
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_vps_projectors_parity(gpu, oracle, prec, tmp_path):
+    """H psi with the projector channels of an MBK .vps file (LoadVPS, qg_vps_projector_specs,
+    PseudoPotIntegrator) against the oracle given the channels the reference builds from the
+    arrays that were written (PseudoPotOperator_mpi_v2.h:64-84, :108-166: energy_up, the NandL
+    cutoff of each l, the mean of the two j-components, r below r[0] taking the first value):
+    the 8-atom cell at 40^3 with s and p channels of different cutoffs."""
+    import torch
+    import vps_synth as S
+    print(f"vps projectors prec={prec}", flush=True)
+    d = S.synth_pp(n=160, channels=((0, 1.0, 0.9, 0.6, 1.6), (1, -0.5, -0.45, 0.55, 1.8)), scale=3.0)
+    path = tmp_path / "c.vps"
+    S.write_vps(str(path), d)
+    pp = Q.LoadVPS(str(path))
+    xi = d["xi"]
+    dxi = (xi[-1] - xi[0]) / (len(xi) - 1)
+    cut = {l: c for (_, _, l, c) in d["nandl"]}
+    want = [[Q.table_projector(l, d["e_up"][k], cut[l], xi[0], dxi, (d["comps"][2 * k] + d["comps"][2 * k + 1]) / 2.0,
+                               r_min=d["r"][0]) for k, l in enumerate(d["l"])]]
+    s = Q.diamond_system(1, 40)
+    n3 = (s["n"],) * 3
+    h3 = (s["h"],) * 3
+    ng = s["n"] ** 3
+    nat = len(s["positions"])
+    onl = oracle.OracleNL(n3, h3, s["positions"], np.zeros(nat, np.int32), want)
+    X = Q.random_states(ng, 8, s["h"] ** 3, seed=91)
+    ref = oracle.hamiltonian_block(n3, h3, s["V"], onl, X)
+    g = Q.Grid(*n3, *h3)
+    integ = Q.PseudoPotIntegrator(g, [pp.projector_specs()])
+    integ.UpdatePosition(s["positions"])
+    H = Q.Hamiltonian(g, s["V"], integ)
+    dt = torch.float64 if prec == "f64" else torch.float32
+    xt = _t(X, dt, gpu)
+    out = torch.empty_like(xt)
+    H.apply(xt, out)
+    torch.cuda.synchronize()
+    assert rel(_np(out), ref) <= (TOL64 if prec == "f64" else TOL32)
+    # the projector part alone: its own relative error plus the rounding of the whole result
+    # (FP32 rounds H psi, of which this part is a fraction)
+    loc = oracle.hamiltonian_block(n3, h3, s["V"], None, X)
+    nl_err = np.linalg.norm((_np(out) - loc) - (ref - loc))
+    if prec == "f64":
+        assert nl_err <= 1e-11 * np.linalg.norm(ref - loc)
+    else:
+        assert nl_err <= 2e-5 * np.linalg.norm(ref - loc) + TOL32 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_hpsi_plans(gpu, oracle, prec):
     """Both kernel families and explicit row-kernel shapes (through qg_hamiltonian_set_plan)
     give the oracle's H psi; asking for the row kernels where they do not serve is refused."""
