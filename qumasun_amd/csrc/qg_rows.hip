@@ -121,7 +121,6 @@ struct RowsHpsiArgs {
     int nitems;                 // total items (item loads are clamped to the array)
     const int* grp_off;         // per (tile, plane): ty * ng4 + 1 run offsets of the 4-point groups
     int ng4;                    // 4-point groups per row
-    int stg_cap;                // item capacity of the per-item value area (the most items of a plane)
     const DevImage* images;
     const DevProj* projs;
     int nproj;
@@ -168,21 +167,29 @@ struct ProjLds {
     T* pr_f;       // [nproj][2]: amp * (Y00 or sqrt(3/4pi)), exponent factor
 };
 
-// LDS bytes of the projector staging (ProjLds) for max_col images and nproj projectors
+// LDS bytes of the projector staging (ProjLds) for max_col images and nproj projectors; the
+// general arrays (im_i, im_f, pr_*) only with gen (the general projector form)
 template <class T>
-__host__ __device__ constexpr size_t proj_lds_bytes(size_t max_col, size_t nproj) {
-    return (8 * sizeof(T) + 16) * max_col + sizeof(T) * (3 * max_col + 2 * nproj) +
-           sizeof(int) * (5 * max_col + 2 * nproj);
+__host__ __device__ constexpr size_t proj_lds_bytes(size_t max_col, size_t nproj, bool gen) {
+    return (8 * sizeof(T) + 16) * max_col +
+           (gen ? sizeof(T) * (3 * max_col + 2 * nproj) + sizeof(int) * (5 * max_col + 2 * nproj) : 0);
 }
 
-// carve the projector staging out of LDS at lp (16-byte aligned)
-template <class T>
+// carve the projector staging out of LDS at lp (16-byte aligned); GEN: with the general arrays
+template <class T, bool GEN>
 __device__ __forceinline__ ProjLds<T> carve_proj(unsigned char* lp, int max_col, int nproj) {
     ProjLds<T> L;
     L.im_q = reinterpret_cast<T*>(lp);
     lp += sizeof(T) * 8 * max_col;
     L.im_n = reinterpret_cast<int*>(lp);
     lp += sizeof(int) * 4 * max_col;
+    if constexpr (!GEN) {
+        L.im_f = nullptr;
+        L.pr_f = nullptr;
+        L.im_i = nullptr;
+        L.pr_i = nullptr;
+        return L;
+    }
     L.im_f = reinterpret_cast<T*>(lp);
     lp += sizeof(T) * 3 * max_col;
     L.pr_f = reinterpret_cast<T*>(lp);
@@ -202,7 +209,7 @@ __device__ __forceinline__ void proj_consts(const DevProj& d, T& amp, T& k) {
     k = (sizeof(T) == 4) ? (T)(d.inv_2s2 * 1.4426950408889634) : (T)d.inv_2s2;
 }
 
-template <class T>
+template <class T, bool GEN>
 __device__ __forceinline__ void stage_proj(const ProjLds<T>& L, const DevImage* images, const DevProj* projs,
                                            int nproj, const int* col_img, int ncol, const double* h, int tid,
                                            int nthr) {
@@ -212,13 +219,17 @@ __device__ __forceinline__ void stage_proj(const ProjLds<T>& L, const DevImage* 
         for (int d = 0; d < 3; ++d) {
             const double u = im.R[d] / h[d];
             const double nn = rint(u);
-            L.im_f[3 * li + d] = (T)(u - nn);
             L.im_q[8 * li + d] = (T)(u - nn);
-            L.im_i[5 * li + d] = im.shift[d] + (int)nn;
             L.im_n[4 * li + d] = im.shift[d] + (int)nn;
+            if constexpr (GEN) {
+                L.im_f[3 * li + d] = (T)(u - nn);
+                L.im_i[5 * li + d] = im.shift[d] + (int)nn;
+            }
         }
-        L.im_i[5 * li + 3] = im.proj_begin;
-        L.im_i[5 * li + 4] = im.proj_end;
+        if constexpr (GEN) {
+            L.im_i[5 * li + 3] = im.proj_begin;
+            L.im_i[5 * li + 4] = im.proj_end;
+        }
         L.im_n[4 * li + 3] = im.proj_begin;
         T as = T(0), ks = T(0), ap = T(0), kp = T(0);
         if (im.proj_end > im.proj_begin) proj_consts<T>(projs[im.proj_begin], as, ks);
@@ -229,11 +240,13 @@ __device__ __forceinline__ void stage_proj(const ProjLds<T>& L, const DevImage* 
         L.im_q[8 * li + 6] = kp;
         L.im_q[8 * li + 7] = T(0);
     }
-    for (int p = tid; p < nproj; p += nthr) {
-        const DevProj& d = projs[p];
-        L.pr_i[2 * p] = d.l;
-        L.pr_i[2 * p + 1] = d.coef_rel;
-        proj_consts<T>(d, L.pr_f[2 * p], L.pr_f[2 * p + 1]);
+    if constexpr (GEN) {
+        for (int p = tid; p < nproj; p += nthr) {
+            const DevProj& d = projs[p];
+            L.pr_i[2 * p] = d.l;
+            L.pr_i[2 * p + 1] = d.coef_rel;
+            proj_consts<T>(d, L.pr_f[2 * p], L.pr_f[2 * p + 1]);
+        }
     }
 }
 
@@ -245,9 +258,10 @@ __device__ __forceinline__ void stage_proj(const ProjLds<T>& L, const DevImage* 
 template <class T, int PF>
 __device__ __forceinline__ void sp_values4(const T (&q)[8], int nx0, int gx, int u0, int np, T hx, T ryz, T (&dx)[4],
                                            T (&es)[4], T (&ep)[4]) {
+    const T dx0 = hx * ((T)(gx - nx0) - q[0]);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-        dx[u] = hx * ((T)(gx + u - nx0) - q[0]);
+        dx[u] = dx0 + (T)u * hx;
         const T rr = dx[u] * dx[u] + ryz;
         const T ir = (T(1.0e-14) > rr) ? T(0) : inv_sqrt<T>(rr);
         T s, p;
@@ -360,11 +374,11 @@ __global__ __launch_bounds__(kProjThreads) void rows_project_kernel(RowsProjArgs
     off += sizeof(uint32_t) * kProjThreads * kItemsPT;
     int* itoffl = reinterpret_cast<int*>(smem + off);
     off = align16(off + sizeof(int) * (a.tz + 1));
-    const ProjLds<T> L = carve_proj<T>(smem + off, a.max_col, a.nproj);
+    const ProjLds<T> L = carve_proj<T, PF == 0>(smem + off, a.max_col, a.nproj);
 
     const int c0 = a.col_off[tile];
     const int ncol = a.col_off[tile + 1] - c0;
-    stage_proj<T>(L, a.images, a.projs, a.nproj, a.col_img + c0, ncol, a.h, tid, kProjThreads);
+    stage_proj<T, PF == 0>(L, a.images, a.projs, a.nproj, a.col_img + c0, ncol, a.h, tid, kProjThreads);
     for (int i = tid; i <= z1 - z0; i += kProjThreads) itoffl[i] = a.it_off[(size_t)tile * nz + z0 + i];
 
     const uint32_t tm = a.thr_map[((size_t)tile * a.nslab + slab) * kProjThreads + tid];
@@ -563,20 +577,21 @@ __global__ __launch_bounds__(NBW >= 4 ? 1024 : 512) void rows_hpsi_kernel(RowsHp
     const int ty = a.ty;
     const int lband = (ty + 8) * W;
     const int nthr = blockDim.x;
-    // LDS layout (rows_hpsi_lds on the host): planes, per-item projector values, E*c, items,
-    // group run offsets, the slab's per-plane item offsets, projector staging
+    // LDS layout (rows_hpsi_lds on the host): planes, per-item projector values of one chunk of
+    // nthr items, E*c, the plane's items, group run offsets (two planes), the slab's per-plane
+    // item offsets, projector staging
     const int ngo = ty * a.ng4;                                // groups of a tile plane
     T* planes = reinterpret_cast<T*>(smem);                    // [2][NBW][ty + 8][W]
-    T* stg = planes + (size_t)2 * NBW * lband;                 // [stg_cap][NBW][4]
-    T* ec = stg + (size_t)a.stg_cap * NBW * 4;                 // [max_col][NBW][NJS]
-    size_t off = sizeof(T) * ((size_t)2 * NBW * lband + (size_t)a.stg_cap * NBW * 4 + (size_t)a.max_col * NBW * NJS);
+    T* stg = planes + (size_t)2 * NBW * lband;                 // [nthr][NBW][4]
+    T* ec = stg + (size_t)nthr * NBW * 4;                      // [max_col][NBW][NJS]
+    size_t off = sizeof(T) * ((size_t)2 * NBW * lband + (size_t)nthr * NBW * 4 + (size_t)a.max_col * NBW * NJS);
     uint32_t* itl = reinterpret_cast<uint32_t*>(smem + off);   // [nthr * kItemsPT]
     off += sizeof(uint32_t) * (size_t)nthr * kItemsPT;
-    int* gol = reinterpret_cast<int*>(smem + off);             // [ngo + 1]
-    off += sizeof(int) * (size_t)(ngo + 1);
+    int* gol = reinterpret_cast<int*>(smem + off);             // [2][ngo + 1]
+    off += sizeof(int) * (size_t)2 * (ngo + 1);
     int* itoffl = reinterpret_cast<int*>(smem + off);          // [tz + 1]
     off = align16(off + sizeof(int) * (a.tz + 1));
-    const ProjLds<T> L = carve_proj<T>(smem + off, a.max_col, a.nproj);
+    const ProjLds<T> L = carve_proj<T, PF == 0>(smem + off, a.max_col, a.nproj);
 
     const int wg = xcd_order(blockIdx.x, gridDim.x);
     const int tile = wg % a.ntile;
@@ -634,7 +649,7 @@ __global__ __launch_bounds__(NBW >= 4 ? 1024 : 512) void rows_hpsi_kernel(RowsHp
     if constexpr (NL) {
         const int c0 = a.col_off[tile];
         const int ncol = a.col_off[tile + 1] - c0;
-        stage_proj<T>(L, a.images, a.projs, a.nproj, a.col_img + c0, ncol, a.h, tid, nthr);
+        stage_proj<T, PF == 0>(L, a.images, a.projs, a.nproj, a.col_img + c0, ncol, a.h, tid, nthr);
         for (int i = tid; i < ncol * NBW * NJS; i += nthr) {
             const int li = i / (NBW * NJS), b = (i / NJS) % NBW, j = i % NJS;
             const int at = a.images[a.col_img[c0 + li]].atom;
@@ -662,22 +677,111 @@ __global__ __launch_bounds__(NBW >= 4 ? 1024 : 512) void rows_hpsi_kernel(RowsHp
     Row<T> qn{}, hn{}, vn{};
     uint32_t ipre[kItemsPT];
     int gpre[kGroupsPT];
-    const int lim = max(a.nitems - 1, 0);
+    // items and group offsets through buffer resources: a load past the end of the item array
+    // returns zero (such slots are never read), and each offset is one add to a per-thread
+    // constant
+    const rsrc_t ritems = rsrc(a.items, (uint32_t)((int64_t)a.nitems * 4));
+    const rsrc_t rgrp = rsrc(a.grp_off, (uint32_t)((int64_t)a.ntile * nz * (ngo + 1) * 4));
     auto prefetch = [&](int z) {
         qn = vload<T>(rin, vo, zoff(z + 4));
         vn = vload<T>(rsec, vo, zoff(z));
         hn = vload<T>(rin, ho, zoff(z));
         if constexpr (NL) {
-            const int ib = itoffl[z - z0];
+            const int ib4 = __builtin_amdgcn_readfirstlane(itoffl[z - z0]) * 4;
 #pragma unroll
-            for (int k = 0; k < kItemsPT; ++k) ipre[k] = a.items[min(ib + tid + nthr * k, lim)];
-            const int* gp = a.grp_off + ((size_t)tile * nz + z) * (ngo + 1);
+            for (int k = 0; k < kItemsPT; ++k)
+                ipre[k] = __builtin_amdgcn_raw_buffer_load_b32(ritems, ib4 + (tid + nthr * k) * 4, 0, 0);
+            const int gb4 = (tile * nz + z) * (ngo + 1) * 4;
 #pragma unroll
-            for (int k = 0; k < kGroupsPT; ++k) gpre[k] = gp[min(tid + nthr * k, ngo)];
+            for (int k = 0; k < kGroupsPT; ++k)
+                gpre[k] = (int)__builtin_amdgcn_raw_buffer_load_b32(rgrp, gb4 + min(tid + nthr * k, ngo) * 4, 0, 0);
         }
     };
     prefetch(z0);
     const T hx = (T)a.h[0], hy = (T)a.h[1], hz = (T)a.h[2];
+
+    // projector values of item i of the plane at z into per-item slot `slot`: the item (points
+    // of one image in one 4-point group) is evaluated once for the workgroup's bands; no two
+    // threads write one slot (no atomics)
+    auto eval_item = [&](int i, int slot, int z) {
+        const uint32_t it = itl[i];
+        const int ir = (int)(it & 31u), xs = (int)((it >> 5) & 1023u);
+        const int np = (int)((it >> 15) & 3u) + 1, li = (int)(it >> 17);
+        T val[NBW][4];
+        if constexpr (PF != 0) {
+            // the item's image constants and the E*c of its atom: independent LDS reads
+            const int4 cn = *reinterpret_cast<const int4*>(L.im_n + 4 * li);
+            T cq[8];
+            load_q<T>(L, li, cq);
+            T e[NBW][4];
+#pragma unroll
+            for (int b = 0; b < NBW; ++b) {
+                const Row<T>* ep = reinterpret_cast<const Row<T>*>(ec + ((size_t)li * NBW + b) * NJS);
+#pragma unroll
+                for (int q4 = 0; q4 < 4 / VN; ++q4) {
+                    const Row<T> w = ep[q4];
+#pragma unroll
+                    for (int u = 0; u < VN; ++u) e[b][q4 * VN + u] = w.v[u];
+                }
+            }
+            const T dy = hy * ((T)(y0 + ir - cn.y) - cq[1]);
+            const T dz = hz * ((T)(z - cn.z) - cq[2]);
+            T dx[4], es[4], ep[4];
+            sp_values4<T, PF>(cq, cn.x, xs & ~3, xs & 3, np, hx, dy * dy + dz * dz, dx, es, ep);
+#pragma unroll
+            for (int b = 0; b < NBW; ++b) {
+                const T ayz = e[b][1] * dy + e[b][2] * dz;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) val[b][u] = e[b][0] * es[u] + ep[u] * (ayz + e[b][3] * dx[u]);
+            }
+        } else {
+            const int* ii = L.im_i + 5 * li;
+            const T* ff = L.im_f + 3 * li;
+            const T dy = hy * ((T)(y0 + ir - ii[1]) - ff[1]);
+            const T dz = hz * ((T)(z - ii[2]) - ff[2]);
+            T e[NBW][NJS];
+#pragma unroll
+            for (int b = 0; b < NBW; ++b) {
+                const Row<T>* ep = reinterpret_cast<const Row<T>*>(ec + ((size_t)li * NBW + b) * NJS);
+#pragma unroll
+                for (int q4 = 0; q4 < NJS / VN; ++q4) {
+                    const Row<T> w = ep[q4];
+#pragma unroll
+                    for (int u = 0; u < VN; ++u) e[b][q4 * VN + u] = w.v[u];
+                }
+            }
+            const int gx = xs & ~3, u0 = xs & 3;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+#pragma unroll
+                for (int b = 0; b < NBW; ++b) val[b][u] = T(0);
+                if (u >= u0 && u < u0 + np) {
+                    const T dx = hx * ((T)(gx + u - ii[0]) - ff[0]);
+                    T beta[NJS];
+                    beta_at<T, NJS, PF>(L, li, dx, dy, dz, beta);
+#pragma unroll
+                    for (int b = 0; b < NBW; ++b) {
+                        T v = T(0);
+#pragma unroll
+                        for (int j = 0; j < NJS; ++j) v += e[b][j] * beta[j];
+                        val[b][u] = v;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < NBW; ++b) {
+            Row<T>* dst = reinterpret_cast<Row<T>*>(stg + ((size_t)slot * NBW + b) * 4);
+#pragma unroll
+            for (int h = 0; h < 4 / VN; ++h) {
+                Row<T> w;
+#pragma unroll
+                for (int u = 0; u < VN; ++u) w.v[u] = val[b][h * VN + u];
+                dst[h] = w;
+            }
+        }
+    };
+
     // a plane's result is stored at the top of the next iteration, after that iteration's
     // loads were issued one iteration earlier: every iteration starts with the same memory
     // operations outstanding (the prefetch, then one store), so waiting for the prefetch leaves
@@ -685,9 +789,11 @@ __global__ __launch_bounds__(NBW >= 4 ? 1024 : 512) void rows_hpsi_kernel(RowsHp
     Row<T> res{};
     int st_off = kOOB, st_z = 0;
 
+    // q[k] is the plane z - 4 + k of the z-queue
     for (int z = z0; z < z1; ++z) {
         const int buf = (z - z0) & 1;
         T* Lb = planes + ((size_t)buf * NBW + (bl < NBW ? bl : 0)) * lband;
+        int* golb = gol + (size_t)buf * (ngo + 1);
         vstore<T>(rout, res, st_off, st_z);
         q[8] = qn;
         const Row<T> vcur = vn;
@@ -701,139 +807,26 @@ __global__ __launch_bounds__(NBW >= 4 ? 1024 : 512) void rows_hpsi_kernel(RowsHp
         }
         int nit = 0;
         if constexpr (NL) {
-            // this plane's items and group runs (the previous plane's readers passed its second
-            // barrier)
+            // this plane's items (their readers of the previous plane passed that plane's last
+            // chunk barrier) and group runs (a buffer per plane parity: the previous plane's
+            // readers may still be gathering)
             nit = itoffl[z - z0 + 1] - itoffl[z - z0];
+            // all nthr * kItemsPT slots (those past the plane's items are never read)
 #pragma unroll
-            for (int k = 0; k < kItemsPT; ++k) {
-                const int i = tid + nthr * k;
-                if (i < nit) itl[i] = ipre[k];
-            }
+            for (int k = 0; k < kItemsPT; ++k) itl[tid + nthr * k] = ipre[k];
 #pragma unroll
             for (int k = 0; k < kGroupsPT; ++k) {
                 const int i = tid + nthr * k;
-                if (i <= ngo) gol[i] = gpre[k];
+                if (nthr * k <= ngo && i <= ngo) golb[i] = gpre[k];
             }
         }
         if (z + 1 < z1) prefetch(z + 1);
         __syncthreads();
 
-        if constexpr (NL) {
-            // projector values of this plane's items, dealt over all threads of the workgroup:
-            // each item (points of one image in one 4-point group) is evaluated once for the
-            // workgroup's bands and its values are written to the item's own slots (no two
-            // threads write one slot, no atomics)
-            if constexpr (PF != 0) {
-                for (int i = tid; i < nit; i += nthr) {
-                    const uint32_t it = itl[i];
-                    const int ir = (int)(it & 31u), xs = (int)((it >> 5) & 1023u);
-                    const int np = (int)((it >> 15) & 3u) + 1, li = (int)(it >> 17);
-                    // the item's image constants and the E*c of its atom: independent LDS reads
-                    const int4 cn = *reinterpret_cast<const int4*>(L.im_n + 4 * li);
-                    T cq[8];
-                    load_q<T>(L, li, cq);
-                    T e[NBW][4];
-#pragma unroll
-                    for (int b = 0; b < NBW; ++b) {
-                        const Row<T>* ep = reinterpret_cast<const Row<T>*>(ec + ((size_t)li * NBW + b) * NJS);
-#pragma unroll
-                        for (int q4 = 0; q4 < 4 / VN; ++q4) {
-                            const Row<T> w = ep[q4];
-#pragma unroll
-                            for (int u = 0; u < VN; ++u) e[b][q4 * VN + u] = w.v[u];
-                        }
-                    }
-                    const T dy = hy * ((T)(y0 + ir - cn.y) - cq[1]);
-                    const T dz = hz * ((T)(z - cn.z) - cq[2]);
-                    T dx[4], es[4], ep[4];
-                    sp_values4<T, PF>(cq, cn.x, xs & ~3, xs & 3, np, hx, dy * dy + dz * dz, dx, es, ep);
-#pragma unroll
-                    for (int b = 0; b < NBW; ++b) {
-                        T val[4];
-#pragma unroll
-                        for (int u = 0; u < 4; ++u)
-                            val[u] = e[b][0] * es[u] + ep[u] * (e[b][1] * dy + e[b][2] * dz + e[b][3] * dx[u]);
-                        Row<T>* dst = reinterpret_cast<Row<T>*>(stg + ((size_t)i * NBW + b) * 4);
-#pragma unroll
-                        for (int h = 0; h < 4 / VN; ++h) {
-                            Row<T> w;
-#pragma unroll
-                            for (int u = 0; u < VN; ++u) w.v[u] = val[h * VN + u];
-                            dst[h] = w;
-                        }
-                    }
-                }
-            } else
-            for (int i = tid; i < nit; i += nthr) {
-                const uint32_t it = itl[i];
-                const int ir = (int)(it & 31u), xs = (int)((it >> 5) & 1023u);
-                const int np = (int)((it >> 15) & 3u) + 1, li = (int)(it >> 17);
-                const int* ii = L.im_i + 5 * li;
-                const T* ff = L.im_f + 3 * li;
-                const T dy = hy * ((T)(y0 + ir - ii[1]) - ff[1]);
-                const T dz = hz * ((T)(z - ii[2]) - ff[2]);
-                T e[NBW][NJS];
-#pragma unroll
-                for (int b = 0; b < NBW; ++b) {
-                    const Row<T>* ep = reinterpret_cast<const Row<T>*>(ec + ((size_t)li * NBW + b) * NJS);
-#pragma unroll
-                    for (int q4 = 0; q4 < NJS / VN; ++q4) {
-                        const Row<T> w = ep[q4];
-#pragma unroll
-                        for (int u = 0; u < VN; ++u) e[b][q4 * VN + u] = w.v[u];
-                    }
-                }
-                const int gx = xs & ~3, u0 = xs & 3;
-                T val[NBW][4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-#pragma unroll
-                    for (int b = 0; b < NBW; ++b) val[b][u] = T(0);
-                    if (u >= u0 && u < u0 + np) {
-                        const T dx = hx * ((T)(gx + u - ii[0]) - ff[0]);
-                        T beta[NJS];
-                        beta_at<T, NJS, PF>(L, li, dx, dy, dz, beta);
-#pragma unroll
-                        for (int b = 0; b < NBW; ++b) {
-                            T v = T(0);
-#pragma unroll
-                            for (int j = 0; j < NJS; ++j) v += e[b][j] * beta[j];
-                            val[b][u] = v;
-                        }
-                    }
-                }
-#pragma unroll
-                for (int b = 0; b < NBW; ++b) {
-                    Row<T>* dst = reinterpret_cast<Row<T>*>(stg + ((size_t)i * NBW + b) * 4);
-#pragma unroll
-                    for (int h = 0; h < 4 / VN; ++h) {
-                        Row<T> w;
-#pragma unroll
-                        for (int u = 0; u < VN; ++u) w.v[u] = val[b][h * VN + u];
-                        dst[h] = w;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-
         if (own) {
+            // V psi and the stencil: centre and z from the register queue, x and y from LDS,
+            // accumulated term by term so few values are live at once
             const T* Lc = Lb + (r + 4) * W + x;   // Lc[4 + i] is point x + i
-#pragma unroll
-            for (int i = 0; i < VN; ++i) res.v[i] = a.add_v ? vcur.v[i] * q[4].v[i] : vcur.v[i];
-            if constexpr (NL) {
-                // projector term: the values of the items of this point's 4-point group
-                const int gi = r * a.ng4 + (x >> 2);
-                const int s1 = gol[gi + 1];
-                const int h = (VN == 4) ? 0 : ((x >> 1) & 1);   // FP64: which half of the group
-                for (int i = gol[gi]; i < s1; ++i) {
-                    const Row<T> w = reinterpret_cast<const Row<T>*>(stg + ((size_t)i * NBW + bl) * 4)[h];
-#pragma unroll
-                    for (int u = 0; u < VN; ++u) res.v[u] += w.v[u];
-                }
-            }
-            // stencil: centre and z from the register queue, x and y from LDS, accumulated
-            // term by term so few values are live at once
             T d[VN];
 #pragma unroll
             for (int i = 0; i < VN; ++i) {
@@ -862,7 +855,33 @@ __global__ __launch_bounds__(NBW >= 4 ? 1024 : 512) void rows_hpsi_kernel(RowsHp
                 for (int i = 0; i < VN; ++i) d[i] += sc.cy[m] * (yp.v[i] + ym.v[i]);
             }
 #pragma unroll
-            for (int i = 0; i < VN; ++i) res.v[i] += d[i];
+            for (int i = 0; i < VN; ++i) res.v[i] = (a.add_v ? vcur.v[i] * q[4].v[i] : vcur.v[i]) + d[i];
+        }
+
+        if constexpr (NL) {
+            // projector term in chunks of nthr items (one item per thread per chunk): values
+            // into the chunk's slots, then each point adds the values of the items of its
+            // 4-point group's run that lie in the chunk
+            int s0 = 0, s1 = 0;
+            if (own) {
+                const int gi = r * a.ng4 + (x >> 2);
+                s0 = golb[gi];
+                s1 = golb[gi + 1];
+            }
+            const int hsel = (VN == 4) ? 0 : ((x >> 1) & 1);   // FP64: which half of the group
+            for (int c0 = 0; c0 < nit; c0 += nthr) {
+                if (c0 + tid < nit) eval_item(c0 + tid, tid, z);
+                __syncthreads();
+                if (own) {
+                    const int e1 = min(s1, c0 + nthr);
+                    for (int i = max(s0, c0); i < e1; ++i) {
+                        const Row<T> w = reinterpret_cast<const Row<T>*>(stg + ((size_t)(i - c0) * NBW + bl) * 4)[hsel];
+#pragma unroll
+                        for (int u = 0; u < VN; ++u) res.v[u] += w.v[u];
+                    }
+                }
+                if (c0 + nthr < nit) __syncthreads();   // the next chunk reuses the slots
+            }
         }
         // every lane stores (next iteration): lanes without a point, or past the last band, use kOOB
         st_off = vst;
@@ -1046,20 +1065,20 @@ static size_t rows_proj_lds(const RowsGeo& rg, const qg_grid& g, int nbp, int np
     size_t off = align16(sizeof(T) * std::max(pl, red));
     off += sizeof(uint32_t) * kProjThreads * kItemsPT;
     off = align16(off + sizeof(int) * (rg.tzp + 1));
-    return off + proj_lds_bytes<T>(std::max(1, rg.max_col), nproj);
+    return off + proj_lds_bytes<T>(std::max(1, rg.max_col), nproj, rg.pform == 0);
 }
 
-// LDS bytes of an apply workgroup of nthr threads (layout of rows_hpsi_kernel); stg_cap items
-// of per-item values, ng4 groups per row
+// LDS bytes of an apply workgroup of nthr threads (layout of rows_hpsi_kernel): per-item values
+// of one chunk of nthr items, ng4 groups per row, projector form pform
 template <class T>
 static size_t rows_hpsi_lds(const qg_grid& g, int ty, int tz, int nbw, int njs, int max_col, int nproj, int nthr,
-                            int stg_cap, int ng4) {
+                            int ng4, int pform) {
     const size_t W = g.n[0] + 8;
-    size_t off = sizeof(T) * (2 * (size_t)nbw * (ty + 8) * W + (size_t)stg_cap * nbw * 4 + (size_t)max_col * nbw * njs);
+    size_t off = sizeof(T) * (2 * (size_t)nbw * (ty + 8) * W + (size_t)nthr * nbw * 4 + (size_t)max_col * nbw * njs);
     off += sizeof(uint32_t) * (size_t)nthr * kItemsPT;
-    off += sizeof(int) * (size_t)(ty * ng4 + 1);
+    off += sizeof(int) * 2 * (size_t)(ty * ng4 + 1);
     off = align16(off + sizeof(int) * (tz + 1));
-    return off + proj_lds_bytes<T>(max_col, nproj);
+    return off + proj_lds_bytes<T>(max_col, nproj, pform == 0);
 }
 
 // Scratch of one application: projector partials and coefficients (grown, never shrunk).
@@ -1127,7 +1146,6 @@ static void rows_launch_hpsi_t(const qg_grid& g, const RowsShape& sh, const Rows
         a.nitems = rg->it_off.empty() ? 0 : rg->it_off.back();
         a.grp_off = rg->d_grp_off.p;
         a.ng4 = rg->ng4;
-        a.stg_cap = rg->max_items;
         a.images = nl->d_images.p; a.projs = nl->d_projs.p; a.nproj = (int)nl->geo.projs.size();
         a.coef = coef; a.coef_energy = nl->d_coef_energy.p; a.atom_coef_off = nl->d_atom_coef_off.p;
         a.ncoef = nl->geo.ncoef;
@@ -1138,7 +1156,7 @@ static void rows_launch_hpsi_t(const qg_grid& g, const RowsShape& sh, const Rows
     QG_REQUIRE(nthr <= (NBW >= 4 ? 1024 : 512), "row tile too wide for one workgroup");
     QG_REQUIRE(!rg || rg->max_items <= nthr * kItemsPT, "a tile plane holds more projector items than staged");
     QG_REQUIRE(!rg || rg->ty * rg->ng4 + 1 <= nthr * kGroupsPT, "a tile plane holds more groups than staged");
-    const size_t lds = rows_hpsi_lds<T>(g, sh.ty, sh.tz, NBW, NJS, a.max_col, a.nproj, nthr, a.stg_cap, a.ng4);
+    const size_t lds = rows_hpsi_lds<T>(g, sh.ty, sh.tz, NBW, NJS, a.max_col, a.nproj, nthr, a.ng4, PF);
     QG_REQUIRE(lds <= lds_limit(), "row tile does not fit in LDS");
     const int64_t nwg = (int64_t)a.ntile * a.nslab * ((nbands + NBW - 1) / NBW);
     QG_REQUIRE(nwg < (1LL << 31), "too many workgroups");
@@ -1195,7 +1213,7 @@ static bool rows_apply_t(qg_ham* h, int nbands, const T* psi, int64_t ld, T* hps
         if (rg->ty * rg->ng4 + 1 > nthr * kGroupsPT) return false;
         if (rows_proj_lds<T>(*rgp, g, sh.nbp, (int)nl->geo.projs.size()) > lds_limit()) return false;
         if (rows_hpsi_lds<T>(g, sh.ty, sh.tz, sh.nbw, rg->njs, std::max(1, rg->max_col), (int)nl->geo.projs.size(),
-                             nthr, rg->max_items, rg->ng4) > lds_limit())
+                             nthr, rg->ng4, rg->pform) > lds_limit())
             return false;
     } else if (rows_hpsi_lds<T>(g, sh.ty, sh.tz, sh.nbw, 4, 1, 0, rows_nthr(g, dtype, sh.ty, sh.nbw), 0, 0) >
                lds_limit()) {
