@@ -163,6 +163,41 @@ int qg_hamiltonian_create(const qg_grid* g, const double* V_host, qg_nonlocal* n
 int qg_hamiltonian_destroy(qg_ham* h);
 /* Replace the local potential (SCF update of ml_Vtot). */
 int qg_hamiltonian_set_potential(qg_ham* h, const double* V_host);
+/* The same from a device array (FP64, Ng values): the SCF step installs the new potential
+ * without leaving the device. */
+int qg_hamiltonian_set_potential_device(qg_ham* h, const double* V_dev, void* stream);
+
+/* ------------------------------------------------------------- SCF step ---- */
+/* The steps on either side of the eigensolve in the SCF loop (QUMASUN_MPI::Execute,
+ * qumasun_mpi.scfloop.h:79-160), spin-unpolarised, every field FP64 on the device.
+ *
+ * Occupancies (host): occ_s = 2 / (exp(min((e_s - mu) / kbT, 80)) + 1) with mu from the
+ * bisection of mSetOccupancySpinOff (SetOccupancy.h:7-18, qumasun_mpi.occupancy.h:61-108; the
+ * occupancies of the last bisection step; *mu_out the final bracket midpoint). */
+int qg_occupancy_fermi(int nbands, const double* eig_host, double num_electrons, double kbT,
+                       double* occ_host, double* mu_out);
+/* rho = sum over bands with occ > 0 of occ * psi^2 (mSetDensityOne, qumasun_mpi.density.h:10-32);
+ * psi band-major (ld >= Ng), QG_F64 or QG_F32; rho overwritten. */
+int qg_density(const qg_grid* g, int dtype, int nbands, const void* psi, int64_t ld,
+               const double* occ_host, double* rho_dev, void* stream);
+/* Linear mixing (mSetDensity, qumasun_mpi.density.h:46-64): rho <- prev + ratio (rho - prev),
+ * *diff_abs = dV sum |rho_in - prev|, *total = dV sum rho_out.  rho_prev_dev NULL: no mixing
+ * (only *total).  Either output pointer may be NULL. */
+int qg_density_mix(const qg_grid* g, double* rho_dev, const double* rho_prev_dev, double ratio,
+                   double* diff_abs, double* total, void* stream);
+/* Hartree potential of the periodic cell by FFT (SetPotentialByPoissonFFT_keepRhok,
+ * poisson_fft.h:131-240): V_H(k) = 4 pi rho(k) / |k|^2, V_H(k = 0) = 0 (rocFFT, FP64). */
+int qg_hartree_potential(const qg_grid* g, const double* rho_dev, double* vhart_dev, void* stream);
+/* LDA exchange and correlation potentials (Perdew-Zunger 1981) of rho + pcc (pcc may be NULL),
+ * and the energy densities e_x, e_c when ex_dev / ec_dev are not NULL (Calc_XC_LDA, Vxc.h:46-75;
+ * zeros where rho + pcc < 1e-8). */
+int qg_xc_lda(const qg_grid* g, const double* rho_dev, const double* pcc_dev, double* vx_dev,
+              double* vc_dev, double* ex_dev, double* ec_dev, void* stream);
+/* The potential step of the loop (mSetPotential, qumasun_mpi.potential.h:23-170 without the
+ * high-resolution grid): V_H of rho, V_x and V_c of rho + pcc, V_tot = V_ext + V_H + V_x + V_c
+ * (vext_dev may be NULL) installed in h and written to vtot_dev when not NULL. */
+int qg_scf_potential(qg_ham* h, const double* rho_dev, const double* pcc_dev, const double* vext_dev,
+                     double* vhart_dev, double* vx_dev, double* vc_dev, double* vtot_dev, void* stream);
 /* hpsi[b] = H psi[b] for b < nbands in one fused pass (V*psi + nonlocal - 1/2 Lap8 psi).
  * Composition order of mHamiltonianMatrix_ddm (zero, V*psi, +ProjectionPP, +Laplacian). */
 int qg_hamiltonian_apply(qg_ham* h, int dtype, int nbands, const void* psi, int64_t ld,

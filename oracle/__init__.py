@@ -5,3 +5,4 @@ package, and only as the oracle the tests compare with (or as the timed CPU base
 qumasun_amd never imports it.
 """
 from .binding import *  # noqa: F401,F403
+from .scf import *  # noqa: F401,F403,E402

@@ -24,7 +24,7 @@ from ._native import (QG_F32, QG_F64, QG_PREC_F32, QG_PREC_F64, QG_PREC_MIXED, Q
 
 __all__ = ["Grid", "Laplasian8th", "PseudoPotIntegrator", "Hamiltonian", "Eigen_d_multi",
            "LearnedPreconditioner", "gaussian_projector", "diamond_system", "QGError", "LoadVPS",
-           "PseudoPot_MBK"]
+           "PseudoPot_MBK", "SetOccupancy", "Density", "DensityMix", "HartreePotential", "XcLDA", "SCF"]
 
 
 def _dtype_code(t):
@@ -280,6 +280,13 @@ class Hamiltonian:
         self._h = h
 
     def set_potential(self, V):
+        """V: host array, or a float64 device tensor of Ng values (installed without a host copy)."""
+        if hasattr(V, "is_cuda") and V.is_cuda:
+            vp, nb, _ = _block(V, self.grid.size)
+            if nb != 1 or V.dtype.itemsize != 8:
+                raise QGError("the potential is one float64 field")
+            check(lib().qg_hamiltonian_set_potential_device(self._h, vp, _stream()))
+            return
         V = np.ascontiguousarray(V, dtype=np.float64).reshape(-1)
         check(lib().qg_hamiltonian_set_potential(self._h, dptr(V)))
 
@@ -354,6 +361,144 @@ def Eigen_d_multi(ham, X, P, scf_step=0, **opts):
                                   ctypes.byref(st)))
     stats = {f: getattr(st, f) for f, _ in N.LobpcgStats._fields_}
     return eig, stats
+
+
+# ----------------------------------------------------------------------- SCF step ----
+KB_HARTREE = 3.166811563e-6    # Boltzmann constant in Hartree / K
+
+
+def _field(t, ng, what):
+    import torch
+    if t is None:
+        return None
+    if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous() and t.numel() == ng):
+        raise QGError(f"{what}: a contiguous float64 device tensor of {ng} values is needed")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def SetOccupancy(eigen_values, num_electrons, kbT):
+    """Fermi-Dirac occupancies of the spin-unpolarised states (mSetOccupancySpinOff,
+    qumasun_mpi.occupancy.h:61-108): (occupancy ndarray, mu)."""
+    e = np.ascontiguousarray(eigen_values, dtype=np.float64)
+    occ = np.zeros_like(e)
+    mu = ctypes.c_double()
+    check(lib().qg_occupancy_fermi(len(e), dptr(e), float(num_electrons), float(kbT), dptr(occ), ctypes.byref(mu)))
+    return occ, mu.value
+
+
+def Density(grid, X, occupancy, out=None):
+    """rho = sum over states with occ > 0 of occ * psi^2 (mSetDensityOne, qumasun_mpi.density.h:10-32);
+    X a band-major float64 or float32 device block; returns the float64 device field."""
+    import torch
+    xp, nb, ld = _block(X, grid.size)
+    occ = np.ascontiguousarray(occupancy, dtype=np.float64)
+    if len(occ) != nb:
+        raise QGError("one occupancy per band")
+    if out is None:
+        out = torch.empty(grid.size, dtype=torch.float64, device=X.device)
+    check(lib().qg_density(grid._h, _dtype_code(X), nb, xp, ld, dptr(occ), _field(out, grid.size, "rho"), _stream()))
+    return out
+
+
+def DensityMix(grid, rho, rho_prev, ratio):
+    """rho <- prev + ratio (rho - prev) in place (mSetDensity, qumasun_mpi.density.h:46-64);
+    returns (int |rho - prev| dV, int rho_mixed dV).  rho_prev None: no mixing."""
+    d, t = ctypes.c_double(), ctypes.c_double()
+    check(lib().qg_density_mix(grid._h, _field(rho, grid.size, "rho"), _field(rho_prev, grid.size, "rho_prev"),
+                               float(ratio), ctypes.byref(d), ctypes.byref(t), _stream()))
+    return d.value, t.value
+
+
+def HartreePotential(grid, rho, out=None):
+    """V_H(k) = 4 pi rho(k) / |k|^2, V_H(0) = 0, by FFT (poisson_fft.h:131-240)."""
+    import torch
+    if out is None:
+        out = torch.empty_like(rho)
+    check(lib().qg_hartree_potential(grid._h, _field(rho, grid.size, "rho"), _field(out, grid.size, "V_H"), _stream()))
+    return out
+
+
+def XcLDA(grid, rho, pcc=None):
+    """Perdew-Zunger LDA of rho + pcc (Calc_XC_LDA, Vxc.h:46-75): (V_x, V_c, e_x, e_c)."""
+    import torch
+    vx, vc, ex, ec = (torch.empty_like(rho) for _ in range(4))
+    f = lambda t, w: _field(t, grid.size, w)   # noqa: E731
+    check(lib().qg_xc_lda(grid._h, f(rho, "rho"), f(pcc, "pcc"), f(vx, "vx"), f(vc, "vc"), f(ex, "ex"), f(ec, "ec"),
+                          _stream()))
+    return vx, vc, ex, ec
+
+
+class SCF:
+    """The SCF loop of QUMASUN_MPI::Execute (qumasun_mpi.scfloop.h:16-160), spin-unpolarised, on one
+    GPU: per step the block LOBPCG (eigen_step_initial iterations on the first step,
+    eigen_step_per_scf after), Fermi-Dirac occupancies, the density linearly mixed with the previous
+    one, and the new potential V_ext + V_H + V_x + V_c installed in the Hamiltonian; converged when
+    int |rho - rho_prev| dV / N_e < threshold (1e-7 in the reference).  Every field stays on the
+    device.  The starting density is rho0 (default: the uniform density N_e / volume; the reference
+    starts from its random states or from atomic orbitals, qumasun_mpi.initialize.h:321-380).
+    lobpcg: options of Eigen_d_multi (precision, preconditioner, basis, tol ...)."""
+
+    def __init__(self, ham, vext, num_electrons, nbands, kbT=300.0 * KB_HARTREE, mixing_ratio=0.5,
+                 eigen_step_initial=100, eigen_step_per_scf=10, scf_step=100, threshold=1.0e-7, pcc=None,
+                 **lobpcg):
+        import torch
+        self.H = ham
+        self.grid = ham.grid
+        ng = self.grid.size
+        self.vext = vext if hasattr(vext, "is_cuda") else torch.from_numpy(
+            np.ascontiguousarray(vext, dtype=np.float64)).cuda()
+        self.pcc = pcc
+        self.ne = float(num_electrons)
+        self.nbands = int(nbands)
+        self.kbT, self.ratio = float(kbT), float(mixing_ratio)
+        self.steps = (int(eigen_step_initial), int(eigen_step_per_scf))
+        self.limit, self.threshold = int(scf_step), float(threshold)
+        self.lobpcg = lobpcg
+        dev = self.vext.device
+        self.rho = torch.empty(ng, dtype=torch.float64, device=dev)
+        self.rho_prev = torch.empty_like(self.rho)
+        self.vh, self.vx, self.vc, self.vtot = (torch.empty_like(self.rho) for _ in range(4))
+        self.history = []
+
+    def set_potential(self):
+        """mSetPotential (qumasun_mpi.potential.h:23-170): V_tot of the current density, installed."""
+        f = lambda t, w: _field(t, self.grid.size, w)   # noqa: E731
+        check(lib().qg_scf_potential(self.H._h, f(self.rho, "rho"), f(self.pcc, "pcc"), f(self.vext, "vext"),
+                                     f(self.vh, "vh"), f(self.vx, "vx"), f(self.vc, "vc"), f(self.vtot, "vtot"),
+                                     _stream()))
+
+    def run(self, X, P=None, rho0=None, log=None):
+        """X: (nbands, Ng) float64 device states (in/out, the Kohn-Sham states at exit); P the kept
+        LOBPCG block (zeros if None).  Returns (eigenvalues, occupancies, converged, steps)."""
+        import torch
+        vol = self.grid.size * self.grid.dvol
+        if P is None:
+            P = torch.zeros_like(X)
+        if rho0 is None:
+            self.rho.fill_(self.ne / vol)
+        else:
+            self.rho.copy_(rho0)
+        self.set_potential()
+        eig = occ = None
+        converged = False
+        step = 0
+        for step in range(self.limit):
+            eig, st = Eigen_d_multi(self.H, X, P, step,
+                                    max_iter=self.steps[0] if step == 0 else self.steps[1], **self.lobpcg)
+            occ, mu = SetOccupancy(eig, self.ne, self.kbT)
+            self.rho_prev.copy_(self.rho)
+            Density(self.grid, X, occ, out=self.rho)
+            diff, total = DensityMix(self.grid, self.rho, self.rho_prev, self.ratio)
+            diff /= self.ne
+            self.set_potential()
+            self.history.append(dict(step=step, diff_rho=diff, total_rho=total, mu=mu, eig0=float(eig[0]),
+                                     lobpcg_iterations=st["iterations"], max_residual=st["max_residual"]))
+            if log:
+                log(self.history[-1])
+            if diff < self.threshold:
+                converged = True
+                break
+        return eig, occ, converged, step + 1
 
 
 class LearnedPreconditioner:

@@ -108,6 +108,14 @@ SIGNATURES = {
     "qg_hamiltonian_create": (c_int, [c_void_p, P(c_double), c_void_p, P(c_void_p)]),
     "qg_hamiltonian_destroy": (c_int, [c_void_p]),
     "qg_hamiltonian_set_potential": (c_int, [c_void_p, P(c_double)]),
+    "qg_hamiltonian_set_potential_device": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "qg_occupancy_fermi": (c_int, [c_int, P(c_double), c_double, c_double, P(c_double), P(c_double)]),
+    "qg_density": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, P(c_double), c_void_p, c_void_p]),
+    "qg_density_mix": (c_int, [c_void_p, c_void_p, c_void_p, c_double, P(c_double), P(c_double), c_void_p]),
+    "qg_hartree_potential": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "qg_xc_lda": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "qg_scf_potential": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                 c_void_p]),
     "qg_hamiltonian_apply": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, c_void_p, c_int64,
                                      c_void_p]),
     "qg_hamiltonian_apply_noalloc": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, c_void_p, c_int64,
