@@ -235,7 +235,11 @@ def eigen_leg(cfg, budget, compare):
     ncell, nb = (1, 16) if cfg == "cfg0" else (3, 512)
     s, g, integ, H = build_system(ncell)
     fixture = FIXTURE_CFG0 if cfg == "cfg0" else FIXTURE_CFG2
-    X0 = Q.random_states(g.size, nb, g.dvol)
+    # 512 wanted bands and 16 guard bands (the residual tests look at the lowest 512 only): the
+    # block's edge then lies in no degenerate multiplet of the wanted states, whose last members
+    # would otherwise converge at the rate of a zero gap
+    nguard = 0 if cfg == "cfg0" else 16
+    X0 = Q.random_states(g.size, nb + nguard, g.dvol)
 
     def progress(label, it, el, st):
         log(f"eigen {cfg} {label}: {it} iterations, {el:.1f} s, max residual {st['max_residual']:.2e}")
@@ -254,12 +258,12 @@ def eigen_leg(cfg, budget, compare):
         t0 = time.perf_counter()
         it, step = 0, 0
         while True:
-            e, st = Q.Eigen_d_multi(H, X, P, step, max_iter=chunk, **kw)
+            e, st = Q.Eigen_d_multi(H, X, P, step, max_iter=chunk, nconv=nb, **kw)
             step += 1
             it += st["iterations"]
             el = time.perf_counter() - t0
             progress(label, it, el, st)
-            if float(np.abs(e - ref).max()) <= 1e-6:
+            if float(np.abs(e[:nb] - ref).max()) <= 1e-6:
                 return el, it, True
             if el > budget or st["iterations"] == 0:
                 return el, it, False
@@ -271,15 +275,15 @@ def eigen_leg(cfg, budget, compare):
         it = 0
         st = {}
         for step, tol in enumerate(tols):
-            e, st = Q.Eigen_d_multi(H, X, P, step, max_iter=100, tol=tol, **kw)
+            e, st = Q.Eigen_d_multi(H, X, P, step, max_iter=100, tol=tol, nconv=nb, **kw)
             while st["max_residual"] > tol and st["iterations"] == 100:
                 it += st["iterations"]
                 progress(label, it, time.perf_counter() - t0, st)
-                e, st = Q.Eigen_d_multi(H, X, P, 1, max_iter=100, tol=tol, **kw)
+                e, st = Q.Eigen_d_multi(H, X, P, 1, max_iter=100, tol=tol, nconv=nb, **kw)
             it += st["iterations"]
             el = time.perf_counter() - t0
             progress(label, it, el, st)
-            if float(np.abs(e - ref).max()) <= 1e-6:
+            if float(np.abs(e[:nb] - ref).max()) <= 1e-6:
                 return el, it, True, st
             kw["precision"] = N.QG_PREC_F64   # later sweeps refine in FP64
         return el, it, False, st
@@ -298,7 +302,8 @@ def eigen_leg(cfg, budget, compare):
         budget = min(300.0, max(60.0, 3.2 * (t_l + t_train)))
     chunk = 10 if cfg == "cfg0" else 50
     t_p, it_p, ok_p = until("plain-fp64", ref, budget, chunk, precision=N.QG_PREC_F64, basis=N.QG_BASIS_ORTHO)
-    res = {"workload": f"{len(s['positions'])} atoms, {s['n']}^3 grid, {nb} bands",
+    res = {"workload": f"{len(s['positions'])} atoms, {s['n']}^3 grid, {nb} bands"
+                       + (f" (+{nguard} guard bands in the block)" if nguard else ""),
            "criterion": "max |e - e_FP64| <= 1e-6 Ha (FP64 fixture " + os.path.relpath(fixture, ROOT) + ")",
            "fp32_learned": {"seconds": round(t_l, 3), "iterations": it_l, "reached": ok_l,
                             "train_seconds": round(t_train, 3), "fallback_iter": st_l.get("fallback_iter"),

@@ -283,7 +283,8 @@ typedef struct qg_lobpcg_opts {
                                 (also when the FP32 residual stalls at its rounding floor: no
                                 30% decrease within 3 * max(stall_window, 3) iterations) */
     int stall_window;        /* QG_PRECOND_LEARNED: iterations without a new minimum of the
-                                max residual before the kinetic preconditioner takes over */
+                                block's residual norm (sqrt of the sum of the band residuals
+                                squared) before the kinetic preconditioner takes over */
     /* Optional foreign operator (the reference's OpeA, lobpcg_d_multi_mpi.h:966-969): when
      * set, ax = A x for nbands band-major device vectors (ld == Ng, dtype QG_F64 or QG_F32)
      * is computed by this callback on the given stream instead of by the Hamiltonian; the
@@ -310,13 +311,17 @@ typedef struct qg_lobpcg_opts {
     double reortho;          /* orthonormal basis: a second orthonormalisation pass runs when the
                                 first one's pivot (or SVQB) ratio is at or below this; 0: 1e-3 in
                                 FP64, always in FP32 */
+    int nconv;               /* the residual tests (tol, switch_tol, stall_window, the reported
+                                max_residual) look at the lowest nconv bands only; the others are
+                                guard bands that keep the block's edge off a degenerate multiplet
+                                of the wanted ones.  0: all bands (the reference's behaviour) */
 } qg_lobpcg_opts;
 
 typedef struct qg_lobpcg_stats {
     int iterations;
     int rollbacks;
     int hpsi_calls;          /* band applications of H */
-    double max_residual;     /* final max_k ||r_k|| (dVol-normalised) */
+    double max_residual;     /* final max_k ||r_k|| over the lowest nconv bands (dVol-normalised) */
     double t_hpsi, t_gram, t_update, t_eig, t_other, t_precond;   /* seconds per sector */
     int refine_iterations;   /* FP64 refinement iterations of an FP32-mode run */
     int fallback_iter;       /* iteration at which a stalled learned preconditioner was

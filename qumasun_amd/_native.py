@@ -62,7 +62,8 @@ class LobpcgOpts(ctypes.Structure):
                 ("precond_shift", c_double), ("verbose", c_int), ("switch_tol", c_double),
                 ("stall_window", c_int), ("op", OP_CALLBACK), ("op_user", c_void_p), ("basis", c_int),
                 ("fallback_shift", c_double), ("watch", WATCH_CALLBACK), ("watch_user", c_void_p),
-                ("orth", c_int), ("gram_chunks", c_int), ("reortho", c_double)]
+                ("orth", c_int), ("gram_chunks", c_int), ("reortho", c_double),
+                ("nconv", c_int)]
 
 
 class HpsiPlan(ctypes.Structure):

@@ -205,6 +205,30 @@ def test_lobpcg_cfg0_reduced_precision(gpu, prec):
     assert np.abs(e - np.array(fx["eigenvalues"])).max() < 1e-6, (e, fx["eigenvalues"])
 
 
+def test_lobpcg_guard_bands(gpu, oracle):
+    """nconv: 16 wanted bands and 4 guard bands of the 8-atom cell (FP64, kinetic preconditioner,
+    orthonormal basis, tol 1e-8): the solve stops when the lowest 16 meet the tolerance, the
+    reported maximum residual is theirs (recomputed with the oracle's H), and their eigenvalues
+    are the fixture's."""
+    import torch
+    print("lobpcg guard bands", flush=True)
+    fx = json.load(open(os.path.join(HERE, "golden", "cfg0_eigenvalues.json")))
+    s, g, integ, H = _cell(1, 40)
+    X = torch.from_numpy(Q.random_states(g.size, 20, g.dvol)).to(gpu)
+    P = torch.zeros_like(X)
+    e, st = Q.Eigen_d_multi(H, X, P, 0, precision=N.QG_PREC_F64, max_iter=400, tol=1e-8, nconv=16,
+                            precond=N.QG_PRECOND_KINETIC, precond_shift=1.0, basis=N.QG_BASIS_ORTHO)
+    assert 0 < st["iterations"] < 400 and st["max_residual"] <= 1e-8, st
+    n3, h3 = (s["n"],) * 3, (s["h"],) * 3
+    nat = len(s["positions"])
+    onl = oracle.OracleNL(n3, h3, s["positions"], np.zeros(nat, np.int32), s["projectors"])
+    xs = X.cpu().numpy()
+    hx = oracle.hamiltonian_block(n3, h3, s["V"], onl, xs)
+    res = np.sqrt(g.dvol * np.sum((hx - e[:, None] * xs) ** 2, axis=1))
+    assert abs(res[:16].max() - st["max_residual"]) <= 0.05 * st["max_residual"] + 1e-12, (res, st)
+    assert np.abs(e[:16] - np.array(fx["eigenvalues"])).max() < 1e-8
+
+
 def test_lobpcg_foreign_operator(gpu):
     """The operator callback (the reference's OpeA) drives the same iteration: with the
     library's own H behind the callback the Ritz values equal the built-in path's."""
