@@ -16,6 +16,7 @@ roofline  = the same bytes over the summed duration of all H psi launches of a s
             traffic = HBM bytes per step from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
             summary of this workload, scaled by its copy-kernel calibration.
 secondary = the same measurement on the 64-atom / 80^3 / 128-band cell.
+forces    = nonlocal energy and forces of the same block, every band occupied (time per call).
 cpu_baseline = the FP64 oracle (a C++ restatement of the reference's CPU loops) on all host
             cores (bands dealt over threads; one pass over a bounded sample of the same block),
             with the single-thread figure beside it.
@@ -68,6 +69,9 @@ def parse():
     ap.add_argument("--lobpcg-iters", type=int, default=0,
                     help="only run this many LOBPCG iterations per precision (FP64, FP32, mixed) on the --eigen "
                          "workload and print the per-sector times: the command the solver's GEMM profile is taken on")
+    ap.add_argument("--forces-only", action="store_true",
+                    help="only time the nonlocal energy + forces of the --workload block and print its line: the "
+                         "command the force kernels' profile is taken on")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device copies of 256 MiB (FP32), the known-size kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -190,6 +194,26 @@ def hpsi_leg(key, dtype, steps, warmup):
     }
     del out
     return res, (s, g, integ, H), psi
+
+
+def forces_leg(integ, psi, dtype, steps):
+    """Nonlocal energy and forces of the whole block (every band occupied, occ = 2): the
+    gradient launch and four projections per run of bands, host-synchronised per call.
+    chain bytes = psi read by the projection and the gradient, the three gradient blocks written
+    and read back by their projections (8 block passes); psi once is the fused minimum."""
+    nb, ng = psi.shape
+    esz = 4 if dtype == "f32" else 8
+    occ = np.full(nb, 2.0)
+    integ.EnergyForceNonlocal(psi, occ)
+    k = max(1, min(steps, 5))
+    t0 = time.perf_counter()
+    for _ in range(k):
+        e, f = integ.EnergyForceNonlocal(psi, occ)
+    ms = (time.perf_counter() - t0) / k * 1e3
+    chain = 8 * nb * ng * esz
+    return {"workload": f"{len(f)} atoms, {nb} occupied bands, {dtype}", "ms_per_call": round(ms, 3),
+            "chain_bytes": chain, "chain_GBps": round(chain / (ms * 1e-3) / 1e9, 1),
+            "psi_once_GBps": round(nb * ng * esz / (ms * 1e-3) / 1e9, 1), "energy": e}
 
 
 def cpu_baseline(s, psi, nb_one, nb_all):
@@ -376,6 +400,16 @@ def main():
     if args.copy_calibration:
         copy_calibration()
         return
+    if args.forces_only:
+        import torch
+        w = WORKLOADS[args.workload]
+        s_, g, integ, _ = build_system(w["ncell"])
+        gen = torch.Generator(device="cuda")
+        gen.manual_seed(20241203)
+        tdt = torch.float32 if args.dtype == "f32" else torch.float64
+        psi = (torch.rand((w["bands"], g.size), generator=gen, device="cuda", dtype=torch.float32) - 0.5).to(tdt)
+        print(json.dumps({"forces": forces_leg(integ, psi.contiguous(), args.dtype, args.steps)}))
+        return
     if args.lobpcg_iters > 0:
         lobpcg_iterations(args.eigen if args.eigen != "none" else "cfg2", args.lobpcg_iters)
         return
@@ -406,6 +440,8 @@ def main():
         nb = r["bands"]
         res["cpu_baseline"] = cpu_baseline(system[0], psi, min(nb, 32 if args.workload == "cfg2" else 64),
                                            min(nb, 256 if args.workload == "cfg2" else 128))
+    log("nonlocal energy + forces")
+    res["forces"] = forces_leg(system[2], psi, args.dtype, args.steps)
     del psi, system
     torch.cuda.empty_cache()
     if not args.no_secondary and args.workload != "cfg1":

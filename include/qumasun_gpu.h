@@ -144,6 +144,22 @@ int qg_nonlocal_num_coef(const qg_nonlocal* nl);
  * which always tabulates (SetBlockYlm.h). */
 int qg_nonlocal_projector_form(const qg_nonlocal* nl);
 
+/* out_x/y/z[b] = coef * (d/dx, d/dy, d/dz) psi[b], periodic 8th-order central differences
+ * (Gradient8th / Gradient8th_ddm, GridGradient8th.h:17-160); psi and the three outputs are
+ * band-major device blocks of one dtype (ld, ld_out >= Ng). */
+int qg_gradient8_apply(const qg_grid* g, int dtype, int nbands, const void* psi, int64_t ld,
+                       void* out_x, void* out_y, void* out_z, int64_t ld_out, double coef, void* stream);
+/* Nonlocal energy and forces of occupied states (mGetEnergyPseudoNonlocal and
+ * mGetForcePseudoNonlocal, qumasun_mpi.energy.h:349-375, qumasun_mpi.force.h:226-258, over
+ * EnergyPPnonlocal / ForceNonlocal, PseudoPotOperator_mpi_v2.h:573-644, :905-1013): for the bands
+ * with occ_host[b] > 0, *energy = sum_b occ_b sum_{a,j} E_j c_baj^2 and, when forces_host is not
+ * NULL, forces_host[3a + d] = sum_b sum_j E_j c_baj (-2 occ_b) dV sum beta_aj d_d psi_b with the
+ * 8th-order gradient (set, not accumulated).  psi: nbands band-major device vectors.  The handle
+ * keeps the gradient scratch of a run of bands (at most 4 GiB) between calls, so calls on one
+ * handle must not overlap. */
+int qg_nonlocal_energy_force(qg_nonlocal* nl, int dtype, int nbands, const void* psi, int64_t ld,
+                             const double* occ_host, double* energy, double* forces_host, void* stream);
+
 /* coef[b*ncoef + j] = dV * sum_i beta_j(i) psi_b(i)  (FP64 device array, ncoef per band).
  * Replaces mInnerNonlocalPsi_v2 (PseudoPotOperator_mpi_v2.h:336-463) for a band block. */
 int qg_nonlocal_project(qg_nonlocal* nl, int dtype, int nbands, const void* psi, int64_t ld,

@@ -24,7 +24,8 @@ from ._native import (QG_F32, QG_F64, QG_PREC_F32, QG_PREC_F64, QG_PREC_MIXED, Q
 
 __all__ = ["Grid", "Laplasian8th", "PseudoPotIntegrator", "Hamiltonian", "Eigen_d_multi",
            "LearnedPreconditioner", "gaussian_projector", "diamond_system", "QGError", "LoadVPS",
-           "PseudoPot_MBK", "SetOccupancy", "Density", "DensityMix", "HartreePotential", "XcLDA", "SCF"]
+           "PseudoPot_MBK", "SetOccupancy", "Density", "DensityMix", "HartreePotential", "XcLDA", "SCF",
+           "Gradient8th"]
 
 
 def _dtype_code(t):
@@ -246,6 +247,20 @@ class PseudoPotIntegrator:
                                         ctypes.c_void_p(coef.data_ptr()), _stream()))
         return coef
 
+    def EnergyForceNonlocal(self, psi, occupancy, forces=True):
+        """Nonlocal energy sum_b occ_b sum E c^2 and, with forces, the (natoms, 3) forces
+        (mGetEnergyPseudoNonlocal / mGetForcePseudoNonlocal with ForceNonlocal): states with
+        occ > 0 of the device block psi.  Returns (energy, forces or None)."""
+        pp, nb, ld = _block(psi, self.grid.size)
+        occ = np.ascontiguousarray(occupancy, dtype=np.float64)
+        if len(occ) != nb:
+            raise QGError("one occupancy per band")
+        e = ctypes.c_double()
+        f = np.zeros((len(self.positions), 3)) if forces else None
+        check(lib().qg_nonlocal_energy_force(self._h, _dtype_code(psi), nb, pp, ld, dptr(occ), ctypes.byref(e),
+                                             dptr(f) if forces else None, _stream()))
+        return e.value, f
+
     def ProjectionPP(self, Hp, psi, accumulate=True):
         """Hp += V_NL psi for every band (ProjectionPP accumulates into Hp)."""
         pp, nb, ld = _block(psi, self.grid.size)
@@ -374,6 +389,17 @@ def _field(t, ng, what):
     if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous() and t.numel() == ng):
         raise QGError(f"{what}: a contiguous float64 device tensor of {ng} values is needed")
     return ctypes.c_void_p(t.data_ptr())
+
+
+def Gradient8th(grid, psi, coef=1.0):
+    """(d/dx, d/dy, d/dz) psi * coef per band, periodic 8th-order central differences
+    (Gradient8th, GridGradient8th.h:17-160): three device blocks like psi."""
+    import torch
+    pp, nb, ld = _block(psi, grid.size)
+    outs = [torch.empty_like(psi) for _ in range(3)]
+    check(lib().qg_gradient8_apply(grid._h, _dtype_code(psi), nb, pp, ld, *[ctypes.c_void_p(o.data_ptr()) for o in outs],
+                                   ld, float(coef), _stream()))
+    return outs
 
 
 def SetOccupancy(eigen_values, num_electrons, kbT):

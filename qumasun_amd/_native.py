@@ -104,6 +104,10 @@ SIGNATURES = {
     "qg_nonlocal_projector_form": (c_int, [c_void_p]),
     "qg_gram_schmidt": (c_int, [c_int, c_int64, c_int, c_void_p, c_int64, c_double, c_void_p]),
     "qg_nonlocal_project": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, c_void_p, c_void_p]),
+    "qg_gradient8_apply": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int64,
+                                   c_double, c_void_p]),
+    "qg_nonlocal_energy_force": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, P(c_double), P(c_double),
+                                         P(c_double), c_void_p]),
     "qg_nonlocal_apply": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, c_void_p, c_int64,
                                   c_int, c_void_p]),
     "qg_hamiltonian_create": (c_int, [c_void_p, P(c_double), c_void_p, P(c_void_p)]),

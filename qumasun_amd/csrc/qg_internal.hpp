@@ -247,6 +247,9 @@ struct qg_nonlocal {
     std::vector<std::unique_ptr<qg::RowsGeo>> rows;
     std::mutex rows_mu;             // guards the creation of entries of `rows`
     qg::DevBuf<double> rows_part;   // projector partials of qg_nonlocal_project
+    // qg_nonlocal_energy_force: gradient blocks of a band run and their coefficients
+    qg::DevBuf<unsigned char> force_grad;
+    qg::DevBuf<double> force_coef;
 };
 
 namespace qg {
