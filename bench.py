@@ -17,6 +17,7 @@ roofline  = the same bytes over the summed duration of all H psi launches of a s
             summary of this workload, scaled by its copy-kernel calibration.
 secondary = the same measurement on the 64-atom / 80^3 / 128-band cell.
 forces    = nonlocal energy and forces of the same block, every band occupied (time per call).
+kpoint    = H_k psi of the same cell at k = (pi/L)(1,1,1) on as many complex64 bands.
 cpu_baseline = the FP64 oracle (a C++ restatement of the reference's CPU loops) on all host
             cores (bands dealt over threads; one pass over a bounded sample of the same block),
             with the single-thread figure beside it.
@@ -69,6 +70,8 @@ def parse():
     ap.add_argument("--lobpcg-iters", type=int, default=0,
                     help="only run this many LOBPCG iterations per precision (FP64, FP32, mixed) on the --eigen "
                          "workload and print the per-sector times: the command the solver's GEMM profile is taken on")
+    ap.add_argument("--kpoint-only", action="store_true",
+                    help="only time the k-point H psi of the --workload cell (complex64 bands) and print its line")
     ap.add_argument("--forces-only", action="store_true",
                     help="only time the nonlocal energy + forces of the --workload block and print its line: the "
                          "command the force kernels' profile is taken on")
@@ -214,6 +217,41 @@ def forces_leg(integ, psi, dtype, steps):
     return {"workload": f"{len(f)} atoms, {nb} occupied bands, {dtype}", "ms_per_call": round(ms, 3),
             "chain_bytes": chain, "chain_GBps": round(chain / (ms * 1e-3) / 1e9, 1),
             "psi_once_GBps": round(nb * ng * esz / (ms * 1e-3) / 1e9, 1), "energy": e}
+
+
+def kpoint_leg(system, nb, steps, warmup):
+    """H_k psi of the same cell at k = (pi/L, pi/L, pi/L), nb complex64 bands (the Bloch phase at
+    the zone boundary; the reference's k-point operator).  bytes = 2 B Ng 8 + Ng 4 (psi read
+    once, H psi written once, V once)."""
+    import math
+    import torch
+    import qumasun_amd as Q
+    s, g, integ, H = system
+    kk = math.pi / s["L"]
+    Hk = Q.KpointHamiltonian(H, (kk, kk, kk))
+    ng = g.size
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(20241204)
+    u = torch.complex(torch.rand((nb, ng), generator=gen, device="cuda") - 0.5,
+                      torch.rand((nb, ng), generator=gen, device="cuda") - 0.5).contiguous()
+    out = torch.empty_like(u)
+    for _ in range(warmup):
+        Hk.apply(u, out)
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(steps):
+        Hk.apply(u, out)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    bytes_step = 2 * nb * ng * 8 + ng * 4
+    del u, out
+    return {"workload": f"H_k psi, {len(s['positions'])} atoms, {s['n']}^3, {nb} complex64 bands, k = (pi/L)(1,1,1)",
+            "ms_per_step": round(ms, 4), "GBps": round(bytes_step / (ms * 1e-3) / 1e9, 1),
+            "algorithmic_bytes_per_step": bytes_step}
 
 
 def cpu_baseline(s, psi, nb_one, nb_all):
@@ -400,6 +438,10 @@ def main():
     if args.copy_calibration:
         copy_calibration()
         return
+    if args.kpoint_only:
+        w = WORKLOADS[args.workload]
+        print(json.dumps({"kpoint": kpoint_leg(build_system(w["ncell"]), w["bands"], args.steps, args.warmup)}))
+        return
     if args.forces_only:
         import torch
         w = WORKLOADS[args.workload]
@@ -442,7 +484,11 @@ def main():
                                            min(nb, 256 if args.workload == "cfg2" else 128))
     log("nonlocal energy + forces")
     res["forces"] = forces_leg(system[2], psi, args.dtype, args.steps)
-    del psi, system
+    del psi
+    torch.cuda.empty_cache()
+    log("k-point H psi")
+    res["kpoint"] = kpoint_leg(system, r["bands"], args.steps, args.warmup)
+    del system
     torch.cuda.empty_cache()
     if not args.no_secondary and args.workload != "cfg1":
         log("H psi cfg1 (secondary)")

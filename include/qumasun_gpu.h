@@ -35,6 +35,8 @@ extern "C" {
 /* arithmetic type of a psi block */
 #define QG_F64 0
 #define QG_F32 1
+#define QG_Z128 2   /* complex FP64, re and im interleaved (k-point blocks) */
+#define QG_C64 3    /* complex FP32, re and im interleaved (k-point blocks) */
 
 const char* qg_last_error(void);
 int qg_version(void);
@@ -182,6 +184,34 @@ int qg_hamiltonian_set_potential(qg_ham* h, const double* V_host);
 /* The same from a device array (FP64, Ng values): the SCF step installs the new potential
  * without leaving the device. */
 int qg_hamiltonian_set_potential_device(qg_ham* h, const double* V_dev, void* stream);
+
+/* -------------------------------------------------------------- k points ---- */
+/* The k-point (complex Bloch) Hamiltonian of a real-space Hamiltonian h (QUMASUN_KPOINT,
+ * qumasun_kpoint.matrix.h:21-110): for the cell-periodic part u of psi = exp(i k.r) u,
+ *   H_k u = -1/2 [L8 u - |k|^2 u + 2 i k . D8 u] + V u + sum_j E_j conj(P_j) (dV P_j^T u),
+ * P_j = beta_j(x_u - R) exp(i k . x_u) on the projector's image blocks (x_u the image-frame grid
+ * coordinate; mCreateProjectorRadialOnBlock, PseudoPotOperator_kpoint_v2.h:287-420).  k in
+ * bohr^-1 (kpoint_i * 2 pi / (L_i * samples_i) in the reference).  The handle keeps the
+ * projector phase tables of h's nonlocal operator and rebuilds them when its geometry changes;
+ * h must outlive it. */
+typedef struct qg_kpoint qg_kpoint;
+int qg_kpoint_create(qg_ham* h, double kx, double ky, double kz, qg_kpoint** out);
+int qg_kpoint_destroy(qg_kpoint* kp);
+/* hpsi[b] = H_k psi[b]: mHamiltonianMatrix_ddm at a k point (LaplasianKpoint8th_ddm,
+ * GridDifferenceKpoint8th.h:17-148; ProjectionPP, PseudoPotOperator_kpoint_v2.h:2581-2690 over
+ * mInnerNonlocalPsi_v2, :836-930).  dtype QG_Z128 or QG_C64; psi and hpsi band-major
+ * interleaved complex, ld in complex elements (the reference's SoAComplex holds re and im as
+ * separate arrays per band). */
+int qg_kpoint_apply(qg_kpoint* kp, int dtype, int nbands, const void* psi, int64_t ld, void* hpsi,
+                    int64_t ld_out, void* stream);
+/* Lowest nbands eigenpairs of H_k by block LOBPCG in complex FP64 (Eigen_z_multi_mpi,
+ * lobpcg_z_multi_mpi.h:988-1600; ZGEMM Grams), on an orthonormal [X W P] basis (SVQB, standard
+ * Hermitian Rayleigh-Ritz with rocSOLVER zheevd).  X: nbands QG_Z128 bands (ld = Ng), the start
+ * on entry and the eigenvectors on return (dV-orthonormal); eig_host: nbands values.  Stops when
+ * the largest residual norm sqrt(dV sum |H x - theta x|^2) over the lowest nconv bands (nconv <= 0:
+ * all) is <= tol, or after iter_max iterations.  No preconditioner (the reference's). */
+int qg_kpoint_lobpcg(qg_kpoint* kp, int nbands, void* X, int64_t ld, double* eig_host, int iter_max,
+                     double tol, int nconv, int* iters_out, double* max_res_out, void* stream);
 
 /* ------------------------------------------------------------- SCF step ---- */
 /* The steps on either side of the eigensolve in the SCF loop (QUMASUN_MPI::Execute,

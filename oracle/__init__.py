@@ -7,3 +7,4 @@ qumasun_amd never imports it.
 from .binding import *  # noqa: F401,F403
 from .scf import *  # noqa: F401,F403,E402
 from .force import *  # noqa: F401,F403,E402
+from .kpoint import *  # noqa: F401,F403,E402

@@ -45,6 +45,7 @@ def olib():
         l.or_nl_num_coef.argtypes = [c_void_p]
         l.or_nl_block_points.argtypes = [c_void_p, c_int, c_int]
         l.or_nl_dense_beta.argtypes = [c_void_p, c_int, DP]
+        l.or_nl_dense_beta_k.argtypes = [c_void_p, c_int, c_double, c_double, c_double, DP, DP]
         l.or_nl_energies.argtypes = [c_void_p, DP]
         l.or_nl_project.argtypes = [c_void_p, DP, DP]
         l.or_nl_apply_coef.argtypes = [c_void_p, DP, DP]
@@ -123,6 +124,23 @@ class OracleNL:
             olib().or_nl_dense_beta(self._h, j, _d(col))
             B[j] = col
         return B
+
+    def dense_beta_k(self, k):
+        """(ncoef, Ng) complex k-point projector columns beta exp(i k . x_u) over the images."""
+        ng = self.n[0] * self.n[1] * self.n[2]
+        B = np.zeros((self.ncoef, ng), dtype=np.complex128)
+        re, im = np.zeros(ng), np.zeros(ng)
+        for j in range(self.ncoef):
+            olib().or_nl_dense_beta_k(self._h, j, float(k[0]), float(k[1]), float(k[2]), _d(re), _d(im))
+            B[j] = re + 1j * im
+        return B
+
+    def dense_beta_k_column(self, j, k):
+        """Column j of dense_beta_k (Ng complex values)."""
+        ng = self.n[0] * self.n[1] * self.n[2]
+        re, im = np.zeros(ng), np.zeros(ng)
+        olib().or_nl_dense_beta_k(self._h, j, float(k[0]), float(k[1]), float(k[2]), _d(re), _d(im))
+        return re + 1j * im
 
     def energies(self):
         e = np.zeros(self.ncoef)

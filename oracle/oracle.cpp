@@ -347,6 +347,44 @@ OR_API void or_nl_dense_beta(const or_nl* nl, int j, double* col) {
     }
 }
 
+// Dense k-point projector column j: beta_j(x_u - R) exp(i k . x_u) summed over the periodic
+// images, x_u = h (i - shift) the image-frame coordinate of grid point i (the projector blocks of
+// mCreateProjectorRadialOnBlock, PseudoPotOperator_kpoint_v2.h:287-420, whose phase is
+// cos/sin(k . x_u) with k . x_u = gx_dx (ix - shift_x) + ...).  For brute-force checks only.
+OR_API void or_nl_dense_beta_k(const or_nl* nl, int j, double kx, double ky, double kz, double* re,
+                               double* im) {
+    const int64_t ng = (int64_t)nl->n[0] * nl->n[1] * nl->n[2];
+    std::fill(re, re + ng, 0.0);
+    std::fill(im, im + ng, 0.0);
+    int idx = 0;
+    for (const auto& at : nl->atoms) {
+        for (size_t kk = 0; kk < at.proj.size(); ++kk) {
+            const int l = nl->specs[at.proj[kk]].l;
+            for (int m1 = 0; m1 < 2 * l + 1; ++m1, ++idx) {
+                if (idx != j) continue;
+                const auto& rad = at.radial[kk];
+                int64_t off = 0;
+                for (const auto& b : at.blocks[l]) {
+                    const int sxb = b.e[0] - b.b[0], syb = b.e[1] - b.b[1];
+                    for (int iz = b.b[2]; iz < b.e[2]; ++iz)
+                        for (int iy = b.b[1]; iy < b.e[1]; ++iy)
+                            for (int ix = b.b[0]; ix < b.e[0]; ++ix) {
+                                const int64_t i = off + (ix - b.b[0]) + sxb * ((iy - b.b[1]) + (int64_t)syb * (iz - b.b[2]));
+                                const double yv = (l == 0) ? Y00c() : at.ylm[l * l + m1][i];
+                                const double th = kx * nl->h[0] * (double)(ix - b.shift[0]) +
+                                                  ky * nl->h[1] * (double)(iy - b.shift[1]) +
+                                                  kz * nl->h[2] * (double)(iz - b.shift[2]);
+                                const int64_t g = ix + (int64_t)nl->n[0] * (iy + (int64_t)nl->n[1] * iz);
+                                re[g] += rad[i] * yv * std::cos(th);
+                                im[g] += rad[i] * yv * std::sin(th);
+                            }
+                    off += block_size(b);
+                }
+            }
+        }
+    }
+}
+
 // Per-projector KB energies in coefficient order.
 OR_API void or_nl_energies(const or_nl* nl, double* e) {
     int idx = 0;

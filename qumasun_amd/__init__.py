@@ -25,7 +25,7 @@ from ._native import (QG_F32, QG_F64, QG_PREC_F32, QG_PREC_F64, QG_PREC_MIXED, Q
 __all__ = ["Grid", "Laplasian8th", "PseudoPotIntegrator", "Hamiltonian", "Eigen_d_multi",
            "LearnedPreconditioner", "gaussian_projector", "diamond_system", "QGError", "LoadVPS",
            "PseudoPot_MBK", "SetOccupancy", "Density", "DensityMix", "HartreePotential", "XcLDA", "SCF",
-           "Gradient8th"]
+           "Gradient8th", "KpointHamiltonian"]
 
 
 def _dtype_code(t):
@@ -389,6 +389,60 @@ def _field(t, ng, what):
     if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous() and t.numel() == ng):
         raise QGError(f"{what}: a contiguous float64 device tensor of {ng} values is needed")
     return ctypes.c_void_p(t.data_ptr())
+
+
+def _kdtype(t):
+    import torch
+    if t.dtype == torch.complex128:
+        return N.QG_Z128
+    if t.dtype == torch.complex64:
+        return N.QG_C64
+    raise QGError(f"k-point blocks are complex128 or complex64, not {t.dtype}")
+
+
+class KpointHamiltonian:
+    """The Hamiltonian at one k point (QUMASUN_KPOINT's mHamiltonianMatrix_ddm,
+    qumasun_kpoint.matrix.h:21-110): for the cell-periodic part u of psi = exp(i k.r) u,
+    H_k u = -1/2 [L8 - |k|^2 + 2 i k . D8] u + V u + sum E conj(P) (dV P^T u) with the Bloch-phased
+    projectors P = beta exp(i k . x_u).  k in bohr^-1; blocks are (nbands, Ng) complex tensors."""
+
+    def __init__(self, ham, k):
+        self.ham = ham
+        self.k = tuple(float(c) for c in k)
+        h = ctypes.c_void_p()
+        check(lib().qg_kpoint_create(ham._h, *self.k, ctypes.byref(h)))
+        self._h = h
+
+    def apply(self, psi, out):
+        """out = H_k psi (complex128 or complex64, band-major)."""
+        g = self.ham.grid.size
+        pp, nb, ld = _block(psi, g)
+        op, nbo, ldo = _block(out, g)
+        if nbo != nb or out.dtype != psi.dtype:
+            raise QGError("out must match psi in bands and dtype")
+        check(lib().qg_kpoint_apply(self._h, _kdtype(psi), nb, pp, ld, op, ldo, _stream()))
+        return out
+
+    def Eigen_z_multi(self, X, iter_max=300, tol=1e-6, nconv=0):
+        """Lowest eigenpairs by complex FP64 LOBPCG (Eigen_z_multi_mpi, lobpcg_z_multi_mpi.h:988):
+        X (nbands, Ng) complex128 start, overwritten with the eigenvectors.  Returns
+        (eigenvalues ndarray, iterations, largest residual norm)."""
+        import torch
+        if X.dtype != torch.complex128:
+            raise QGError("the k-point LOBPCG runs in complex128")
+        g = self.ham.grid.size
+        pp, nb, ld = _block(X, g)
+        eig = np.zeros(nb)
+        it = ctypes.c_int()
+        res = ctypes.c_double()
+        check(lib().qg_kpoint_lobpcg(self._h, nb, pp, ld, dptr(eig), int(iter_max), float(tol), int(nconv),
+                                     ctypes.byref(it), ctypes.byref(res), _stream()))
+        return eig, it.value, res.value
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().qg_kpoint_destroy(self._h)
+            self._h = None
 
 
 def Gradient8th(grid, psi, coef=1.0):

@@ -12,6 +12,8 @@ LIB_PATH = os.path.join(HERE, "lib", "libqumasun_gpu.so")
 QG_OK = 0
 QG_F64 = 0
 QG_F32 = 1
+QG_Z128 = 2
+QG_C64 = 3
 QG_RADIAL_GAUSSIAN = 0
 QG_RADIAL_TABLE = 1
 QG_PREC_F64 = 0
@@ -104,6 +106,11 @@ SIGNATURES = {
     "qg_nonlocal_projector_form": (c_int, [c_void_p]),
     "qg_gram_schmidt": (c_int, [c_int, c_int64, c_int, c_void_p, c_int64, c_double, c_void_p]),
     "qg_nonlocal_project": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, c_void_p, c_void_p]),
+    "qg_kpoint_create": (c_int, [c_void_p, c_double, c_double, c_double, P(c_void_p)]),
+    "qg_kpoint_destroy": (c_int, [c_void_p]),
+    "qg_kpoint_apply": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, c_void_p, c_int64, c_void_p]),
+    "qg_kpoint_lobpcg": (c_int, [c_void_p, c_int, c_void_p, c_int64, P(c_double), c_int, c_double, c_int, P(c_int),
+                                 P(c_double), c_void_p]),
     "qg_gradient8_apply": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int64,
                                    c_double, c_void_p]),
     "qg_nonlocal_energy_force": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, P(c_double), P(c_double),

@@ -1450,6 +1450,7 @@ static void nl_build(qg_nonlocal* nl, int natoms, const double* xyz, const int* 
     NlGeometry geo;
     build_nl_geometry(nl->gcopy, natoms, xyz, species, nl->nspecies, nl->nproj.data(), nl->specs.data(), geo);
     nl->geo = std::move(geo);
+    ++nl->geo_version;
     const auto& gm = nl->geo;
     if (!gm.projs.empty()) nl->d_projs.upload(gm.projs.data(), gm.projs.size());
     if (!gm.tables.empty()) nl->d_tables.upload(gm.tables.data(), gm.tables.size());

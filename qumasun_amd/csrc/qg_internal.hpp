@@ -247,6 +247,9 @@ struct qg_nonlocal {
     std::vector<std::unique_ptr<qg::RowsGeo>> rows;
     std::mutex rows_mu;             // guards the creation of entries of `rows`
     qg::DevBuf<double> rows_part;   // projector partials of qg_nonlocal_project
+    // geometry generation: bumped by every (re)build of the projector geometry, so state derived
+    // from it elsewhere (the k-point phase tables) knows when to rebuild
+    int geo_version = 0;
     // qg_nonlocal_energy_force: gradient blocks of a band run and their coefficients
     qg::DevBuf<unsigned char> force_grad;
     qg::DevBuf<double> force_coef;

@@ -1,0 +1,153 @@
+"""GPU parity of the k-point Hamiltonian and its complex LOBPCG against the FP64 oracle.
+
+Tolerances (relative 2-norm over the block): complex FP64 <= 1e-12, complex FP32 <= 2e-6;
+eigenvalues within 1e-9 Ha of the dense Hermitian solve of the oracle's H_k.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import qumasun_amd as Q
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _cell(n, h, seed=3, table=False):
+    rng = np.random.default_rng(seed)
+    L = [n[d] * h[d] for d in range(3)]
+    pos = np.array([[0.3 * L[0], 0.4 * L[1], 0.2 * L[2]], [0.8 * L[0], 0.7 * L[1], 0.9 * L[2]]])
+    p = Q.gaussian_projector(1, -0.5, 0.5, 1.6)
+    if table:
+        xi = np.linspace(math.log(1e-5), math.log(3.0), 2000)
+        r = np.exp(xi)
+        amp = math.sqrt(4.0 / (math.sqrt(math.pi) * 0.5 ** 3))
+        p = Q.table_projector(1, -0.5, 1.6, xi[0], xi[1] - xi[0], amp * r * np.exp(-r * r / 0.5))
+    projs = [[Q.gaussian_projector(0, 1.0, 0.5, 1.6), p]]
+    V = rng.uniform(-0.5, 0.5, n[0] * n[1] * n[2])
+    return pos, projs, V
+
+
+def _engine(n, h, pos, projs, V):
+    g = Q.Grid(*n, *h)
+    integ = Q.PseudoPotIntegrator(g, projs)
+    integ.UpdatePosition(pos)
+    return g, integ, Q.Hamiltonian(g, V, integ)
+
+
+CASES = [((12, 10, 14), (0.35, 0.4, 0.3), (0.4, -0.25, 0.3), False),
+         ((12, 10, 14), (0.35, 0.4, 0.3), (0.0, 0.0, 0.0), False),
+         ((9, 7, 5), (0.45, 0.5, 0.6), (0.7, 0.0, -0.5), False),
+         ((16, 12, 10), (0.3, 0.33, 0.36), (0.0, 0.6, 0.0), True),
+         ((70, 6, 5), (0.2, 0.5, 0.6), (0.3, 0.2, 0.1), False)]
+
+
+@pytest.mark.parametrize("n,h,k,table", CASES)
+@pytest.mark.parametrize("prec", ["z128", "c64"])
+def test_kpoint_apply(gpu, oracle, n, h, k, table, prec):
+    import torch
+    print(f"kpoint apply n={n} k={k} table={table} prec={prec}", flush=True)
+    pos, projs, V = _cell(n, h, table=table)
+    g, integ, H = _engine(n, h, pos, projs, V)
+    onl = oracle.OracleNL(n, h, pos, np.zeros(2, np.int32), projs)
+    rng = np.random.default_rng(7)
+    ng = n[0] * n[1] * n[2]
+    U = rng.standard_normal((3, ng)) + 1j * rng.standard_normal((3, ng))
+    ref = oracle.kpoint_hamiltonian(n, h, V, onl, k, U)
+    dt = torch.complex128 if prec == "z128" else torch.complex64
+    Hk = Q.KpointHamiltonian(H, k)
+    u = torch.from_numpy(U).to(gpu).to(dt)
+    out = torch.empty_like(u)
+    Hk.apply(u, out)
+    torch.cuda.synchronize()
+    err = rel(out.cpu().numpy().astype(np.complex128), ref)
+    assert err <= (1e-12 if prec == "z128" else 2e-6), err
+
+
+def test_kpoint_apply_moved_atoms(gpu, oracle):
+    """The phase tables follow UpdatePosition of the integrator."""
+    import torch
+    n, h, k = (12, 10, 14), (0.35, 0.4, 0.3), (0.4, -0.25, 0.3)
+    pos, projs, V = _cell(n, h)
+    g, integ, H = _engine(n, h, pos, projs, V)
+    Hk = Q.KpointHamiltonian(H, k)
+    pos2 = pos + np.array([0.31, -0.22, 0.17])
+    integ.UpdatePosition(pos2)
+    onl = oracle.OracleNL(n, h, pos2, np.zeros(2, np.int32), projs)
+    rng = np.random.default_rng(8)
+    U = rng.standard_normal((2, g.size)) + 1j * rng.standard_normal((2, g.size))
+    ref = oracle.kpoint_hamiltonian(n, h, V, onl, k, U)
+    u = torch.from_numpy(U).to(gpu)
+    out = torch.empty_like(u)
+    Hk.apply(u, out)
+    torch.cuda.synchronize()
+    assert rel(out.cpu().numpy(), ref) <= 1e-12
+
+
+def test_kpoint_apply_64atoms_80cube(gpu, oracle):
+    """The 64-atom diamond cell on the 80^3 grid at k = (pi/L, 0, 0): 2 bands against the oracle."""
+    import torch
+    print("kpoint apply 64 atoms 80^3", flush=True)
+    s = Q.diamond_system(2, 40)
+    n = (s["n"],) * 3
+    h = (s["h"],) * 3
+    k = (math.pi / s["L"], 0.0, 0.0)
+    g, integ, H = _engine(n, h, s["positions"], s["projectors"], s["V"])
+    onl = oracle.OracleNL(n, h, s["positions"], np.zeros(len(s["positions"]), np.int32), s["projectors"])
+    rng = np.random.default_rng(9)
+    U = rng.standard_normal((2, g.size)) + 1j * rng.standard_normal((2, g.size))
+    ref = oracle.kpoint_hamiltonian(n, h, s["V"], onl, k, U)
+    Hk = Q.KpointHamiltonian(H, k)
+    for dt, tol in ((torch.complex128, 1e-12), (torch.complex64, 2e-6)):
+        u = torch.from_numpy(U).to(gpu).to(dt)
+        out = torch.empty_like(u)
+        Hk.apply(u, out)
+        torch.cuda.synchronize()
+        assert rel(out.cpu().numpy().astype(np.complex128), ref) <= tol
+
+
+@pytest.mark.parametrize("k", [(0.4, -0.25, 0.3), (0.0, 0.0, 0.0)])
+def test_kpoint_lobpcg_dense(gpu, oracle, k):
+    import torch
+    print(f"kpoint lobpcg k={k}", flush=True)
+    n, h = (12, 10, 14), (0.35, 0.4, 0.3)
+    pos, projs, V = _cell(n, h)
+    g, integ, H = _engine(n, h, pos, projs, V)
+    onl = oracle.OracleNL(n, h, pos, np.zeros(2, np.int32), projs)
+    Hd = oracle.kpoint_dense(n, h, V, onl, k)
+    e_ref = np.linalg.eigvalsh(Hd)
+    nb, nconv = 10, 6
+    rng = np.random.default_rng(11)
+    X0 = rng.standard_normal((nb, g.size)) + 1j * rng.standard_normal((nb, g.size))
+    X = torch.from_numpy(X0).to(gpu).contiguous()
+    Hk = Q.KpointHamiltonian(H, k)
+    eig, it, res = Hk.Eigen_z_multi(X, iter_max=400, tol=1e-8, nconv=nconv)
+    print(f"  iterations {it}, residual {res:.2e}, eig {eig[:nconv]}", flush=True)
+    assert res <= 1e-8
+    assert np.abs(eig[:nconv] - e_ref[:nconv]).max() <= 1e-9
+    Xh = X.cpu().numpy()
+    dv = h[0] * h[1] * h[2]
+    G = dv * (Xh.conj() @ Xh.T)
+    assert np.abs(G - np.eye(nb)).max() <= 1e-10
+    R = oracle.kpoint_hamiltonian(n, h, V, onl, k, Xh[:nconv]) - eig[:nconv, None] * Xh[:nconv]
+    assert np.sqrt(dv * (np.abs(R) ** 2).sum(1)).max() <= 2e-8
+
+
+def test_kpoint_time_reversal(gpu):
+    """E(k) = E(-k): H_{-k} = conj(H_k) for real V and real projectors."""
+    import torch
+    n, h, k = (12, 10, 14), (0.35, 0.4, 0.3), (0.5, 0.2, -0.35)
+    pos, projs, V = _cell(n, h)
+    g, integ, H = _engine(n, h, pos, projs, V)
+    rng = np.random.default_rng(12)
+    X0 = rng.standard_normal((8, g.size)) + 1j * rng.standard_normal((8, g.size))
+    out = []
+    for kk in (k, tuple(-c for c in k)):
+        X = torch.from_numpy(X0).to(gpu).contiguous()
+        eig, it, res = Q.KpointHamiltonian(H, kk).Eigen_z_multi(X, iter_max=400, tol=1e-8, nconv=5)
+        out.append(eig[:5])
+    assert np.abs(out[0] - out[1]).max() <= 1e-9
