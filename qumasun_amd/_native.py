@@ -183,6 +183,15 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise QGError(f"{LIB_PATH} is missing: build it with `python -m qumasun_amd.build` "
                           "(there is no CPU fallback)")
+        # torch's wheel ships its own HIP and HSA runtimes under the same sonames as
+        # /opt/rocm's.  Loaded after torch, this library's libamdhip64.so.7 and
+        # libhsa-runtime64.so.1 resolve to torch's copies (one runtime in the process); loaded
+        # before it, torch later maps a second HIP and HSA runtime onto the same device, and its
+        # first device call fails (hipErrorNoDevice) or crashes.  So torch comes first.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         l = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(l, name)

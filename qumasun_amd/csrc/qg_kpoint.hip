@@ -14,6 +14,7 @@
 //                                                          PseudoPotOperator_kpoint_v2.h:287-420
 //   project   mInnerNonlocalPsi_v2: c = dV sum P u (no conjugate)       :836-930
 //   apply     ProjectionPP: Hp += E conj(P) c                           :2581-2690
+//             (atoms coloured so that no two of one launch share a grid point: no atomics)
 //   solver    Eigen_z_multi_mpi (ZGEMM Grams, ZHEGVX)       lobpcg_z_multi_mpi.h:988-1600
 //             restated on an orthonormal basis: the Rayleigh-Ritz problem of [X W P] is a
 //             standard Hermitian one (zheevd), so no B-metric solve is needed
@@ -41,6 +42,10 @@ struct qg_kpoint {
     qg::DevBuf<double2> phase64;
     qg::DevBuf<float2> phase32;
     qg::DevBuf<double2> coef;   // projector coefficients [nb][ncoef]
+    // atoms grouped by colour: no two atoms of one colour share a grid point, so the
+    // application runs one launch per colour without atomics
+    std::vector<int> color_off;   // atoms of colour c: color_atoms[color_off[c] .. color_off[c+1])
+    qg::DevBuf<int> color_atoms;
     // LOBPCG state
     rocblas_handle blas = nullptr;
     ~qg_kpoint() {
@@ -292,56 +297,65 @@ __global__ __launch_bounds__(kProjBlock) void kproject_kernel(KProjArgs a, const
     }
 }
 
-// out_b(g(i)) += conj(phase(i)) sum_j beta_j(i) E_j c[b][j]   (hardware atomic adds: points
-// of neighbouring atoms share grid points)
+// out_b(g(i)) += conj(phase(i)) sum_j beta_j(i) E_j c[b][j] for the atoms of one colour (one
+// workgroup per (atom, chunk of NBA bands)).  Atoms of one colour share no grid point and the
+// image blocks of an atom (periodic images, cutoff groups) are visited one after another with a
+// barrier between them (two blocks of one atom may hold the same grid point; one block holds
+// each point once), so plain read-modify-write suffices.
 template <class T, int NBA, int NJS>
-__global__ __launch_bounds__(kProjBlock) void kapply_kernel(KProjArgs a, typename Cx<T>::t* __restrict__ out,
-                                                             int64_t ld_out, int nitems) {
+__global__ __launch_bounds__(kProjBlock) void kapply_kernel(KProjArgs a, const int* __restrict__ atoms, int natoms_c,
+                                                             typename Cx<T>::t* __restrict__ out, int64_t ld_out,
+                                                             int nitems) {
     using C = typename Cx<T>::t;
-    __shared__ C ec[NJS][NBA];
+    __shared__ T ecr[NJS][NBA], eci[NJS][NBA];
     const int item = xcd_item(blockIdx.x, gridDim.x);
     if (item >= nitems) return;   // padding of the remap: the whole workgroup leaves
-    const int atom = item % a.natoms;
-    const int chunk = item / a.natoms;
+    const int atom = atoms[item % natoms_c];
+    const int chunk = item / natoms_c;
     const int b0 = chunk * NBA;
     const int tid = threadIdx.x;
     const int cbase = a.atom_coef_off[atom];
     const int count = a.atom_coef_off[atom + 1] - cbase;
     if (tid < NJS * NBA) {
         const int j = tid / NBA, b = tid % NBA;
-        C v{T(0), T(0)};
+        T vr = T(0), vi = T(0);
         if (j < count && b0 + b < a.nbands) {
             const double2 cc = a.coef[(int64_t)(b0 + b) * a.ncoef + cbase + j];
             const double e = a.energy[cbase + j];
-            v = C{(T)(e * cc.x), (T)(e * cc.y)};
+            vr = (T)(e * cc.x);
+            vi = (T)(e * cc.y);
         }
-        ec[j][b] = v;
+        ecr[j][b] = vr;
+        eci[j][b] = vi;
     }
     __syncthreads();
     const T* beta = reinterpret_cast<const T*>(a.beta);
     const C* ph = reinterpret_cast<const C*>(a.phase);
     const int nb = min(NBA, a.nbands - b0);
-    const int64_t t0 = a.img_tbl_off[a.atom_img_off[atom]];
-    const int64_t t1 = a.img_tbl_off[a.atom_img_off[atom + 1]];
-    for (int64_t i = t0 + tid; i < t1; i += kProjBlock) {
-        const int g = a.gidx[i];
-        const C e = ph[i];
-        T bj[NJS];
+    T* o = reinterpret_cast<T*>(out);
+    for (int im = a.atom_img_off[atom]; im < a.atom_img_off[atom + 1]; ++im) {
+        const int64_t t0 = a.img_tbl_off[im], t1 = a.img_tbl_off[im + 1];
+        for (int64_t i = t0 + tid; i < t1; i += kProjBlock) {
+            const int g = a.gidx[i];
+            const C e = ph[i];
+            T bj[NJS];
 #pragma unroll
-        for (int j = 0; j < NJS; ++j) bj[j] = beta[i * NJS + j];
+            for (int j = 0; j < NJS; ++j) bj[j] = beta[i * NJS + j];
 #pragma unroll
-        for (int b = 0; b < NBA; ++b) {
-            if (b >= nb) break;
-            T vr = T(0), vi = T(0);
+            for (int b = 0; b < NBA; ++b) {
+                if (b >= nb) break;
+                T vr = T(0), vi = T(0);
 #pragma unroll
-            for (int j = 0; j < NJS; ++j) {
-                vr += bj[j] * ec[j][b].x;
-                vi += bj[j] * ec[j][b].y;
+                for (int j = 0; j < NJS; ++j) {
+                    vr += bj[j] * ecr[j][b];
+                    vi += bj[j] * eci[j][b];
+                }
+                const int64_t off = 2 * ((int64_t)(b0 + b) * ld_out + g);
+                o[off] += e.x * vr + e.y * vi;
+                o[off + 1] += e.x * vi - e.y * vr;
             }
-            T* o = reinterpret_cast<T*>(out + (int64_t)(b0 + b) * ld_out + g);
-            unsafeAtomicAdd(o, e.x * vr + e.y * vi);
-            unsafeAtomicAdd(o + 1, e.x * vi - e.y * vr);
         }
+        __syncthreads();   // the next block of this atom may hold the same grid points
     }
 }
 
@@ -374,6 +388,30 @@ void build_phases(qg_kpoint* kp) {
         kp->phase64.upload(p64.data(), p64.size());
         kp->phase32.upload(p32.data(), p32.size());
     }
+    // greedy colouring of the atoms: the colours of the atoms already placed on an atom's grid
+    // points (one bit per colour and point) are excluded, the lowest free colour is taken
+    const int natoms = geo.natoms;
+    std::vector<uint64_t> used((size_t)g.ng(), 0);
+    std::vector<int> color(natoms, 0);
+    int ncolor = 0;
+    for (int at = 0; at < natoms; ++at) {
+        const int64_t t0 = geo.img_tbl_off[geo.atom_img_off[at]], t1 = geo.img_tbl_off[geo.atom_img_off[at + 1]];
+        uint64_t mask = 0;
+        for (int64_t t = t0; t < t1; ++t) mask |= used[geo.gidx[t]];
+        int c = 0;
+        while (c < 64 && (mask >> c) & 1u) ++c;
+        QG_REQUIRE(c < 64, "k-point projectors: more than 64 atom colours");
+        color[at] = c;
+        ncolor = std::max(ncolor, c + 1);
+        for (int64_t t = t0; t < t1; ++t) used[geo.gidx[t]] |= (uint64_t)1 << c;
+    }
+    kp->color_off.assign(ncolor + 1, 0);
+    for (int at = 0; at < natoms; ++at) ++kp->color_off[color[at] + 1];
+    for (int c = 0; c < ncolor; ++c) kp->color_off[c + 1] += kp->color_off[c];
+    std::vector<int> order(natoms);
+    std::vector<int> fill(kp->color_off.begin(), kp->color_off.end() - 1);
+    for (int at = 0; at < natoms; ++at) order[fill[color[at]]++] = at;
+    if (natoms) kp->color_atoms.upload(order.data(), order.size());
     kp->geo_version = nl->geo_version;
 }
 
@@ -398,19 +436,31 @@ void launch_nl(qg_kpoint* kp, int nbands, const typename Cx<T>::t* psi, int64_t 
     a.energy = nl->d_coef_energy.p;
     a.coef = kp->coef.p;
     a.ncoef = geo.ncoef;
-    const int64_t items = (int64_t)geo.natoms * ((nbands + kNBA - 1) / kNBA);
+    const int64_t chunks = (nbands + kNBA - 1) / kNBA;
+    const int64_t items = (int64_t)geo.natoms * chunks;
     QG_REQUIRE(items < (int64_t(1) << 31) - kXcds, "too many projector items in one launch");
-    const unsigned grid = (unsigned)(((items + kXcds - 1) / kXcds) * kXcds);
-    if (geo.njs == 4) {
-        hipLaunchKernelGGL((kproject_kernel<T, kNBA, 4>), dim3(grid), dim3(kProjBlock), 0, s, a, psi, (int)items);
-        hipLaunchKernelGGL((kapply_kernel<T, kNBA, 4>), dim3(grid), dim3(kProjBlock), 0, s, a, out, ld_out,
+    auto padded = [](int64_t n) { return (unsigned)(((n + kXcds - 1) / kXcds) * kXcds); };
+    if (geo.njs == 4)
+        hipLaunchKernelGGL((kproject_kernel<T, kNBA, 4>), dim3(padded(items)), dim3(kProjBlock), 0, s, a, psi,
                            (int)items);
-    } else {
-        hipLaunchKernelGGL((kproject_kernel<T, kNBA, 8>), dim3(grid), dim3(kProjBlock), 0, s, a, psi, (int)items);
-        hipLaunchKernelGGL((kapply_kernel<T, kNBA, 8>), dim3(grid), dim3(kProjBlock), 0, s, a, out, ld_out,
+    else
+        hipLaunchKernelGGL((kproject_kernel<T, kNBA, 8>), dim3(padded(items)), dim3(kProjBlock), 0, s, a, psi,
                            (int)items);
-    }
     QG_HIP(hipGetLastError());
+    // the application, one launch per atom colour
+    for (size_t c = 0; c + 1 < kp->color_off.size(); ++c) {
+        const int na = kp->color_off[c + 1] - kp->color_off[c];
+        if (na == 0) continue;
+        const int* atoms = kp->color_atoms.p + kp->color_off[c];
+        const int64_t ci = (int64_t)na * chunks;
+        if (geo.njs == 4)
+            hipLaunchKernelGGL((kapply_kernel<T, kNBA, 4>), dim3(padded(ci)), dim3(kProjBlock), 0, s, a, atoms, na,
+                               out, ld_out, (int)ci);
+        else
+            hipLaunchKernelGGL((kapply_kernel<T, kNBA, 8>), dim3(padded(ci)), dim3(kProjBlock), 0, s, a, atoms, na,
+                               out, ld_out, (int)ci);
+        QG_HIP(hipGetLastError());
+    }
 }
 
 template <class T>
