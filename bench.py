@@ -63,7 +63,7 @@ def parse():
     ap.add_argument("--eigen", choices=["none", "cfg0", "cfg2"], default="cfg2",
                     help="also time LOBPCG to 1e-6 Ha: FP32 + learned preconditioner vs plain FP64")
     ap.add_argument("--eigen-budget", type=float, default=0.0,
-                    help="seconds after which the plain FP64 run stops (0: 3.2 x the learned run, training "
+                    help="seconds after which the plain FP64 run stops (0: 3.9 x the learned run, training "
                          "included, at least 60 s and at most 300 s)")
     ap.add_argument("--eigen-compare", action="store_true",
                     help="also time FP32 + kinetic and FP64 + kinetic (the same preconditioner in both precisions)")
@@ -361,7 +361,7 @@ def eigen_leg(cfg, budget, compare):
                                    precond_shift=2.0, switch_tol=1e-3)
     net.detach()
     if budget <= 0:
-        budget = min(300.0, max(60.0, 3.2 * (t_l + t_train)))
+        budget = min(300.0, max(60.0, 3.9 * (t_l + t_train)))
     chunk = 10 if cfg == "cfg0" else 50
     t_p, it_p, ok_p = until("plain-fp64", ref, budget, chunk, precision=N.QG_PREC_F64, basis=N.QG_BASIS_ORTHO)
     res = {"workload": f"{len(s['positions'])} atoms, {s['n']}^3 grid, {nb} bands"

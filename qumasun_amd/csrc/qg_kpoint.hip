@@ -359,7 +359,10 @@ __global__ __launch_bounds__(kProjBlock) void kapply_kernel(KProjArgs a, const i
     }
 }
 
-constexpr int kNBA = 4;
+// bands per projector workgroup: each table point's beta, phase and grid index are read once
+// per chunk, so FP32 takes 8 (FP64 keeps 4 to stay within registers)
+template <class T>
+constexpr int kNBA = sizeof(T) == 4 ? 8 : 4;
 
 // phase tables of the k point on the projector points: (cos, sin)(k . h (i - shift))
 void build_phases(qg_kpoint* kp) {
@@ -436,15 +439,15 @@ void launch_nl(qg_kpoint* kp, int nbands, const typename Cx<T>::t* psi, int64_t 
     a.energy = nl->d_coef_energy.p;
     a.coef = kp->coef.p;
     a.ncoef = geo.ncoef;
-    const int64_t chunks = (nbands + kNBA - 1) / kNBA;
+    const int64_t chunks = (nbands + kNBA<T> - 1) / kNBA<T>;
     const int64_t items = (int64_t)geo.natoms * chunks;
     QG_REQUIRE(items < (int64_t(1) << 31) - kXcds, "too many projector items in one launch");
     auto padded = [](int64_t n) { return (unsigned)(((n + kXcds - 1) / kXcds) * kXcds); };
     if (geo.njs == 4)
-        hipLaunchKernelGGL((kproject_kernel<T, kNBA, 4>), dim3(padded(items)), dim3(kProjBlock), 0, s, a, psi,
+        hipLaunchKernelGGL((kproject_kernel<T, kNBA<T>, 4>), dim3(padded(items)), dim3(kProjBlock), 0, s, a, psi,
                            (int)items);
     else
-        hipLaunchKernelGGL((kproject_kernel<T, kNBA, 8>), dim3(padded(items)), dim3(kProjBlock), 0, s, a, psi,
+        hipLaunchKernelGGL((kproject_kernel<T, kNBA<T>, 8>), dim3(padded(items)), dim3(kProjBlock), 0, s, a, psi,
                            (int)items);
     QG_HIP(hipGetLastError());
     // the application, one launch per atom colour
@@ -454,10 +457,10 @@ void launch_nl(qg_kpoint* kp, int nbands, const typename Cx<T>::t* psi, int64_t 
         const int* atoms = kp->color_atoms.p + kp->color_off[c];
         const int64_t ci = (int64_t)na * chunks;
         if (geo.njs == 4)
-            hipLaunchKernelGGL((kapply_kernel<T, kNBA, 4>), dim3(padded(ci)), dim3(kProjBlock), 0, s, a, atoms, na,
+            hipLaunchKernelGGL((kapply_kernel<T, kNBA<T>, 4>), dim3(padded(ci)), dim3(kProjBlock), 0, s, a, atoms, na,
                                out, ld_out, (int)ci);
         else
-            hipLaunchKernelGGL((kapply_kernel<T, kNBA, 8>), dim3(padded(ci)), dim3(kProjBlock), 0, s, a, atoms, na,
+            hipLaunchKernelGGL((kapply_kernel<T, kNBA<T>, 8>), dim3(padded(ci)), dim3(kProjBlock), 0, s, a, atoms, na,
                                out, ld_out, (int)ci);
         QG_HIP(hipGetLastError());
     }
