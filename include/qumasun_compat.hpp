@@ -319,6 +319,66 @@ private:
     qg_ham* h_ = nullptr;
 };
 
+// The k-point operator of QUMASUN_KPOINT (qumasun_kpoint.matrix.h:21-110) on one GPU, for one
+// spin-k set: H_k of a HamiltonianGPU at k = (kx, ky, kz) bohr^-1 (kpoint_i 2 pi / (L_i samples_i)
+// in the reference).  Complex bands in the reference's layout: per band ng real parts, then ng
+// imaginary parts (SoAComplex with re, im pointers), converted on the device to the engine's
+// interleaved complex.
+class KpointHamiltonianGPU {
+public:
+    KpointHamiltonianGPU(HamiltonianGPU& H, double kx, double ky, double kz) : H_(H) {
+        check(qg_kpoint_create(H.get(), kx, ky, kz, &kp_), "qg_kpoint_create");
+    }
+    ~KpointHamiltonianGPU() {
+        qg_kpoint_destroy(kp_);
+        if (zin_) (void)hipFree(zin_);
+        if (zout_) (void)hipFree(zout_);
+    }
+    KpointHamiltonianGPU(const KpointHamiltonianGPU&) = delete;
+    KpointHamiltonianGPU& operator=(const KpointHamiltonianGPU&) = delete;
+    // Ax = H_k x for nbands planar complex bands (2 ng doubles each; host or device)
+    void ApplyPlanar(int nbands, double* Ax, const double* x, hipStream_t stream = nullptr) {
+        const size_t ng = H_.size();
+        Staged<double> sa(Ax, 2 * ng * nbands, false);
+        Staged<double> sx(const_cast<double*>(x), 2 * ng * nbands, true);
+        reserve(nbands);
+        check(qg_complex_from_planar((int64_t)ng, nbands, sx.get(), QG_Z128, zin_, stream), "qg_complex_from_planar");
+        check(qg_kpoint_apply(kp_, QG_Z128, nbands, zin_, (int64_t)ng, zout_, (int64_t)ng, stream), "qg_kpoint_apply");
+        check(qg_complex_to_planar((int64_t)ng, nbands, zout_, QG_Z128, sa.get(), stream), "qg_complex_to_planar");
+        hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        sa.sync_back();
+    }
+    // OpeA(Ax, x) of the reference's k-point solver: one band with re / im host arrays of ng
+    template <class SoA>
+    void operator()(SoA& Ax, const SoA& x) {
+        const size_t ng = H_.size();
+        std::vector<double> in(2 * ng), out(2 * ng);
+        std::copy(x.re, x.re + ng, in.begin());
+        std::copy(x.im, x.im + ng, in.begin() + ng);
+        ApplyPlanar(1, out.data(), in.data());
+        std::copy(out.begin(), out.begin() + ng, Ax.re);
+        std::copy(out.begin() + ng, out.end(), Ax.im);
+    }
+    qg_kpoint* get() const { return kp_; }
+    size_t size() const { return H_.size(); }
+
+private:
+    void reserve(int nbands) {
+        const size_t need = H_.size() * nbands * 2 * sizeof(double);
+        if (need <= cap_) return;
+        if (zin_) (void)hipFree(zin_);
+        if (zout_) (void)hipFree(zout_);
+        hip_check(hipMalloc(&zin_, need), "hipMalloc");
+        hip_check(hipMalloc(&zout_, need), "hipMalloc");
+        cap_ = need;
+    }
+    HamiltonianGPU& H_;
+    qg_kpoint* kp_ = nullptr;
+    void* zin_ = nullptr;
+    void* zout_ = nullptr;
+    size_t cap_ = 0;
+};
+
 // GramSchmidt(N, num_solution, psi_set, dVol) (GramSchmidt.h:6-38) / GramSchmidt_ddm
 // (GramSchmidt_mpi.h:9-53) on a num_solution x N FP64 band-major block, in place.
 inline void GramSchmidt(int64_t N, int num_solution, double* psi, double dVol) {
@@ -421,6 +481,32 @@ void solve(int nx, int ny, int nz, double dVol, int num_solution, int iter_max, 
 }  // namespace detail
 
 namespace LOBPCG {
+// Eigen_z_multi_mpi's task (lobpcg_z_multi_mpi.h:988) on the device for one spin-k set: the
+// lowest num_solution eigenpairs of H_k by the engine's complex FP64 LOBPCG, from the planar
+// complex start X (num_solution bands of 2 ng doubles, host or device), overwritten with the
+// eigenvectors.  Stops at tol (largest residual norm) or iter_max iterations; returns the
+// iterations run.
+inline int Eigen_z_multi(KpointHamiltonianGPU& Hk, int num_solution, int iter_max, double* eigen_values, double* X,
+                         double tol, double* max_residual = nullptr, hipStream_t stream = nullptr) {
+    const size_t ng = Hk.size();
+    Staged<double> sx(X, 2 * ng * num_solution, true);
+    void* z = nullptr;
+    hip_check(hipMalloc(&z, 2 * ng * num_solution * sizeof(double)), "hipMalloc");
+    int iters = 0;
+    double res = 0.0;
+    int rc = qg_complex_from_planar((int64_t)ng, num_solution, sx.get(), QG_Z128, z, stream);
+    if (rc == QG_OK)
+        rc = qg_kpoint_lobpcg(Hk.get(), num_solution, z, (int64_t)ng, eigen_values, iter_max, tol, 0, &iters, &res,
+                              stream);
+    if (rc == QG_OK) rc = qg_complex_to_planar((int64_t)ng, num_solution, z, QG_Z128, sx.get(), stream);
+    if (rc == QG_OK) rc = hipStreamSynchronize(stream) == hipSuccess ? QG_OK : QG_ERR_HIP;
+    (void)hipFree(z);
+    check(rc, "Eigen_z_multi");
+    sx.sync_back();
+    if (max_residual) *max_residual = res;
+    return iters;
+}
+
 
 // Eigen_d_multi_mpi(l_grid, dVol, num_solution, iter_max, eigen_values, eigen_vectors, keep_P,
 // work, keep_S_matrix, SCF_step, OpeA, watch) (lobpcg_d_multi_mpi.h:882-890): the reference's

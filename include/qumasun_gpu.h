@@ -213,6 +213,13 @@ int qg_kpoint_apply(qg_kpoint* kp, int dtype, int nbands, const void* psi, int64
 int qg_kpoint_lobpcg(qg_kpoint* kp, int nbands, void* X, int64_t ld, double* eig_host, int iter_max,
                      double tol, int nconv, int* iters_out, double* max_res_out, void* stream);
 
+/* Layout conversion between the reference's complex band blocks (per band ng real parts then
+ * ng imaginary parts, FP64: the SoAComplex arrays of Eigen_z_multi_mpi's IntervalPointers,
+ * lobpcg_z_multi_mpi.h:1004-1011) and the engine's interleaved complex bands (dtype QG_Z128 or
+ * QG_C64).  Device pointers; asynchronous on stream. */
+int qg_complex_from_planar(int64_t ng, int nbands, const double* planar, int dtype, void* z, void* stream);
+int qg_complex_to_planar(int64_t ng, int nbands, const void* z, int dtype, double* planar, void* stream);
+
 /* ------------------------------------------------------------- SCF step ---- */
 /* The steps on either side of the eigensolve in the SCF loop (QUMASUN_MPI::Execute,
  * qumasun_mpi.scfloop.h:79-160), spin-unpolarised, every field FP64 on the device.

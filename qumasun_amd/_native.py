@@ -106,6 +106,8 @@ SIGNATURES = {
     "qg_nonlocal_projector_form": (c_int, [c_void_p]),
     "qg_gram_schmidt": (c_int, [c_int, c_int64, c_int, c_void_p, c_int64, c_double, c_void_p]),
     "qg_nonlocal_project": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, c_void_p, c_void_p]),
+    "qg_complex_from_planar": (c_int, [c_int64, c_int, c_void_p, c_int, c_void_p, c_void_p]),
+    "qg_complex_to_planar": (c_int, [c_int64, c_int, c_void_p, c_int, c_void_p, c_void_p]),
     "qg_kpoint_create": (c_int, [c_void_p, c_double, c_double, c_double, P(c_void_p)]),
     "qg_kpoint_destroy": (c_int, [c_void_p]),
     "qg_kpoint_apply": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, c_void_p, c_int64, c_void_p]),

@@ -512,6 +512,29 @@ void check_kblock(const qg_kpoint* kp, int dtype, int nbands, const void* psi, i
     }
 }
 
+// planar FP64 bands [re(ng) | im(ng)] (the reference's SoAComplex blocks) <-> interleaved complex
+template <class C>
+__global__ void planar_to_z_kernel(int64_t ng, int nb, const double* __restrict__ planar, C* __restrict__ z) {
+    const int64_t n = ng * nb;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = t / ng, i = t - b * ng;
+        const double* p = planar + 2 * b * ng;
+        z[t].x = (decltype(z[t].x))p[i];
+        z[t].y = (decltype(z[t].y))p[ng + i];
+    }
+}
+
+template <class C>
+__global__ void z_to_planar_kernel(int64_t ng, int nb, const C* __restrict__ z, double* __restrict__ planar) {
+    const int64_t n = ng * nb;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = t / ng, i = t - b * ng;
+        double* p = planar + 2 * b * ng;
+        p[i] = (double)z[t].x;
+        p[ng + i] = (double)z[t].y;
+    }
+}
+
 // ------------------------------------------------------------------ LOBPCG (complex FP64)
 using zc = rocblas_double_complex;
 
@@ -791,6 +814,40 @@ int qg_kpoint_apply(qg_kpoint* kp, int dtype, int nbands, const void* psi, int64
             kpoint_apply<double>(kp, nbands, psi, ld, hpsi, ld_out, s);
         else
             kpoint_apply<float>(kp, nbands, psi, ld, hpsi, ld_out, s);
+    });
+}
+
+int qg_complex_from_planar(int64_t ng, int nbands, const double* planar, int dtype, void* z, void* stream) {
+    return guarded([&] {
+        QG_REQUIRE(ng >= 0 && nbands >= 0, "negative size");
+        QG_REQUIRE(dtype == QG_Z128 || dtype == QG_C64, "complex dtype required (QG_Z128 or QG_C64)");
+        if (ng == 0 || nbands == 0) return;
+        QG_REQUIRE(planar && z, "null pointer");
+        const unsigned grid = (unsigned)std::min<int64_t>((ng * nbands + 255) / 256, 65536);
+        if (dtype == QG_Z128)
+            hipLaunchKernelGGL(planar_to_z_kernel<double2>, dim3(grid), dim3(256), 0, as_stream(stream), ng, nbands,
+                               planar, (double2*)z);
+        else
+            hipLaunchKernelGGL(planar_to_z_kernel<float2>, dim3(grid), dim3(256), 0, as_stream(stream), ng, nbands,
+                               planar, (float2*)z);
+        QG_HIP(hipGetLastError());
+    });
+}
+
+int qg_complex_to_planar(int64_t ng, int nbands, const void* z, int dtype, double* planar, void* stream) {
+    return guarded([&] {
+        QG_REQUIRE(ng >= 0 && nbands >= 0, "negative size");
+        QG_REQUIRE(dtype == QG_Z128 || dtype == QG_C64, "complex dtype required (QG_Z128 or QG_C64)");
+        if (ng == 0 || nbands == 0) return;
+        QG_REQUIRE(planar && z, "null pointer");
+        const unsigned grid = (unsigned)std::min<int64_t>((ng * nbands + 255) / 256, 65536);
+        if (dtype == QG_Z128)
+            hipLaunchKernelGGL(z_to_planar_kernel<double2>, dim3(grid), dim3(256), 0, as_stream(stream), ng, nbands,
+                               (const double2*)z, planar);
+        else
+            hipLaunchKernelGGL(z_to_planar_kernel<float2>, dim3(grid), dim3(256), 0, as_stream(stream), ng, nbands,
+                               (const float2*)z, planar);
+        QG_HIP(hipGetLastError());
     });
 }
 

@@ -10,6 +10,8 @@
 //   5. the same 12 iterations with a per-band user operator OpeA(Ax, x) on host pointers
 //   6. UpdatePosition with moved nuclei, then Hamiltonian Apply on band 0
 //   7. EigenLOBPCG_d_multi(N, dVol, 8, 40, ev, X, keep, work, true, &H, watch)
+//   8. KpointHamiltonianGPU(H, k) as OpeA(Ax, x) on one SoAComplex band, then
+//      LOBPCG::Eigen_z_multi on 4 planar complex bands
 // Writes every result to argv[1] (raw float64, in that order) and prints the sector counts
 // of the watch.
 #include "qumasun_compat.hpp"
@@ -102,6 +104,31 @@ int main(int argc, char** argv) {
     Watch w3;
     EigenLOBPCG_d_multi((int)ng, dvol, nb, 40, ev3.data(), X3.data(), K3.data(), work.data(), true, &H, w3);
     put(ev3.data(), nb);
+
+    {
+        // 8. the k-point operator through the reference's OpeA(Ax, x) on one SoAComplex band
+        //    (re = band 0, im = band 1 of the start), then a 4-band complex solve on planar bands
+        //    (re = band b, im = band b + 4)
+        struct SoA {
+            double* re;
+            double* im;
+        };
+        KpointHamiltonianGPU Hk(H, 0.3, -0.2, 0.25);
+        std::vector<double> xr(X0.begin(), X0.begin() + ng), xi(X0.begin() + ng, X0.begin() + 2 * ng), ar(ng), ai(ng);
+        SoA x{xr.data(), xi.data()}, ax{ar.data(), ai.data()};
+        Hk(ax, x);
+        put(ar.data(), ng);
+        put(ai.data(), ng);
+        const int nk = 4;
+        std::vector<double> Z(2 * ng * nk), evk(nk);
+        for (int b = 0; b < nk; ++b) {
+            std::copy(X0.begin() + b * ng, X0.begin() + (b + 1) * ng, Z.begin() + 2 * b * ng);
+            std::copy(X0.begin() + (b + 4) * ng, X0.begin() + (b + 5) * ng, Z.begin() + (2 * b + 1) * ng);
+        }
+        LOBPCG::Eigen_z_multi(Hk, nk, 500, evk.data(), Z.data(), 1e-8);
+        put(evk.data(), nk);
+        put(Z.data(), 2 * ng * nk);
+    }
 
     if (argc > 2) {
         // projector channels from an MBK .vps file: Load(Z, filepath, comm), then ProjectionPP

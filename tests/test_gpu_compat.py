@@ -5,7 +5,8 @@ N, iter_max, ev, X, keep_P, work, keep_S_matrix, SCF_step, OpeA, watch) with the
 Hamiltonian and with a per-band user operator, and EigenLOBPCG_d_multi, all on host arrays.
 Every result is compared with the FP64 oracle on the same seeded inputs; the watch must have
 been driven with the reference's sector ids.  A second integrator takes its channels from an
-MBK .vps file through Load(Z, filepath, comm)."""
+MBK .vps file through Load(Z, filepath, comm).  The k-point operator runs as the reference's
+OpeA(Ax, x) on SoAComplex bands and through LOBPCG::Eigen_z_multi on planar complex bands."""
 import os
 import re
 import subprocess
@@ -42,10 +43,10 @@ def test_compat_surface_against_oracle(gpu, oracle, tmp_path):
     ng = n[0] ** 3
     nb = 8
     d = np.fromfile(str(out), dtype=np.float64)
-    sizes = [ng, ng, 4 * ng, nb, nb * ng, nb, ng, nb, ng]
+    sizes = [ng, ng, 4 * ng, nb, nb * ng, nb, ng, nb, ng, ng, 4, 8 * ng, ng]
     assert d.size == sum(sizes), (d.size, sum(sizes))
     parts = np.split(d, np.cumsum(sizes)[:-1])
-    lap, ppv, hp, ev, X, ev_user, hp_moved, ev_single, ppv_vps = parts
+    lap, ppv, hp, ev, X, ev_user, hp_moved, ev_single, kre, kim, evk, Zk, ppv_vps = parts
     X0 = Q.random_states(ng, nb, h[0] ** 3)
     nat = len(s["positions"])
     onl = oracle.OracleNL(n, h, s["positions"], np.zeros(nat, np.int32), s["projectors"])
@@ -73,6 +74,17 @@ def test_compat_surface_against_oracle(gpu, oracle, tmp_path):
     Xs, Ps = X0.copy(), np.zeros_like(X0)
     es, ss = oracle.lobpcg_hamiltonian(n, h, s["V"], onl, Xs, Ps, 40, tol=np.sqrt(1e-10 * h[0] ** 3))
     assert np.abs(ev_single - es).max() < 1e-8, np.abs(ev_single - es).max()
+    # the k-point operator as OpeA(Ax, x) on SoAComplex bands, and Eigen_z_multi on planar bands
+    k = (0.3, -0.2, 0.25)
+    refk = oracle.kpoint_hamiltonian(n, h, s["V"], onl, k, X0[0] + 1j * X0[1])[0]
+    assert rel(kre + 1j * kim, refk) <= 1e-12
+    Zc = Zk.reshape(4, 2, ng)
+    Zc = Zc[:, 0] + 1j * Zc[:, 1]
+    dv = h[0] ** 3
+    assert np.abs(dv * (Zc.conj() @ Zc.T) - np.eye(4)).max() <= 1e-10
+    Rk = oracle.kpoint_hamiltonian(n, h, s["V"], onl, k, Zc) - evk[:, None] * Zc
+    assert np.sqrt(dv * (np.abs(Rk) ** 2).sum(1)).max() <= 1e-7
+    assert np.all(np.diff(evk) >= 0)
     # Load(Z, filepath, comm) from the .vps file: the channels the reference builds from the
     # arrays written (energy_up, NandL cutoff per l, mean of the two j-components)
     xi = vd["xi"]
