@@ -151,6 +151,23 @@ void Laplasian8th(const G& l_grid, T* out, const T* p, double coef, double dx, d
     so.sync_back();
 }
 
+// Gradient8th_ddm(l_grid, dpsi_dx, dpsi_dy, dpsi_dz, psi, coef, dx, dy, dz)
+// (GridGradient8th.h:17-160): the three coef-scaled 8th-order derivatives of one band (set).
+template <class G>
+void Gradient8th_ddm(const G& l_grid, double* gx, double* gy, double* gz, const double* psi, double coef, double dx,
+                     double dy, double dz) {
+    qg_grid* g = cached_grid(l_grid.SizeX(), l_grid.SizeY(), l_grid.SizeZ(), dx, dy, dz);
+    const size_t ng = (size_t)l_grid.Size3D();
+    Staged<double> sx(gx, ng, false), sy(gy, ng, false), sz(gz, ng, false);
+    Staged<double> sp(const_cast<double*>(psi), ng, true);
+    check(qg_gradient8_apply(g, QG_F64, 1, sp.get(), (int64_t)ng, sx.get(), sy.get(), sz.get(), (int64_t)ng, coef,
+                             nullptr),
+          "qg_gradient8_apply");
+    sx.sync_back();
+    sy.sync_back();
+    sz.sync_back();
+}
+
 // Host-array form kept for existing callers of this header.
 template <class T, class G>
 void Laplasian8thHost(const G& g, T* out, const T* p, double coef, double dx, double dy, double dz) {
@@ -245,6 +262,7 @@ public:
                   "qg_nonlocal_create");
         }
         natoms_ = n;
+        species_ = species;
     }
     // ProjectionPP(Hp, psi, l_grid, nuclei, num_nuclei) (PseudoPotOperator_mpi_v2.h:465):
     // Hp += V_NL psi for one band, with the nuclei of the last UpdatePosition.
@@ -269,12 +287,76 @@ public:
         hip_check(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
         sh.sync_back();
     }
+    // EnergyPPnonlocal(psi, l_grid, nuclei, num_nuclei) (PseudoPotOperator_mpi_v2.h:573-644):
+    // sum_j E_j c_j^2 of one band, c = dV sum beta psi
+    template <class G>
+    double EnergyPPnonlocal(const double* psi, const G& l_grid, const Nucleus* nuclei, int num_nuclei) {
+        (void)nuclei;
+        check_call(l_grid, num_nuclei, "EnergyPPnonlocal");
+        Staged<double> sp(const_cast<double*>(psi), size(), true);
+        const double occ = 1.0;
+        double e = 0.0;
+        check(qg_nonlocal_energy_force(nl_, QG_F64, 1, sp.get(), (int64_t)size(), &occ, &e, nullptr, nullptr),
+              "qg_nonlocal_energy_force");
+        return e;
+    }
+    // ForceNonlocal(forces, psi, dpsi_dx, l_grid, nuclei, num_nuclei) (:905-1013): forces[3a + d]
+    // += sum_j E_j c_j d_j with c = dV sum beta psi and d = dV sum beta dpsi_d, where dpsi_dx holds
+    // the x, y and z gradient blocks one after another, scaled by the caller (-2 occ in
+    // mGetForcePseudoNonlocal, qumasun_mpi.force.h:226-258).  Returns sum_j E_j c_j^2.
+    template <class G>
+    double ForceNonlocal(double* forces, const double* psi, const double* dpsi_dx, const G& l_grid,
+                         const Nucleus* nuclei, int num_nuclei) {
+        (void)nuclei;
+        check_call(l_grid, num_nuclei, "ForceNonlocal");
+        const size_t ng = size();
+        const int ncoef = qg_nonlocal_num_coef(nl_);
+        Staged<double> sp(const_cast<double*>(psi), ng, true);
+        Staged<double> sd(const_cast<double*>(dpsi_dx), 3 * ng, true);
+        double* cdev = nullptr;
+        hip_check(hipMalloc(&cdev, 4 * (size_t)ncoef * sizeof(double)), "hipMalloc");
+        int rc = qg_nonlocal_project(nl_, QG_F64, 1, sp.get(), (int64_t)ng, cdev, nullptr);
+        if (rc == QG_OK) rc = qg_nonlocal_project(nl_, QG_F64, 3, sd.get(), (int64_t)ng, cdev + ncoef, nullptr);
+        std::vector<double> c(4 * (size_t)ncoef);
+        if (rc == QG_OK)
+            rc = hipMemcpy(c.data(), cdev, c.size() * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess ? QG_OK
+                                                                                                          : QG_ERR_HIP;
+        (void)hipFree(cdev);
+        check(rc, "ForceNonlocal");
+        // coefficients in the order atom, projector, m; each projector's KB energy
+        double e = 0.0;
+        int j = 0;
+        for (int a = 0; a < natoms_; ++a)
+            for (const qg_proj_spec& sp_ : elements_[species_[a]].second)
+                for (int m = 0; m < 2 * sp_.l + 1; ++m, ++j) {
+                    e += sp_.energy * c[j] * c[j];
+                    for (int d = 0; d < 3; ++d) forces[3 * a + d] += sp_.energy * c[j] * c[(size_t)(d + 1) * ncoef + j];
+                }
+        return e;
+    }
+    // mGetEnergyPseudoNonlocal + mGetForcePseudoNonlocal in one call on a band block (the
+    // gradients are formed on the device): *energy = sum over bands with occ > 0 of
+    // occ sum E c^2, forces (3 per atom, may be NULL) set, not accumulated.
+    void EnergyForceNonlocal(int nbands, const double* psi, const double* occupancy, double* energy, double* forces) {
+        if (!nl_) throw std::runtime_error("UpdatePosition first");
+        Staged<double> sp(const_cast<double*>(psi), size() * nbands, true);
+        check(qg_nonlocal_energy_force(nl_, QG_F64, nbands, sp.get(), (int64_t)size(), occupancy, energy, forces,
+                                       nullptr),
+              "qg_nonlocal_energy_force");
+    }
     qg_nonlocal* handle() const { return nl_; }
     qg_grid* grid_handle() const { return grid_->get(); }
     size_t size() const { return (size_t)nx_ * ny_ * nz_; }
     double dvol() const { return d_[0] * d_[1] * d_[2]; }
 
 private:
+    template <class G>
+    void check_call(const G& l_grid, int num_nuclei, const char* what) const {
+        if (!nl_) throw std::runtime_error(std::string(what) + ": UpdatePosition first");
+        if ((size_t)l_grid.Size3D() != size()) throw std::runtime_error(std::string(what) + ": grid differs");
+        if (num_nuclei != natoms_) throw std::runtime_error(std::string(what) + ": nuclei differ from UpdatePosition");
+    }
+    std::vector<int> species_;
     int nx_ = 0, ny_ = 0, nz_ = 0, natoms_ = 0;
     double d_[3] = {0, 0, 0};
     std::unique_ptr<Grid> grid_;

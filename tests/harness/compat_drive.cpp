@@ -12,6 +12,8 @@
 //   7. EigenLOBPCG_d_multi(N, dVol, 8, 40, ev, X, keep, work, true, &H, watch)
 //   8. KpointHamiltonianGPU(H, k) as OpeA(Ax, x) on one SoAComplex band, then
 //      LOBPCG::Eigen_z_multi on 4 planar complex bands
+//   9. Gradient8th_ddm, EnergyPPnonlocal, ForceNonlocal on band 0 and EnergyForceNonlocal on
+//      the block
 // Writes every result to argv[1] (raw float64, in that order) and prints the sector counts
 // of the watch.
 #include "qumasun_compat.hpp"
@@ -128,6 +130,27 @@ int main(int argc, char** argv) {
         LOBPCG::Eigen_z_multi(Hk, nk, 500, evk.data(), Z.data(), 1e-8);
         put(evk.data(), nk);
         put(Z.data(), 2 * ng * nk);
+    }
+
+    {
+        // 9. nonlocal energy and forces: Gradient8th_ddm of band 0 scaled by -2 occ, then
+        //    EnergyPPnonlocal and ForceNonlocal on band 0 (occ 0.7), and EnergyForceNonlocal on
+        //    the 8-band block with occupancies 2, 2, 1.5, 1, 0.5, 0, 0.25, 0
+        const double occ0 = 0.7;
+        std::vector<double> grad(3 * ng), fk(24, 0.0);
+        Gradient8th_ddm(g, grad.data(), grad.data() + ng, grad.data() + 2 * ng, X0.data(), -2.0 * occ0, h, h, h);
+        put(grad.data(), 3 * ng);
+        const double e0 = pp.EnergyPPnonlocal(X0.data(), g, nuc, 8);
+        const double ef = pp.ForceNonlocal(fk.data(), X0.data(), grad.data(), g, nuc, 8);
+        put(&e0, 1);
+        put(&ef, 1);
+        put(fk.data(), 24);
+        const double occ[8] = {2.0, 2.0, 1.5, 1.0, 0.5, 0.0, 0.25, 0.0};
+        double eb = 0.0;
+        std::vector<double> fb(24);
+        pp.EnergyForceNonlocal(8, X0.data(), occ, &eb, fb.data());
+        put(&eb, 1);
+        put(fb.data(), 24);
     }
 
     if (argc > 2) {

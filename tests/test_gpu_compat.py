@@ -6,7 +6,9 @@ Hamiltonian and with a per-band user operator, and EigenLOBPCG_d_multi, all on h
 Every result is compared with the FP64 oracle on the same seeded inputs; the watch must have
 been driven with the reference's sector ids.  A second integrator takes its channels from an
 MBK .vps file through Load(Z, filepath, comm).  The k-point operator runs as the reference's
-OpeA(Ax, x) on SoAComplex bands and through LOBPCG::Eigen_z_multi on planar complex bands."""
+OpeA(Ax, x) on SoAComplex bands and through LOBPCG::Eigen_z_multi on planar complex bands;
+Gradient8th_ddm, EnergyPPnonlocal, ForceNonlocal and EnergyForceNonlocal give the nonlocal
+energy and forces."""
 import os
 import re
 import subprocess
@@ -43,10 +45,11 @@ def test_compat_surface_against_oracle(gpu, oracle, tmp_path):
     ng = n[0] ** 3
     nb = 8
     d = np.fromfile(str(out), dtype=np.float64)
-    sizes = [ng, ng, 4 * ng, nb, nb * ng, nb, ng, nb, ng, ng, 4, 8 * ng, ng]
+    sizes = [ng, ng, 4 * ng, nb, nb * ng, nb, ng, nb, ng, ng, 4, 8 * ng, 3 * ng, 1, 1, 24, 1, 24, ng]
     assert d.size == sum(sizes), (d.size, sum(sizes))
     parts = np.split(d, np.cumsum(sizes)[:-1])
-    lap, ppv, hp, ev, X, ev_user, hp_moved, ev_single, kre, kim, evk, Zk, ppv_vps = parts
+    (lap, ppv, hp, ev, X, ev_user, hp_moved, ev_single, kre, kim, evk, Zk, grad, e0, ef, f0, eb, fb,
+     ppv_vps) = parts
     X0 = Q.random_states(ng, nb, h[0] ** 3)
     nat = len(s["positions"])
     onl = oracle.OracleNL(n, h, s["positions"], np.zeros(nat, np.int32), s["projectors"])
@@ -85,6 +88,18 @@ def test_compat_surface_against_oracle(gpu, oracle, tmp_path):
     Rk = oracle.kpoint_hamiltonian(n, h, s["V"], onl, k, Zc) - evk[:, None] * Zc
     assert np.sqrt(dv * (np.abs(Rk) ** 2).sum(1)).max() <= 1e-7
     assert np.all(np.diff(evk) >= 0)
+    # Gradient8th_ddm, EnergyPPnonlocal / ForceNonlocal on one band, EnergyForceNonlocal on the block
+    gref = oracle.gradient8(X0[0], n, h, -2.0 * 0.7)
+    assert rel(grad, np.concatenate(gref)) <= 1e-12
+    sp_ = np.zeros(nat, np.int32)
+    e_one, f_one = oracle.nonlocal_energy_force(n, h, onl, X0[:1], np.array([0.7]), sp_, s["projectors"])
+    assert abs(e0[0] - e_one / 0.7) <= 1e-12 * abs(e_one / 0.7)
+    assert abs(ef[0] - e0[0]) <= 1e-13 * abs(e0[0])
+    assert rel(f0, f_one.reshape(-1)) <= 1e-11
+    occ = np.array([2.0, 2.0, 1.5, 1.0, 0.5, 0.0, 0.25, 0.0])
+    e_blk, f_blk = oracle.nonlocal_energy_force(n, h, onl, X0, occ, sp_, s["projectors"])
+    assert abs(eb[0] - e_blk) <= 1e-12 * abs(e_blk)
+    assert rel(fb, f_blk.reshape(-1)) <= 1e-11
     # Load(Z, filepath, comm) from the .vps file: the channels the reference builds from the
     # arrays written (energy_up, NandL cutoff per l, mean of the two j-components)
     xi = vd["xi"]
