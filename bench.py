@@ -290,6 +290,31 @@ def cpu_baseline(s, psi, nb_one, nb_all):
                          "sample": f"{nb_one} bands on 1 thread, {dt1:.2f} s"}}
 
 
+def eigen_cpu_sample(s, X0, nb=32, iters=2):
+    """The FP64 oracle's LOBPCG (the C++ restatement of the reference's Eigen_d_multi loop) on a
+    bounded sample of the same problem: nb bands of the seeded start, iters iterations; dense
+    products through OpenBLAS on the host threads, H psi band by band on one thread."""
+    import oracle as O
+    lib = O.olib()
+    th = host_threads()
+    lib.or_set_threads(th)
+    n = (s["n"],) * 3
+    h = (s["h"],) * 3
+    nat = len(s["positions"])
+    onl = O.OracleNL(n, h, s["positions"], np.zeros(nat, np.int32), s["projectors"])
+    X = np.ascontiguousarray(X0[:nb])
+    P = np.zeros_like(X)
+    t0 = time.perf_counter()
+    _, st = O.lobpcg_hamiltonian(n, h, s["V"], onl, X, P, iters)
+    dt = time.perf_counter() - t0
+    lib.or_set_threads(1)
+    return {"value": round(dt / max(1, st["iterations"]), 3), "unit": "s per LOBPCG iteration", "cores": th,
+            "kind": "port",
+            "sample": f"FP64 oracle LOBPCG, {nb} bands of the seeded start on the {s['n']}^3 / {nat}-atom cell, "
+                      f"{st['iterations']} iterations in {dt:.1f} s (dense products on {th} OpenBLAS threads, "
+                      f"H psi on one thread) on {cpu_model()}"}
+
+
 def eigen_leg(cfg, budget, compare):
     """Seconds to eigenvalues within 1e-6 Ha of the FP64 reference."""
     import torch
@@ -399,6 +424,10 @@ def eigen_leg(cfg, budget, compare):
                                           "note": "the network saves iterations; its apply costs more than they do"}
     if ok_k:
         res["speedup_fp32_kinetic_vs_plain_fp64"] = ratio(t_p, ok_p, t_k) if ok_l else None
+    if cfg == "cfg2":
+        log("eigen cfg2: CPU sample (FP64 oracle LOBPCG)")
+        res["cpu_baseline"] = eigen_cpu_sample(s, X0)
+        res["cpu_baseline"]["gpu_plain_fp64_s_per_iteration_528_bands"] = round(t_p / max(1, it_p), 3)
     return res
 
 
