@@ -209,7 +209,8 @@ int qg_kpoint_apply(qg_kpoint* kp, int dtype, int nbands, const void* psi, int64
  * Hermitian Rayleigh-Ritz with rocSOLVER zheevd).  X: nbands QG_Z128 bands (ld = Ng), the start
  * on entry and the eigenvectors on return (dV-orthonormal); eig_host: nbands values.  Stops when
  * the largest residual norm sqrt(dV sum |H x - theta x|^2) over the lowest nconv bands (nconv <= 0:
- * all) is <= tol, or after iter_max iterations.  No preconditioner (the reference's). */
+ * all) is <= tol, or after iter_max iterations.  No preconditioner (the reference's).  Device
+ * workspace: 9 blocks of nbands complex FP64 bands ([X W P], their H images, three work blocks). */
 int qg_kpoint_lobpcg(qg_kpoint* kp, int nbands, void* X, int64_t ld, double* eig_host, int iter_max,
                      double tol, int nconv, int* iters_out, double* max_res_out, void* stream);
 

@@ -441,7 +441,10 @@ class KpointHamiltonian:
 
     def __del__(self):
         if getattr(self, "_h", None):
-            lib().qg_kpoint_destroy(self._h)
+            try:
+                lib().qg_kpoint_destroy(self._h)
+            except Exception:   # interpreter shutdown: the module's globals may be gone
+                pass
             self._h = None
 
 

@@ -563,10 +563,28 @@ __global__ void colnorm2_kernel(const double2* V, int64_t K, double* out) {
     }
 }
 
-// T[:, dst] = U[:, j] / sqrt(lambda_j) for the kept eigenvalues (lambda_j > tau lambda_max),
+// SVQB's scaling of the Gram matrix G (n x n, in place) to unit diagonal: d_i = G_ii^-1/2 (0 for a
+// zero column), G <- D G D.  Residual columns span many orders of magnitude as bands converge;
+// the eigensolve and the relative drop threshold see the scaled matrix.  One workgroup.
+__global__ void svqb_scale_kernel(double2* G, int n, double* d) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double gii = G[(int64_t)i * n + i].x;
+        d[i] = gii > 0.0 ? 1.0 / sqrt(gii) : 0.0;
+    }
+    __syncthreads();
+    for (int64_t t = threadIdx.x; t < (int64_t)n * n; t += blockDim.x) {
+        const int i = (int)(t % n), j = (int)(t / n);
+        const double sc = d[i] * d[j];
+        G[t].x *= sc;
+        G[t].y *= sc;
+    }
+}
+
+// T[:, dst] = D U[:, j] / sqrt(lambda_j) for the kept eigenvalues (lambda_j > tau lambda_max),
 // packed to the front largest first (dst = kept columns above j); one workgroup per source
 // column; workgroup 0 writes the kept count
-__global__ void svqb_pack_kernel(const double2* U, const double* lam, int n, double tau, double2* T, int* keep) {
+__global__ void svqb_pack_kernel(const double2* U, const double* lam, const double* d, int n, double tau, double2* T,
+                                 int* keep) {
     const int j = blockIdx.x;
     const double lmax = lam[n - 1];
     auto kept = [&](int c) { return lam[c] > tau * lmax && lam[c] > 0.0; };
@@ -581,7 +599,22 @@ __global__ void svqb_pack_kernel(const double2* U, const double* lam, int n, dou
     const double sc = 1.0 / sqrt(lam[j]);
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const double2 u = U[(int64_t)j * n + i];
-        T[(int64_t)dst * n + i] = double2{u.x * sc, u.y * sc};
+        T[(int64_t)dst * n + i] = double2{u.x * sc * d[i], u.y * sc * d[i]};
+    }
+}
+
+constexpr int kGramChunks = 64;
+
+// G[t] = sum over chunks of part[c][t]
+__global__ void sum_partials_kernel(const double2* __restrict__ part, int nc, int64_t mn, double2* __restrict__ G) {
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < mn; t += (int64_t)gridDim.x * blockDim.x) {
+        double2 acc{0.0, 0.0};
+        for (int c = 0; c < nc; ++c) {
+            const double2 v = part[(int64_t)c * mn + t];
+            acc.x += v.x;
+            acc.y += v.y;
+        }
+        G[t] = acc;
     }
 }
 
@@ -592,8 +625,8 @@ struct KLobpcg {
     int64_t K;
     double dv;
     DevBuf<double2> U, Gm, Tm;   // small matrices
-    DevBuf<double> lam, E;
-    DevBuf<int> info;
+    DevBuf<double> lam, E, dsc;
+    DevBuf<int> info, keep;   // rocSOLVER status; kept-column count of SVQB
 
     void gemm(rocblas_operation ta, rocblas_operation tb, int m, int n, int k, double alpha, const double2* A, int64_t lda,
               const double2* B, int64_t ldb, double beta, double2* Cm, int64_t ldc) {
@@ -601,12 +634,32 @@ struct KLobpcg {
         QG_KBLAS(rocblas_zgemm(hb, ta, tb, m, n, k, &al, (const zc*)A, (int)lda, (const zc*)B, (int)ldb, &be, (zc*)Cm,
                                (int)ldc));
     }
-    // G (m x n) = dV A^H B, A: K x m, B: K x n
+    // G (m x n) = dV A^H B, A: K x m, B: K x n.  The products are skinny (m, n <= 3N, K = Ng), so
+    // K is split into chunks, each chunk's product one batch entry of a strided-batched zgemm,
+    // and the partial products summed: a single zgemm would keep only a few workgroups busy.
+    DevBuf<double2> part;
     void gram(const double2* A, int m, const double2* B, int n, double2* G) {
-        gemm(rocblas_operation_conjugate_transpose, rocblas_operation_none, m, n, (int)K, dv, A, K, B, K, 0.0, G, m);
+        const int64_t kc = std::max<int64_t>(4096, (K + kGramChunks - 1) / kGramChunks);
+        const int nc = (int)((K + kc - 1) / kc);
+        const int64_t last = K - (int64_t)(nc - 1) * kc;
+        const int64_t mn = (int64_t)m * n;
+        part.alloc((size_t)nc * mn);
+        const zc al{dv, 0.0}, be{0.0, 0.0};
+        if (nc > 1)
+            QG_KBLAS(rocblas_zgemm_strided_batched(hb, rocblas_operation_conjugate_transpose, rocblas_operation_none, m, n,
+                                                   (int)kc, &al, (const zc*)A, (int)K, kc, (const zc*)B, (int)K, kc,
+                                                   &be, (zc*)part.p, m, mn, nc - 1));
+        // the last chunk (ragged) as one more product into its partial slot
+        QG_KBLAS(rocblas_zgemm(hb, rocblas_operation_conjugate_transpose, rocblas_operation_none, m, n, (int)last, &al,
+                               (const zc*)(A + (nc - 1) * kc), (int)K, (const zc*)(B + (nc - 1) * kc), (int)K, &be,
+                               (zc*)(part.p + (int64_t)(nc - 1) * mn), m));
+        hipLaunchKernelGGL(sum_partials_kernel, dim3((unsigned)std::min<int64_t>((mn + 255) / 256, 4096)), dim3(256), 0,
+                           s, part.p, nc, mn, G);
+        QG_HIP(hipGetLastError());
     }
     int heevd(double2* A, int n, double* w) {
         E.alloc(n);
+        QG_HIP(hipMemsetAsync(info.p, 0, sizeof(int), s));
         QG_KBLAS(rocsolver_zheevd(hb, rocblas_evect_original, rocblas_fill_upper, n, (zc*)A, n, w, E.p, info.p));
         int inf = 0;
         QG_HIP(hipMemcpyAsync(&inf, info.p, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -626,21 +679,25 @@ struct KLobpcg {
             if (AV) gemm(rocblas_operation_none, rocblas_operation_none, (int)K, n, q, -1.0, AQ, K, Gm.p, q, 1.0, AV, K);
         }
     }
-    // orthonormalise the n columns of V (and AV alongside) by SVQB, dropping directions below
-    // tau of the largest; the kept columns are packed to the front of V (and of AV).  Work:
-    // W (K x n).  Returns the number kept.
+    // orthonormalise the n columns of V (and AV alongside) by SVQB on the unit-diagonal scaled
+    // Gram matrix, dropping directions below tau of the largest; the kept columns are packed to
+    // the front of V (and of AV).  Work: W (K x n).  Returns the number kept.
     int svqb(double2* V, double2* AV, int n, double2* W) {
         if (n == 0) return 0;
         Gm.alloc((size_t)n * n);
         U.alloc((size_t)n * n);
         Tm.alloc((size_t)n * n);
         lam.alloc(n);
-        gram(V, n, V, n, Gm.p);
-        QG_HIP(hipMemcpyAsync(U.p, Gm.p, (size_t)n * n * sizeof(double2), hipMemcpyDeviceToDevice, s));
-        QG_REQUIRE(heevd(U.p, n, lam.p) == 0, "k-point LOBPCG: Gram eigensolve failed");
-        hipLaunchKernelGGL(svqb_pack_kernel, dim3(n), dim3(256), 0, s, U.p, lam.p, n, 1e-13, Tm.p, info.p);
+        dsc.alloc(n);
+        gram(V, n, V, n, U.p);
+        hipLaunchKernelGGL(svqb_scale_kernel, dim3(1), dim3(256), 0, s, U.p, n, dsc.p);
+        const int inf = heevd(U.p, n, lam.p);
+        QG_REQUIRE(inf == 0, "k-point LOBPCG: Gram eigensolve failed (zheevd info " + std::to_string(inf) +
+                                 ", n " + std::to_string(n) + ")");
+        keep.alloc(1);
+        hipLaunchKernelGGL(svqb_pack_kernel, dim3(n), dim3(256), 0, s, U.p, lam.p, dsc.p, n, 1e-13, Tm.p, keep.p);
         int m = 0;
-        QG_HIP(hipMemcpyAsync(&m, info.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        QG_HIP(hipMemcpyAsync(&m, keep.p, sizeof(int), hipMemcpyDeviceToHost, s));
         QG_HIP(hipStreamSynchronize(s));
         if (m == 0) return 0;
         gemm(rocblas_operation_none, rocblas_operation_none, (int)K, m, n, 1.0, V, K, Tm.p, n, 0.0, W, K);
@@ -667,11 +724,12 @@ int kpoint_lobpcg(qg_kpoint* kp, int N, double2* X, int64_t ld, double* eig_host
     const size_t blk = (size_t)K * N;
     // S = [X W P] and AS = [AX AW AP], contiguous so that one product forms the Rayleigh-Ritz
     // matrix; T1, T2 work blocks
-    DevBuf<double2> S, AS, T1, T2;
+    DevBuf<double2> S, AS, T1, T2, T3;
     S.alloc(3 * blk);
     AS.alloc(3 * blk);
     T1.alloc(blk);
     T2.alloc(blk);
+    T3.alloc(blk);   // AP while P is orthonormalised
     double2* Sx = S.p;
     double2* ASx = AS.p;
     QG_HIP(hipMemcpyAsync(Sx, X, blk * sizeof(double2), hipMemcpyDeviceToDevice, s));
@@ -685,7 +743,8 @@ int kpoint_lobpcg(qg_kpoint* kp, int N, double2* X, int64_t ld, double* eig_host
     theta.alloc(3 * N);
     nrm.alloc(N);
     L.gram(Sx, N, ASx, N, A.p);
-    QG_REQUIRE(L.heevd(A.p, N, theta.p) == 0, "k-point LOBPCG: Rayleigh-Ritz eigensolve failed");
+    const int inf0 = L.heevd(A.p, N, theta.p);
+    QG_REQUIRE(inf0 == 0, "k-point LOBPCG: Rayleigh-Ritz eigensolve failed (zheevd info " + std::to_string(inf0) + ")");
     L.gemm(rocblas_operation_none, rocblas_operation_none, (int)K, N, N, 1.0, Sx, K, A.p, N, 0.0, T1.p, K);
     L.gemm(rocblas_operation_none, rocblas_operation_none, (int)K, N, N, 1.0, ASx, K, A.p, N, 0.0, T2.p, K);
     QG_HIP(hipMemcpyAsync(Sx, T1.p, blk * sizeof(double2), hipMemcpyDeviceToDevice, s));
@@ -710,11 +769,8 @@ int kpoint_lobpcg(qg_kpoint* kp, int N, double2* X, int64_t ld, double* eig_host
         // W: the residuals with X (and P) projected out, orthonormalised
         // P occupies the slot after the kept W columns; move it out of the way first
         double2* Pcur = T2.p;
-        double2* APcur = nullptr;
-        DevBuf<double2> APtmp;
+        double2* APcur = T3.p;
         if (np > 0) {
-            APtmp.alloc((size_t)K * np);
-            APcur = APtmp.p;
             QG_HIP(hipMemcpyAsync(Pcur, Sx + (int64_t)K * 2 * N, (size_t)K * np * sizeof(double2),
                                   hipMemcpyDeviceToDevice, s));
             QG_HIP(hipMemcpyAsync(APcur, ASx + (int64_t)K * 2 * N, (size_t)K * np * sizeof(double2),
@@ -740,7 +796,9 @@ int kpoint_lobpcg(qg_kpoint* kp, int N, double2* X, int64_t ld, double* eig_host
         const int m = N + mw + mp;
         // Rayleigh-Ritz on the orthonormal basis S (m columns)
         L.gram(Sx, m, ASx, m, A.p);
-        QG_REQUIRE(L.heevd(A.p, m, theta.p) == 0, "k-point LOBPCG: Rayleigh-Ritz eigensolve failed");
+        const int infm = L.heevd(A.p, m, theta.p);
+        QG_REQUIRE(infm == 0, "k-point LOBPCG: Rayleigh-Ritz eigensolve failed (zheevd info " + std::to_string(infm) +
+                                  ", n " + std::to_string(m) + ")");
         // new P = S[:, N:m] C[N:m, :N], AP alike (into T1 / T2), then X = S C[:, :N]
         const int mr = m - N;
         if (mr > 0) {
