@@ -151,3 +151,30 @@ def test_kpoint_time_reversal(gpu):
         eig, it, res = Q.KpointHamiltonian(H, kk).Eigen_z_multi(X, iter_max=400, tol=1e-8, nconv=5)
         out.append(eig[:5])
     assert np.abs(out[0] - out[1]).max() <= 1e-9
+
+
+def test_kpoint_lobpcg_64atoms_80cube(gpu):
+    """40 bands of the 64-atom 80^3 cell at k = (pi/L)(1, 1, 1), 30 iterations: the block stays
+    dV-orthonormal and the returned eigenvalues are the Rayleigh quotients of the returned
+    vectors under the applied operator (the size at which residual columns of very different
+    norms reach the orthonormalisation)."""
+    import torch
+    print("kpoint lobpcg 64 atoms 80^3 40 bands", flush=True)
+    s = Q.diamond_system(2, 40)
+    n = (s["n"],) * 3
+    h = (s["h"],) * 3
+    g, integ, H = _engine(n, h, s["positions"], s["projectors"], s["V"])
+    kk = math.pi / s["L"]
+    Hk = Q.KpointHamiltonian(H, (kk, kk, kk))
+    rng = np.random.default_rng(5)
+    X = torch.from_numpy(rng.standard_normal((40, g.size)) + 1j * rng.standard_normal((40, g.size))).to(gpu)
+    eig, it, res = Hk.Eigen_z_multi(X.contiguous(), iter_max=30, tol=0.0, nconv=32)
+    assert it == 30
+    dv = g.dvol
+    G = (X.conj() @ X.T) * dv
+    assert float((G - torch.eye(40, dtype=G.dtype, device=gpu)).abs().max()) <= 1e-10
+    out = torch.empty_like(X)
+    Hk.apply(X, out)
+    rq = ((X.conj() * out).sum(1).real * dv).cpu().numpy()
+    assert np.abs(rq - eig).max() <= 1e-9 * np.abs(eig).max()
+    assert np.all(np.diff(eig) >= -1e-12)
