@@ -567,9 +567,10 @@ namespace LOBPCG {
 // lowest num_solution eigenpairs of H_k by the engine's complex FP64 LOBPCG, from the planar
 // complex start X (num_solution bands of 2 ng doubles, host or device), overwritten with the
 // eigenvectors.  Stops at tol (largest residual norm) or iter_max iterations; returns the
-// iterations run.
+// iterations run.  precond_shift > 0 preconditions with (T_k + shift)^-1 (not in the reference).
 inline int Eigen_z_multi(KpointHamiltonianGPU& Hk, int num_solution, int iter_max, double* eigen_values, double* X,
-                         double tol, double* max_residual = nullptr, hipStream_t stream = nullptr) {
+                         double tol, double* max_residual = nullptr, hipStream_t stream = nullptr,
+                         double precond_shift = 0.0) {
     const size_t ng = Hk.size();
     Staged<double> sx(X, 2 * ng * num_solution, true);
     void* z = nullptr;
@@ -578,8 +579,8 @@ inline int Eigen_z_multi(KpointHamiltonianGPU& Hk, int num_solution, int iter_ma
     double res = 0.0;
     int rc = qg_complex_from_planar((int64_t)ng, num_solution, sx.get(), QG_Z128, z, stream);
     if (rc == QG_OK)
-        rc = qg_kpoint_lobpcg(Hk.get(), num_solution, z, (int64_t)ng, eigen_values, iter_max, tol, 0, &iters, &res,
-                              stream);
+        rc = qg_kpoint_lobpcg(Hk.get(), num_solution, z, (int64_t)ng, eigen_values, iter_max, tol, 0, precond_shift,
+                              &iters, &res, stream);
     if (rc == QG_OK) rc = qg_complex_to_planar((int64_t)ng, num_solution, z, QG_Z128, sx.get(), stream);
     if (rc == QG_OK) rc = hipStreamSynchronize(stream) == hipSuccess ? QG_OK : QG_ERR_HIP;
     (void)hipFree(z);

@@ -209,10 +209,14 @@ int qg_kpoint_apply(qg_kpoint* kp, int dtype, int nbands, const void* psi, int64
  * Hermitian Rayleigh-Ritz with rocSOLVER zheevd).  X: nbands QG_Z128 bands (ld = Ng), the start
  * on entry and the eigenvectors on return (dV-orthonormal); eig_host: nbands values.  Stops when
  * the largest residual norm sqrt(dV sum |H x - theta x|^2) over the lowest nconv bands (nconv <= 0:
- * all) is <= tol, or after iter_max iterations.  No preconditioner (the reference's).  Device
- * workspace: 9 blocks of nbands complex FP64 bands ([X W P], their H images, three work blocks). */
+ * all) is <= tol, or after iter_max iterations.  precond_shift <= 0: no preconditioner (the
+ * reference's iteration); > 0: the residuals are preconditioned by (T_k + precond_shift)^-1, the
+ * exact inverse of the shifted k-point kinetic operator (complex FFTs, rocFFT; no counterpart in
+ * the reference).  Device workspace: 9 blocks of nbands complex FP64 bands ([X W P], their H
+ * images, three work blocks). */
 int qg_kpoint_lobpcg(qg_kpoint* kp, int nbands, void* X, int64_t ld, double* eig_host, int iter_max,
-                     double tol, int nconv, int* iters_out, double* max_res_out, void* stream);
+                     double tol, int nconv, double precond_shift, int* iters_out, double* max_res_out,
+                     void* stream);
 
 /* Layout conversion between the reference's complex band blocks (per band ng real parts then
  * ng imaginary parts, FP64: the SoAComplex arrays of Eigen_z_multi_mpi's IntervalPointers,

@@ -423,9 +423,10 @@ class KpointHamiltonian:
         check(lib().qg_kpoint_apply(self._h, _kdtype(psi), nb, pp, ld, op, ldo, _stream()))
         return out
 
-    def Eigen_z_multi(self, X, iter_max=300, tol=1e-6, nconv=0):
+    def Eigen_z_multi(self, X, iter_max=300, tol=1e-6, nconv=0, precond_shift=0.0):
         """Lowest eigenpairs by complex FP64 LOBPCG (Eigen_z_multi_mpi, lobpcg_z_multi_mpi.h:988):
-        X (nbands, Ng) complex128 start, overwritten with the eigenvectors.  Returns
+        X (nbands, Ng) complex128 start, overwritten with the eigenvectors; precond_shift > 0
+        preconditions the residuals by (T_k + shift)^-1 (the reference runs without).  Returns
         (eigenvalues ndarray, iterations, largest residual norm)."""
         import torch
         if X.dtype != torch.complex128:
@@ -436,7 +437,7 @@ class KpointHamiltonian:
         it = ctypes.c_int()
         res = ctypes.c_double()
         check(lib().qg_kpoint_lobpcg(self._h, nb, pp, ld, dptr(eig), int(iter_max), float(tol), int(nconv),
-                                     ctypes.byref(it), ctypes.byref(res), _stream()))
+                                     float(precond_shift), ctypes.byref(it), ctypes.byref(res), _stream()))
         return eig, it.value, res.value
 
     def __del__(self):

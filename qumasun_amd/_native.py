@@ -111,8 +111,8 @@ SIGNATURES = {
     "qg_kpoint_create": (c_int, [c_void_p, c_double, c_double, c_double, P(c_void_p)]),
     "qg_kpoint_destroy": (c_int, [c_void_p]),
     "qg_kpoint_apply": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, c_void_p, c_int64, c_void_p]),
-    "qg_kpoint_lobpcg": (c_int, [c_void_p, c_int, c_void_p, c_int64, P(c_double), c_int, c_double, c_int, P(c_int),
-                                 P(c_double), c_void_p]),
+    "qg_kpoint_lobpcg": (c_int, [c_void_p, c_int, c_void_p, c_int64, P(c_double), c_int, c_double, c_int, c_double,
+                                 P(c_int), P(c_double), c_void_p]),
     "qg_gradient8_apply": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int64,
                                    c_double, c_void_p]),
     "qg_nonlocal_energy_force": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, P(c_double), P(c_double),

@@ -25,7 +25,11 @@
 #include <vector>
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
+#include <rocfft/rocfft.h>
 #include <rocsolver/rocsolver.h>
+#include <map>
+#include <memory>
+#include <mutex>
 #include "qg_internal.hpp"
 
 #define QG_KBLAS(call)                                                                       \
@@ -34,6 +38,25 @@
         if (st_ != rocblas_status_success)                                                   \
             throw ::qg::Error{QG_ERR_LIB, std::string(#call) + ": " + rocblas_status_to_string(st_)}; \
     } while (0)
+
+#define QG_KFFT(call)                                                                          \
+    do {                                                                                       \
+        rocfft_status st_ = (call);                                                            \
+        if (st_ != rocfft_status_success)                                                      \
+            throw ::qg::Error{QG_ERR_LIB, std::string(#call) + " failed: " + std::to_string((int)st_)}; \
+    } while (0)
+
+// complex-to-complex FFT plans of one batch size for the k-point kinetic preconditioner
+struct KFftPlan {
+    rocfft_plan fwd = nullptr, inv = nullptr;
+    rocfft_execution_info info = nullptr;
+    qg::DevBuf<unsigned char> work;
+    ~KFftPlan() {
+        if (fwd) rocfft_plan_destroy(fwd);
+        if (inv) rocfft_plan_destroy(inv);
+        if (info) rocfft_execution_info_destroy(info);
+    }
+};
 
 struct qg_kpoint {
     qg_ham* h = nullptr;   // not owned
@@ -46,6 +69,11 @@ struct qg_kpoint {
     // application runs one launch per colour without atomics
     std::vector<int> color_off;   // atoms of colour c: color_atoms[color_off[c] .. color_off[c+1])
     qg::DevBuf<int> color_atoms;
+    // kinetic preconditioner: plans per batch size, spectrum buffer, per-axis symbol parts
+    std::map<int, std::unique_ptr<KFftPlan>> fft;
+    qg::DevBuf<double2> spec;
+    qg::DevBuf<double> tx, ty, tz;
+    double tmin = 0.0;   // smallest value of the symbol over all modes
     // LOBPCG state
     rocblas_handle blas = nullptr;
     ~qg_kpoint() {
@@ -512,6 +540,99 @@ void check_kblock(const qg_kpoint* kp, int dtype, int nbands, const void* psi, i
     }
 }
 
+// ----------------------------------------------------------- kinetic preconditioner
+// K = (T_k + s)^-1 with T_k = -1/2 [L8 - |k|^2 + 2 i k . D8], the k-point kinetic operator of
+// the stencil kernel.  On the periodic grid it is diagonal in the discrete Fourier basis: for the
+// mode exp(i theta_d j_d) per axis, T_k = sum_d t_d(theta_d) with
+//   t_d = -1/2 h^-2 [w0 + 2 sum_m w_m cos(m theta)] + 1/2 k_d^2 + k_d (2 / (840 h)) sum_m c_m sin(m theta),
+// (w0..w4) = (-205/72, 8/5, -1/5, 8/315, -1/560), (c1..c4) = (672, -168, 32, -3), so K is one
+// complex FFT, a pointwise scale by 1 / (T_k + s) / Ng and the inverse FFT, batched over bands.
+static std::vector<double> kinetic_symbol_k(int n, double h, double k) {
+    const double w[5] = {-205.0 / 72.0, 8.0 / 5.0, -1.0 / 5.0, 8.0 / 315.0, -1.0 / 560.0};
+    const double c[4] = {672.0, -168.0, 32.0, -3.0};
+    std::vector<double> t(n);
+    for (int q = 0; q < n; ++q) {
+        const double th = 2.0 * M_PI * q / n;
+        double s2 = w[0], s1 = 0.0;
+        for (int m = 1; m <= 4; ++m) {
+            s2 += 2.0 * w[m] * std::cos(m * th);
+            s1 += c[m - 1] * std::sin(m * th);
+        }
+        t[q] = -0.5 * s2 / (h * h) + 0.5 * k * k + k * 2.0 * s1 / (840.0 * h);
+    }
+    return t;
+}
+
+__global__ void kinetic_scale_c2c_kernel(double2* spec, int nx, int ny, int nz, int nb, const double* tx,
+                                         const double* ty, const double* tz, double shift, double inv_ng) {
+    const int64_t per = (int64_t)nx * ny * nz;
+    const int64_t total = per * nb;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = i % per;
+        const int kx = (int)(j % nx);
+        const int ky = (int)((j / nx) % ny);
+        const int kz = (int)(j / ((int64_t)nx * ny));
+        const double f = inv_ng / (tx[kx] + ty[ky] + tz[kz] + shift);
+        spec[i].x *= f;
+        spec[i].y *= f;
+    }
+}
+
+void ensure_rocfft() {
+    static std::once_flag once;
+    std::call_once(once, [] { rocfft_setup(); });
+}
+
+// R <- K R for nb complex FP64 bands (ld = Ng)
+void kinetic_precond(qg_kpoint* kp, int nb, double2* R, double shift, hipStream_t s) {
+    ensure_rocfft();
+    const qg_grid& g = kp->h->grid;
+    const int64_t ng = g.ng();
+    if (!kp->tx.p) {
+        auto a = kinetic_symbol_k(g.n[0], g.h[0], kp->k[0]);
+        auto b = kinetic_symbol_k(g.n[1], g.h[1], kp->k[1]);
+        auto c = kinetic_symbol_k(g.n[2], g.h[2], kp->k[2]);
+        kp->tmin = *std::min_element(a.begin(), a.end()) + *std::min_element(b.begin(), b.end()) +
+                   *std::min_element(c.begin(), c.end());
+        kp->tx.upload(a.data(), a.size());
+        kp->ty.upload(b.data(), b.size());
+        kp->tz.upload(c.data(), c.size());
+    }
+    QG_REQUIRE(kp->tmin + shift > 0.0, "k-point preconditioner: T_k + shift is not positive on every mode");
+    // bands per FFT batch: about 256 MB of spectrum
+    const int batch = (int)std::max<int64_t>(1, std::min<int64_t>(nb, (256ll << 20) / (ng * 16)));
+    kp->spec.alloc((size_t)batch * ng);
+    for (int b0 = 0; b0 < nb; b0 += batch) {
+        const int nc = std::min(batch, nb - b0);
+        auto& slot = kp->fft[nc];
+        if (!slot) {
+            slot.reset(new KFftPlan);
+            const size_t len[3] = {(size_t)g.n[0], (size_t)g.n[1], (size_t)g.n[2]};
+            QG_KFFT(rocfft_plan_create(&slot->fwd, rocfft_placement_notinplace, rocfft_transform_type_complex_forward,
+                                       rocfft_precision_double, 3, len, nc, nullptr));
+            QG_KFFT(rocfft_plan_create(&slot->inv, rocfft_placement_notinplace, rocfft_transform_type_complex_inverse,
+                                       rocfft_precision_double, 3, len, nc, nullptr));
+            QG_KFFT(rocfft_execution_info_create(&slot->info));
+            size_t w1 = 0, w2 = 0;
+            QG_KFFT(rocfft_plan_get_work_buffer_size(slot->fwd, &w1));
+            QG_KFFT(rocfft_plan_get_work_buffer_size(slot->inv, &w2));
+            slot->work.alloc(std::max<size_t>(1, std::max(w1, w2)));
+        }
+        QG_KFFT(rocfft_execution_info_set_stream(slot->info, s));
+        QG_KFFT(rocfft_execution_info_set_work_buffer(slot->info, slot->work.p, slot->work.n));
+        void* in[1] = {R + (int64_t)b0 * ng};
+        void* out[1] = {kp->spec.p};
+        QG_KFFT(rocfft_execute(slot->fwd, in, out, slot->info));
+        hipLaunchKernelGGL(kinetic_scale_c2c_kernel, dim3((unsigned)std::min<int64_t>((ng * nc + 255) / 256, 8192)),
+                           dim3(256), 0, s, kp->spec.p, g.n[0], g.n[1], g.n[2], nc, kp->tx.p, kp->ty.p, kp->tz.p, shift,
+                           1.0 / (double)ng);
+        QG_HIP(hipGetLastError());
+        void* in2[1] = {kp->spec.p};
+        void* out2[1] = {R + (int64_t)b0 * ng};
+        QG_KFFT(rocfft_execute(slot->inv, in2, out2, slot->info));
+    }
+}
+
 // planar FP64 bands [re(ng) | im(ng)] (the reference's SoAComplex blocks) <-> interleaved complex
 template <class C>
 __global__ void planar_to_z_kernel(int64_t ng, int nb, const double* __restrict__ planar, C* __restrict__ z) {
@@ -711,7 +832,7 @@ struct KLobpcg {
 };
 
 int kpoint_lobpcg(qg_kpoint* kp, int N, double2* X, int64_t ld, double* eig_host, int iter_max, double tol, int nconv,
-                  int* iters_out, double* max_res_out, hipStream_t s) {
+                  double precond_shift, int* iters_out, double* max_res_out, hipStream_t s) {
     qg_ham* h = kp->h;
     const int64_t K = h->grid.ng();
     QG_REQUIRE(ld == K, "k-point LOBPCG: X must be contiguous bands (ld = Ng)");
@@ -777,6 +898,7 @@ int kpoint_lobpcg(qg_kpoint* kp, int N, double2* X, int64_t ld, double* eig_host
                                   hipMemcpyDeviceToDevice, s));
         }
         QG_HIP(hipMemcpyAsync(Sw, T1.p, blk * sizeof(double2), hipMemcpyDeviceToDevice, s));
+        if (precond_shift > 0.0) kinetic_precond(kp, N, Sw, precond_shift, s);   // W = K R
         L.project_out(Sx, nullptr, N, Sw, nullptr, N);
         const int mw = L.svqb(Sw, nullptr, N, T1.p);
         L.apply(Sw, ASw, mw);
@@ -910,12 +1032,12 @@ int qg_complex_to_planar(int64_t ng, int nbands, const void* z, int dtype, doubl
 }
 
 int qg_kpoint_lobpcg(qg_kpoint* kp, int nbands, void* X, int64_t ld, double* eig_host, int iter_max, double tol,
-                     int nconv, int* iters_out, double* max_res_out, void* stream) {
+                     int nconv, double precond_shift, int* iters_out, double* max_res_out, void* stream) {
     return guarded([&] {
         QG_REQUIRE(kp && kp->h && X && eig_host, "null pointer");
         QG_REQUIRE(iter_max >= 0, "negative iteration limit");
-        kpoint_lobpcg(kp, nbands, (double2*)X, ld, eig_host, iter_max, tol, nconv, iters_out, max_res_out,
-                      as_stream(stream));
+        kpoint_lobpcg(kp, nbands, (double2*)X, ld, eig_host, iter_max, tol, nconv, precond_shift, iters_out,
+                      max_res_out, as_stream(stream));
     });
 }
 

@@ -178,3 +178,27 @@ def test_kpoint_lobpcg_64atoms_80cube(gpu):
     rq = ((X.conj() * out).sum(1).real * dv).cpu().numpy()
     assert np.abs(rq - eig).max() <= 1e-9 * np.abs(eig).max()
     assert np.all(np.diff(eig) >= -1e-12)
+
+
+def test_kpoint_lobpcg_preconditioned(gpu, oracle):
+    """The kinetic preconditioner (T_k + 0.5)^-1 (an extension: the reference iterates without
+    one): the same eigenvalues as the dense solve, in fewer iterations than the plain iteration
+    from the same start."""
+    import torch
+    n, h, k = (12, 10, 14), (0.35, 0.4, 0.3), (0.4, -0.25, 0.3)
+    pos, projs, V = _cell(n, h)
+    g, integ, H = _engine(n, h, pos, projs, V)
+    onl = oracle.OracleNL(n, h, pos, np.zeros(2, np.int32), projs)
+    e_ref = np.linalg.eigvalsh(oracle.kpoint_dense(n, h, V, onl, k))
+    rng = np.random.default_rng(11)
+    X0 = rng.standard_normal((10, g.size)) + 1j * rng.standard_normal((10, g.size))
+    Hk = Q.KpointHamiltonian(H, k)
+    its = {}
+    for shift in (0.0, 0.5):
+        X = torch.from_numpy(X0.copy()).to(gpu).contiguous()
+        eig, it, res = Hk.Eigen_z_multi(X, iter_max=400, tol=1e-8, nconv=6, precond_shift=shift)
+        print(f"  shift {shift}: {it} iterations, residual {res:.1e}", flush=True)
+        assert res <= 1e-8
+        assert np.abs(eig[:6] - e_ref[:6]).max() <= 1e-9
+        its[shift] = it
+    assert its[0.5] < its[0.0], its
