@@ -17,7 +17,8 @@ roofline  = the same bytes over the summed duration of all H psi launches of a s
             summary of this workload, scaled by its copy-kernel calibration.
 secondary = the same measurement on the 64-atom / 80^3 / 128-band cell.
 forces    = nonlocal energy and forces of the same block, every band occupied (time per call).
-kpoint    = H_k psi of the same cell at k = (pi/L)(1,1,1) on as many complex64 bands.
+kpoint    = H_k psi of the same cell at k = (pi/L)(1,1,1) on as many complex64 bands, and the
+            complex LOBPCG there (32 bands + 8 guard bands to residual 1e-4, preconditioned).
 cpu_baseline = the FP64 oracle (a C++ restatement of the reference's CPU loops) on all host
             cores (bands dealt over threads; one pass over a bounded sample of the same block),
             with the single-thread figure beside it.
@@ -250,9 +251,24 @@ def kpoint_leg(system, nb, steps, warmup):
     ms = e0.elapsed_time(e1) / steps
     bytes_step = 2 * nb * ng * 8 + ng * 4
     del u, out
+    # the complex LOBPCG at the same k point: 32 wanted bands (+8 guard bands), residual 1e-4,
+    # residuals preconditioned by (T_k + 0.5)^-1
+    nbl, nguard = 32, 8
+    rng = np.random.default_rng(20241205)
+    X = torch.from_numpy(rng.standard_normal((nbl + nguard, ng)) + 1j * rng.standard_normal((nbl + nguard, ng)))
+    X = X.cuda().contiguous()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eig, it, res = Hk.Eigen_z_multi(X, iter_max=200, tol=1e-4, nconv=nbl, precond_shift=0.5)
+    torch.cuda.synchronize()
+    t_lob = time.perf_counter() - t0
+    del X
     return {"workload": f"H_k psi, {len(s['positions'])} atoms, {s['n']}^3, {nb} complex64 bands, k = (pi/L)(1,1,1)",
             "ms_per_step": round(ms, 4), "GBps": round(bytes_step / (ms * 1e-3) / 1e9, 1),
-            "algorithmic_bytes_per_step": bytes_step}
+            "algorithmic_bytes_per_step": bytes_step,
+            "lobpcg": {"bands": nbl, "guard_bands": nguard, "tol": 1e-4, "precond": "(T_k + 0.5)^-1",
+                       "iterations": it, "seconds": round(t_lob, 3), "max_residual": res,
+                       "lowest_eigenvalues": [round(float(v), 8) for v in eig[:3]]}}
 
 
 def cpu_baseline(s, psi, nb_one, nb_all):
