@@ -193,7 +193,8 @@ int qg_hamiltonian_set_potential_device(qg_ham* h, const double* V_dev, void* st
  * coordinate; mCreateProjectorRadialOnBlock, PseudoPotOperator_kpoint_v2.h:287-420).  k in
  * bohr^-1 (kpoint_i * 2 pi / (L_i * samples_i) in the reference).  The handle keeps the
  * projector phase tables of h's nonlocal operator and rebuilds them when its geometry changes;
- * h must outlive it. */
+ * h must outlive it.  The handle also keeps its scratch (projector coefficients, FFT plans and
+ * spectrum, the solver's BLAS handle) between calls, so calls on one handle must not overlap. */
 typedef struct qg_kpoint qg_kpoint;
 int qg_kpoint_create(qg_ham* h, double kx, double ky, double kz, qg_kpoint** out);
 int qg_kpoint_destroy(qg_kpoint* kp);
