@@ -14,7 +14,7 @@
 //                                                          PseudoPotOperator_kpoint_v2.h:287-420
 //   project   mInnerNonlocalPsi_v2: c = dV sum P u (no conjugate)       :836-930
 //   apply     ProjectionPP: Hp += E conj(P) c                           :2581-2690
-//             (atoms coloured so that no two of one launch share a grid point: no atomics)
+//             (gathered per grid point over the projector entries that touch it: no atomics)
 //   solver    Eigen_z_multi_mpi (ZGEMM Grams, ZHEGVX)       lobpcg_z_multi_mpi.h:988-1600
 //             restated on an orthonormal basis: the Rayleigh-Ritz problem of [X W P] is a
 //             standard Hermitian one (zheevd), so no B-metric solve is needed
@@ -65,10 +65,10 @@ struct qg_kpoint {
     qg::DevBuf<double2> phase64;
     qg::DevBuf<float2> phase32;
     qg::DevBuf<double2> coef;   // projector coefficients [nb][ncoef]
-    // atoms grouped by colour: no two atoms of one colour share a grid point, so the
-    // application runs one launch per colour without atomics
-    std::vector<int> color_off;   // atoms of colour c: color_atoms[color_off[c] .. color_off[c+1])
-    qg::DevBuf<int> color_atoms;
+    // the projector table entries of every grid point (CSR over grid points): entries of point
+    // g are pt_ent[pt_off[g] .. pt_off[g+1]), each a table point and its atom's first coefficient
+    qg::DevBuf<int> pt_off, pt_ent, pt_cb;
+    qg::DevBuf<unsigned char> ec;   // E c per band, T2, rows of ncoef + 8 (zero padded)
     // kinetic preconditioner: plans per batch size, spectrum buffer, per-axis symbol parts
     std::map<int, std::unique_ptr<KFftPlan>> fft;
     qg::DevBuf<double2> spec;
@@ -325,65 +325,75 @@ __global__ __launch_bounds__(kProjBlock) void kproject_kernel(KProjArgs a, const
     }
 }
 
-// out_b(g(i)) += conj(phase(i)) sum_j beta_j(i) E_j c[b][j] for the atoms of one colour (one
-// workgroup per (atom, chunk of NBA bands)).  Atoms of one colour share no grid point and the
-// image blocks of an atom (periodic images, cutoff groups) are visited one after another with a
-// barrier between them (two blocks of one atom may hold the same grid point; one block holds
-// each point once), so plain read-modify-write suffices.
-template <class T, int NBA, int NJS>
-__global__ __launch_bounds__(kProjBlock) void kapply_kernel(KProjArgs a, const int* __restrict__ atoms, int natoms_c,
-                                                             typename Cx<T>::t* __restrict__ out, int64_t ld_out,
-                                                             int nitems) {
-    using C = typename Cx<T>::t;
-    __shared__ T ecr[NJS][NBA], eci[NJS][NBA];
-    const int item = xcd_item(blockIdx.x, gridDim.x);
-    if (item >= nitems) return;   // padding of the remap: the whole workgroup leaves
-    const int atom = atoms[item % natoms_c];
-    const int chunk = item / natoms_c;
-    const int b0 = chunk * NBA;
-    const int tid = threadIdx.x;
-    const int cbase = a.atom_coef_off[atom];
-    const int count = a.atom_coef_off[atom + 1] - cbase;
-    if (tid < NJS * NBA) {
-        const int j = tid / NBA, b = tid % NBA;
-        T vr = T(0), vi = T(0);
-        if (j < count && b0 + b < a.nbands) {
-            const double2 cc = a.coef[(int64_t)(b0 + b) * a.ncoef + cbase + j];
-            const double e = a.energy[cbase + j];
-            vr = (T)(e * cc.x);
-            vi = (T)(e * cc.y);
+// ec[b][j] = E_j c[b][j] in the states' precision, rows of ld_ec = ncoef + 8 with zero padding
+template <class T>
+__global__ void kscale_coef_kernel(const double2* __restrict__ c, const double* __restrict__ energy, int ncoef,
+                                   int ld_ec, int nbands, typename Cx<T>::t* __restrict__ ec) {
+    const int64_t n = (int64_t)nbands * ld_ec;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        const int b = (int)(t / ld_ec), j = (int)(t - (int64_t)b * ld_ec);
+        typename Cx<T>::t v{T(0), T(0)};
+        if (j < ncoef) {
+            const double2 cc = c[(int64_t)b * ncoef + j];
+            v.x = (T)(energy[j] * cc.x);
+            v.y = (T)(energy[j] * cc.y);
         }
-        ecr[j][b] = vr;
-        eci[j][b] = vi;
+        ec[t] = v;
     }
-    __syncthreads();
-    const T* beta = reinterpret_cast<const T*>(a.beta);
-    const C* ph = reinterpret_cast<const C*>(a.phase);
-    const int nb = min(NBA, a.nbands - b0);
-    T* o = reinterpret_cast<T*>(out);
-    for (int im = a.atom_img_off[atom]; im < a.atom_img_off[atom + 1]; ++im) {
-        const int64_t t0 = a.img_tbl_off[im], t1 = a.img_tbl_off[im + 1];
-        for (int64_t i = t0 + tid; i < t1; i += kProjBlock) {
-            const int g = a.gidx[i];
-            const C e = ph[i];
-            T bj[NJS];
+}
+
+// out_b(g) += sum over the table entries of g of conj(phase) sum_j beta_j E_j c[b][cb + j]: one
+// thread per grid point and chunk of NB bands, one read-modify-write of the output per point and
+// band (the projector points of all atoms that touch g are gathered, so no two threads write one
+// value).  Slots j past an atom's coefficients read the next atom's (or the padding's) finite
+// values against beta_j = 0.
+template <class T, int NB, int NJS>
+__global__ __launch_bounds__(256) void kgather_kernel(const int* __restrict__ pt_off, const int* __restrict__ pt_ent,
+                                                       const int* __restrict__ pt_cb, const T* __restrict__ beta,
+                                                       const typename Cx<T>::t* __restrict__ phase,
+                                                       const typename Cx<T>::t* __restrict__ ec, int ld_ec, int nbands,
+                                                       int64_t ng, typename Cx<T>::t* __restrict__ out,
+                                                       int64_t ld_out) {
+    using C = typename Cx<T>::t;
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (g >= ng) return;
+    const int e0 = pt_off[g], e1 = pt_off[g + 1];
+    if (e0 == e1) return;
+    const int b0 = blockIdx.y * NB;
+    const int nb = min(NB, nbands - b0);
+    T ar[NB], ai[NB];
 #pragma unroll
-            for (int j = 0; j < NJS; ++j) bj[j] = beta[i * NJS + j];
+    for (int b = 0; b < NB; ++b) ar[b] = ai[b] = T(0);
+    for (int e = e0; e < e1; ++e) {
+        const int t = pt_ent[e];
+        const int cb = pt_cb[e];
+        const C ph = phase[t];
+        T bj[NJS];
 #pragma unroll
-            for (int b = 0; b < NBA; ++b) {
-                if (b >= nb) break;
-                T vr = T(0), vi = T(0);
+        for (int j = 0; j < NJS; ++j) bj[j] = beta[(int64_t)t * NJS + j];
 #pragma unroll
-                for (int j = 0; j < NJS; ++j) {
-                    vr += bj[j] * ecr[j][b];
-                    vi += bj[j] * eci[j][b];
-                }
-                const int64_t off = 2 * ((int64_t)(b0 + b) * ld_out + g);
-                o[off] += e.x * vr + e.y * vi;
-                o[off + 1] += e.x * vi - e.y * vr;
+        for (int b = 0; b < NB; ++b) {
+            if (b >= nb) break;
+            const C* row = ec + (int64_t)(b0 + b) * ld_ec + cb;
+            T vr = T(0), vi = T(0);
+#pragma unroll
+            for (int j = 0; j < NJS; ++j) {
+                const C w = row[j];
+                vr += bj[j] * w.x;
+                vi += bj[j] * w.y;
             }
+            ar[b] += ph.x * vr + ph.y * vi;
+            ai[b] += ph.x * vi - ph.y * vr;
         }
-        __syncthreads();   // the next block of this atom may hold the same grid points
+    }
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        if (b >= nb) break;
+        C* o = out + (int64_t)(b0 + b) * ld_out + g;
+        C v = *o;
+        v.x += ar[b];
+        v.y += ai[b];
+        *o = v;
     }
 }
 
@@ -419,30 +429,25 @@ void build_phases(qg_kpoint* kp) {
         kp->phase64.upload(p64.data(), p64.size());
         kp->phase32.upload(p32.data(), p32.size());
     }
-    // greedy colouring of the atoms: the colours of the atoms already placed on an atom's grid
-    // points (one bit per colour and point) are excluded, the lowest free colour is taken
-    const int natoms = geo.natoms;
-    std::vector<uint64_t> used((size_t)g.ng(), 0);
-    std::vector<int> color(natoms, 0);
-    int ncolor = 0;
-    for (int at = 0; at < natoms; ++at) {
-        const int64_t t0 = geo.img_tbl_off[geo.atom_img_off[at]], t1 = geo.img_tbl_off[geo.atom_img_off[at + 1]];
-        uint64_t mask = 0;
-        for (int64_t t = t0; t < t1; ++t) mask |= used[geo.gidx[t]];
-        int c = 0;
-        while (c < 64 && (mask >> c) & 1u) ++c;
-        QG_REQUIRE(c < 64, "k-point projectors: more than 64 atom colours");
-        color[at] = c;
-        ncolor = std::max(ncolor, c + 1);
-        for (int64_t t = t0; t < t1; ++t) used[geo.gidx[t]] |= (uint64_t)1 << c;
+    // the entries of every grid point (CSR), with their atom's first coefficient
+    const int64_t ngp = g.ng();
+    std::vector<int> off((size_t)ngp + 1, 0), ent((size_t)np), cbv((size_t)np);
+    for (int64_t t = 0; t < np; ++t) ++off[(size_t)geo.gidx[t] + 1];
+    for (int64_t i = 0; i < ngp; ++i) off[(size_t)i + 1] += off[(size_t)i];
+    std::vector<int> fill(off.begin(), off.end() - 1);
+    for (size_t im = 0; im + 1 < geo.img_tbl_off.size(); ++im) {
+        const int cb = geo.atom_coef_off[geo.images[im].atom];
+        for (int64_t t = geo.img_tbl_off[im]; t < geo.img_tbl_off[im + 1]; ++t) {
+            const int pos = fill[geo.gidx[t]]++;
+            ent[pos] = (int)t;
+            cbv[pos] = cb;
+        }
     }
-    kp->color_off.assign(ncolor + 1, 0);
-    for (int at = 0; at < natoms; ++at) ++kp->color_off[color[at] + 1];
-    for (int c = 0; c < ncolor; ++c) kp->color_off[c + 1] += kp->color_off[c];
-    std::vector<int> order(natoms);
-    std::vector<int> fill(kp->color_off.begin(), kp->color_off.end() - 1);
-    for (int at = 0; at < natoms; ++at) order[fill[color[at]]++] = at;
-    if (natoms) kp->color_atoms.upload(order.data(), order.size());
+    kp->pt_off.upload(off.data(), off.size());
+    if (np) {
+        kp->pt_ent.upload(ent.data(), ent.size());
+        kp->pt_cb.upload(cbv.data(), cbv.size());
+    }
     kp->geo_version = nl->geo_version;
 }
 
@@ -478,20 +483,26 @@ void launch_nl(qg_kpoint* kp, int nbands, const typename Cx<T>::t* psi, int64_t 
         hipLaunchKernelGGL((kproject_kernel<T, kNBA<T>, 8>), dim3(padded(items)), dim3(kProjBlock), 0, s, a, psi,
                            (int)items);
     QG_HIP(hipGetLastError());
-    // the application, one launch per atom colour
-    for (size_t c = 0; c + 1 < kp->color_off.size(); ++c) {
-        const int na = kp->color_off[c + 1] - kp->color_off[c];
-        if (na == 0) continue;
-        const int* atoms = kp->color_atoms.p + kp->color_off[c];
-        const int64_t ci = (int64_t)na * chunks;
-        if (geo.njs == 4)
-            hipLaunchKernelGGL((kapply_kernel<T, kNBA<T>, 4>), dim3(padded(ci)), dim3(kProjBlock), 0, s, a, atoms, na,
-                               out, ld_out, (int)ci);
-        else
-            hipLaunchKernelGGL((kapply_kernel<T, kNBA<T>, 8>), dim3(padded(ci)), dim3(kProjBlock), 0, s, a, atoms, na,
-                               out, ld_out, (int)ci);
-        QG_HIP(hipGetLastError());
-    }
+    // the application: E c per band (padded rows), then one gather launch
+    const int ld_ec = geo.ncoef + 8;
+    using C = typename Cx<T>::t;
+    kp->ec.alloc((size_t)nbands * ld_ec * sizeof(C));
+    C* ec = reinterpret_cast<C*>(kp->ec.p);
+    hipLaunchKernelGGL(kscale_coef_kernel<T>, dim3((unsigned)std::min<int64_t>(((int64_t)nbands * ld_ec + 255) / 256, 4096)),
+                       dim3(256), 0, s, (const double2*)kp->coef.p, nl->d_coef_energy.p, geo.ncoef, ld_ec, nbands, ec);
+    const int64_t ng = nl->gcopy.ng();
+    constexpr int NB = sizeof(T) == 4 ? 8 : 4;
+    const dim3 grid((unsigned)((ng + 255) / 256), (unsigned)((nbands + NB - 1) / NB));
+    QG_REQUIRE(grid.y <= 65535, "too many bands in one k-point application");
+    const T* beta = sizeof(T) == 4 ? (const T*)nl->d_beta32.p : (const T*)nl->d_beta64.p;
+    const C* ph = sizeof(T) == 4 ? (const C*)kp->phase32.p : (const C*)kp->phase64.p;
+    if (geo.njs == 4)
+        hipLaunchKernelGGL((kgather_kernel<T, NB, 4>), grid, dim3(256), 0, s, kp->pt_off.p, kp->pt_ent.p, kp->pt_cb.p,
+                           beta, ph, ec, ld_ec, nbands, ng, out, ld_out);
+    else
+        hipLaunchKernelGGL((kgather_kernel<T, NB, 8>), grid, dim3(256), 0, s, kp->pt_off.p, kp->pt_ent.p, kp->pt_cb.p,
+                           beta, ph, ec, ld_ec, nbands, ng, out, ld_out);
+    QG_HIP(hipGetLastError());
 }
 
 template <class T>
