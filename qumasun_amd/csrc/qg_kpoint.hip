@@ -85,7 +85,7 @@ namespace qg {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kTX = 64, kTY = 4;
+constexpr int kTX = 32, kTY = 8;
 constexpr int kHX = kTX + 8, kHY = kTY + 8;   // the tile with its 4-point halo
 constexpr int kHalo = kHX * kHY;
 constexpr int kLoadsPT = (kHalo + kThreads - 1) / kThreads;
